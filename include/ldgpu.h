@@ -1,0 +1,193 @@
+/* ldgpu.h -- C ABI of the MI355X RF demodulation + TBC library (libldgpu.so).
+ *
+ * Drop-in GPU path for two call sites of ld-decode (Python; no FFI exists in
+ * the reference, so the boundary mirrors its Python interfaces):
+ *   - RFDecode.demodblock(data, mtf_level, fftdata, cut)  lddecode/core.py:747-879
+ *     together with the DemodCache block stitching        lddecode/core.py:1204-1413
+ *   - Field.downscale -> computewow_scaled -> scale_field -> hz_to_output
+ *                                        lddecode/core.py:2591-2722, utils.py:127-183,
+ *                                        utils.py:1061-1075
+ *   - Field.refine_linelocs_hsync        lddecode/core.py:2484-2551
+ *
+ * Conventions
+ *   - Every function returns 0 on success and a negative LD_E* code on
+ *     failure; nothing throws across the ABI.  ld_last_error() returns the
+ *     message of the calling thread's most recent failure.
+ *   - Sample buffers are caller-owned DEVICE pointers (e.g. torch tensors'
+ *     data_ptr()); filter tables and line locations passed as `host_*` are host
+ *     pointers.  `stream` is a hipStream_t (NULL = default stream).
+ *   - Handles are not internally locked: use one handle per host thread (the
+ *     reference gives each DemodCache worker thread its own RFDecode,
+ *     core.py:1165).  Calls only enqueue work on `stream`.
+ */
+#ifndef LDGPU_H
+#define LDGPU_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LDGPU_ABI_VERSION 1
+
+enum {
+  LD_OK = 0,
+  LD_EINVAL = -1,   /* bad argument (null pointer, size, id) */
+  LD_ERANGE = -2,   /* a block or tap would read outside the input buffer */
+  LD_EHIP = -3,     /* HIP runtime error */
+  LD_ESTATE = -4    /* required tables not uploaded yet */
+};
+
+/* Filter table ids (RFDecode.Filters names, core.py:544-679). */
+enum {
+  LD_FILT_RFHPF = 0,    /* Frfhpf                                  core.py:554-555 */
+  LD_FILT_RFVIDEO = 1,  /* RFVideo (analytic mask folded) * MTF^lvl core.py:569-612, :800-803 */
+  LD_FILT_VIDEO = 2,    /* FVideo (LPF * deemph^s * GD all-pass)   core.py:632-643 */
+  LD_FILT_VIDEO05 = 3,  /* FVideo05 (LPF * deemph * firwin65)      core.py:648-652 */
+  LD_FILT_BURST = 4,    /* FVideoBurst (LPF * deemph * firwin81)   core.py:654-658 */
+  LD_FILT_PILOT = 5,    /* FVideoPilot, PAL only                   core.py:660-669 */
+  LD_NFILT = 6
+};
+
+enum { LD_IN_INT16 = 0, LD_IN_FLOAT32 = 1 };
+
+typedef struct ld_handle ld_handle;
+
+/* Library ABI version (LDGPU_ABI_VERSION). */
+int ld_abi_version(void);
+
+/* Message of the calling thread's last failure ("" if none). */
+const char* ld_last_error(void);
+
+/* Create a handle on HIP device `device` for blocks of `blocklen` samples
+ * (RFDecode(blocklen=...), core.py:286; 32768 supported).  Uploads the FFT
+ * twiddle table.
+ * Replaces RFDecode.__init__'s table set-up, core.py:284-403. */
+int ld_create(int device, int blocklen, ld_handle** out);
+void ld_destroy(ld_handle* h);
+
+/* Upload one filter as its half spectrum, bins 0..blocklen/2, interleaved
+ * (re, im) float64 pairs; stored as float32 after multiplying by `scale`.
+ * `dc_gain` is the filter's true bin-0 response (added back as dc_gain times
+ * the DC offset removed before the second FFT); `shift` is the output roll in
+ * samples (np.roll(x, -shift), core.py:814, :817).
+ * Replaces the table build of RFDecode.computefilters, core.py:405-420. */
+int ld_set_filter(ld_handle* h, int filter_id, const double* host_reim, int nbins,
+                  double scale, double dc_gain, int shift);
+
+/* Audio stage-1 channel (0 = left, 1 = right): `nbins`-bin sliced filter
+ * filt1 (core.py:716-718) as interleaved float64 pairs, the slice's low bin,
+ * its decimated rate a1_freq and the frequency of its low bin (core.py:707-714). */
+int ld_set_audio(ld_handle* h, int channel, const double* host_reim, int nbins, int lowbin,
+                 double a1_freq, double low_freq);
+
+/* Scalar demodulation parameters: sample rate (Hz), the clip applied to
+ * demod_raw before the second FFT (core.py:809), the constant subtracted from
+ * the clipped signal before that FFT (float32 range protection; added back
+ * through each filter's dc_gain), and video_rot (core.py:765-772, :1032-1034). */
+int ld_set_demod_params(ld_handle* h, double fs_hz, double clip_lo, double clip_hi,
+                        double dc_offset, int video_rot);
+
+/* Output channels of ld_demod_blocks.  A NULL pointer skips that channel.
+ * Block b writes its samples at ptr + b * video_stride (video and rfhpf) or
+ * ptr + b * audio_stride (audio). */
+typedef struct ld_demod_outputs {
+  float* rfhpf;
+  float* demod;
+  float* demod_raw;
+  float* demod_05;
+  float* demod_burst;
+  float* demod_pilot;
+  float* audio_left;
+  float* audio_right;
+  int64_t video_stride;
+  int64_t audio_stride;
+} ld_demod_outputs;
+
+/* Demodulate `nblocks` blocks; block b reads input[first + b*step .. + blocklen).
+ * cut != 0: video outputs hold samples [1024, blocklen-32) of each block
+ * (blocklen-1056 values), audio [32, nbins-1) (RFDecode.demodblock cut=True,
+ * core.py:839-875); with step = blocklen-1056 and strides equal to the cut
+ * lengths this is DemodCache's stitched field buffer (core.py:1392-1403).
+ * cut == 0: full blocklen / nbins outputs.  rfhpf is always cut
+ * [1024-video_rot, blocklen-32-video_rot) (core.py:769-772).
+ * `workspace` is device memory of at least ld_demod_workspace_bytes(h, nblocks).
+ * Replaces RFDecode.demodblock(data=..., mtf_level, cut), core.py:747-879 (the
+ * MTF level is folded into LD_FILT_RFVIDEO by the host). */
+size_t ld_demod_workspace_bytes(const ld_handle* h, int nblocks);
+int ld_demod_blocks(ld_handle* h, const void* input, int input_dtype, int64_t input_len,
+                    int64_t first, int64_t step, int nblocks, int cut,
+                    const ld_demod_outputs* out, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
+/* Upload the 16-tap sinc interpolation table: (65536 + 1) rows x 16 float32
+ * (row p = Kaiser(beta=5)-windowed sinc weights at phase p/65536; the extra
+ * row repeats the last), as lddecode/sinc_lut.npz (utils.py:73-124).
+ * Replaces the np.load of core.py:314-318. */
+int ld_set_sinc_lut(ld_handle* h, const float* host_lut, int rows, int taps);
+
+/* Per-field TBC description for ld_tbc_fields (a HOST array; the library
+ * validates it and copies it into the workspace).  Sample positions in the
+ * line locations are relative to `demod + demod_offset`. */
+typedef struct ld_tbc_field {
+  int64_t demod_offset;   /* first sample of this field's demod buffer */
+  int64_t demod_len;      /* samples readable from there (taps outside read 0) */
+  int64_t lineloc_offset; /* first entry of this field in the linelocs array */
+  int32_t nlines;         /* line locations (proclines), 2..1024 */
+  int32_t lineoffset;     /* Field.lineoffset (core.py:1614, :3231) */
+  int32_t linesout;       /* output lines (outlinecount, core.py:1624) */
+  int32_t outwidth;       /* output pixels per line (outlinelen) */
+  double inlinelen;       /* Field.inlinelen = RFDecode.linelen (core.py:1609) */
+  int64_t out_offset;     /* first output pixel of this field */
+} ld_tbc_field;
+
+/* Output level model of hz_to_output_array (utils.py:1061-1075). */
+typedef struct ld_level_params {
+  double ire0, hz_ire, output_zero, vsync_ire, out_scale;
+} ld_level_params;
+
+/* Time-base correct `nfields` fields: per output pixel, FP64 source position
+ * and wow from the piecewise-linear line map (computewow_scaled,
+ * core.py:2591-2623), wow level adjust with median/MAD outlier clamp
+ * (utils.py:131-149), 16-tap sinc-LUT interpolation (utils.py:151-183) and,
+ * when out_u16 is non-NULL, the Hz -> uint16 code map (utils.py:1061-1075).
+ * coord_f32 != 0 reproduces the reference's float32 cast of the source
+ * coordinate (utils.py:160); 0 keeps FP64.  smoothing > 0 applies the
+ * per-pixel EMA of utils.py:142-149.  Taps outside [0, demod_len) read 0.
+ * `demod` (demod_total samples) and `linelocs` (nlinelocs doubles) are DEVICE
+ * arrays; out_f32 (may be NULL) receives the level-adjusted Hz values
+ * (scale_field's dsout), out_u16 (may be NULL) the output codes; both hold
+ * out_total elements.  `workspace` is device memory of at least
+ * ld_tbc_workspace_bytes(h, nfields).
+ * Replaces Field.downscale's video path, core.py:2626-2722. */
+size_t ld_tbc_workspace_bytes(const ld_handle* h, int nfields);
+int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const ld_tbc_field* fields,
+                  int nfields, const double* linelocs, int64_t nlinelocs, int coord_f32,
+                  double smoothing, double threshold, const ld_level_params* level,
+                  float* out_f32, uint16_t* out_u16, int64_t out_total, void* workspace,
+                  size_t workspace_bytes, void* stream);
+
+/* Sub-sample hsync refinement of every field's line locations on demod_05
+ * (Field.refine_linelocs_hsync, core.py:2484-2551, with calczc
+ * utils.py:795-854), one launch for the batch.  `fields` is a HOST array (only
+ * demod_offset, demod_len, lineloc_offset and nlines are used); line
+ * locations of field f are entries [lineloc_offset, +nlines) of the DEVICE
+ * arrays linelocs1 / linebad_in (may be NULL) / linelocs2 / linebad_out, in
+ * samples relative to demod_05 + demod_offset.  `freq` is samples per
+ * microsecond; levels are Hz (iretohz of vsync_ire/2, -55 and 30 IRE); `pal`
+ * skips lines 1-2 as well as 3-6.  Reads outside a field's span count as
+ * "not found" (the reference would index negatively or raise).
+ * `workspace`: device memory of ld_hsync_workspace_bytes(h, nfields). */
+size_t ld_hsync_workspace_bytes(const ld_handle* h, int nfields);
+int ld_refine_hsync(ld_handle* h, const float* demod_05, int64_t demod_total,
+                    const ld_tbc_field* fields, int nfields, const double* linelocs1,
+                    const uint8_t* linebad_in, int64_t nlinelocs, double freq, double sync_mid_hz,
+                    double lo_hz, double hi_hz, int pal, double* linelocs2, uint8_t* linebad_out,
+                    void* workspace, size_t workspace_bytes, void* stream);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* LDGPU_H */

@@ -1,0 +1,124 @@
+"""ctypes binding of libldgpu.so (declarations: include/ldgpu.h).
+
+The library is loaded from this package directory.  There is no fallback: if
+it is missing, every entry point raises :class:`LdGpuUnavailable`.
+"""
+
+import ctypes
+import os
+import pathlib
+import threading
+
+LIB_PATH = pathlib.Path(__file__).resolve().with_name("libldgpu.so")
+
+LD_OK, LD_EINVAL, LD_ERANGE, LD_EHIP, LD_ESTATE = 0, -1, -2, -3, -4
+LD_FILT_RFHPF, LD_FILT_RFVIDEO, LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT = range(6)
+LD_IN_INT16, LD_IN_FLOAT32 = 0, 1
+ABI_VERSION = 1
+
+# every symbol include/ldgpu.h declares
+EXPORTS = (
+    "ld_abi_version", "ld_last_error", "ld_create", "ld_destroy", "ld_set_filter", "ld_set_audio",
+    "ld_set_demod_params", "ld_demod_workspace_bytes", "ld_demod_blocks", "ld_set_sinc_lut",
+    "ld_tbc_workspace_bytes", "ld_tbc_fields", "ld_hsync_workspace_bytes", "ld_refine_hsync",
+)
+
+
+class LdGpuUnavailable(RuntimeError):
+    """libldgpu.so is missing or could not be loaded."""
+
+
+class LdGpuError(RuntimeError):
+    """A libldgpu call returned an error code."""
+
+    def __init__(self, code, message):
+        super().__init__(f"libldgpu error {code}: {message}")
+        self.code = code
+
+
+class DemodOutputs(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_void_p) for n in (
+        "rfhpf", "demod", "demod_raw", "demod_05", "demod_burst", "demod_pilot",
+        "audio_left", "audio_right")] + [("video_stride", ctypes.c_int64), ("audio_stride", ctypes.c_int64)]
+
+
+class TbcField(ctypes.Structure):
+    _fields_ = [
+        ("demod_offset", ctypes.c_int64),
+        ("demod_len", ctypes.c_int64),
+        ("lineloc_offset", ctypes.c_int64),
+        ("nlines", ctypes.c_int32),
+        ("lineoffset", ctypes.c_int32),
+        ("linesout", ctypes.c_int32),
+        ("outwidth", ctypes.c_int32),
+        ("inlinelen", ctypes.c_double),
+        ("out_offset", ctypes.c_int64),
+    ]
+
+
+class LevelParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("ire0", "hz_ire", "output_zero", "vsync_ire", "out_scale")]
+
+
+_lib = None
+_lock = threading.Lock()
+
+
+def _declare(lib):
+    c_int, c_i64, c_vp, c_dbl, c_sz = ctypes.c_int, ctypes.c_int64, ctypes.c_void_p, ctypes.c_double, ctypes.c_size_t
+    P = ctypes.POINTER
+    sig = {
+        "ld_abi_version": (c_int, []),
+        "ld_last_error": (ctypes.c_char_p, []),
+        "ld_create": (c_int, [c_int, c_int, P(c_vp)]),
+        "ld_destroy": (None, [c_vp]),
+        "ld_set_filter": (c_int, [c_vp, c_int, P(c_dbl), c_int, c_dbl, c_dbl, c_int]),
+        "ld_set_audio": (c_int, [c_vp, c_int, P(c_dbl), c_int, c_int, c_dbl, c_dbl]),
+        "ld_set_demod_params": (c_int, [c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_int]),
+        "ld_demod_workspace_bytes": (c_sz, [c_vp, c_int]),
+        "ld_demod_blocks": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
+                                    P(DemodOutputs), c_vp, c_sz, c_vp]),
+        "ld_set_sinc_lut": (c_int, [c_vp, P(ctypes.c_float), c_int, c_int]),
+        "ld_tbc_workspace_bytes": (c_sz, [c_vp, c_int]),
+        "ld_tbc_fields": (c_int, [c_vp, c_vp, c_i64, P(TbcField), c_int, c_vp, c_i64, c_int, c_dbl, c_dbl,
+                                  P(LevelParams), c_vp, c_vp, c_i64, c_vp, c_sz, c_vp]),
+        "ld_hsync_workspace_bytes": (c_sz, [c_vp, c_int]),
+        "ld_refine_hsync": (c_int, [c_vp, c_vp, c_i64, P(TbcField), c_int, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_dbl,
+                                    c_dbl, c_int, c_vp, c_vp, c_vp, c_sz, c_vp]),
+    }
+    for name, (res, args) in sig.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+def load(path=None):
+    """Load (once) and return the ctypes library."""
+    global _lib
+    with _lock:
+        if _lib is not None and path is None:
+            return _lib
+        p = pathlib.Path(path) if path else LIB_PATH
+        if not p.exists():
+            raise LdGpuUnavailable(
+                f"{p} not found: build it with `python -m lddecode_amd.build` "
+                "(there is no CPU fallback)")
+        try:
+            lib = ctypes.CDLL(os.fspath(p))
+        except OSError as e:
+            raise LdGpuUnavailable(f"cannot load {p}: {e}") from e
+        _declare(lib)
+        if lib.ld_abi_version() != ABI_VERSION:
+            raise LdGpuUnavailable(f"{p}: ABI version {lib.ld_abi_version()} != {ABI_VERSION}")
+        if path is None:
+            _lib = lib
+        return lib
+
+
+def check(rc):
+    """Raise LdGpuError for a non-zero return code."""
+    if rc != LD_OK:
+        msg = load().ld_last_error()
+        raise LdGpuError(rc, msg.decode() if msg else "")
+    return rc

@@ -1,0 +1,127 @@
+"""Whole-capture GPU decode: the batched demod + hsync + TBC path.
+
+A capture resident in device memory is demodulated block by block into
+stitched buffers (DemodCache.read, core.py:1344-1413, without the LRU or
+threads: every block is demodulated exactly once in one set of launches).
+Fields are then described by their buffer origin and line locations, as
+``LDdecode.readfield`` / ``Field`` see them (core.py:4100-4199): every field's
+hsync refinement runs in one launch and every field's TBC in one more.
+"""
+
+import numpy as np
+
+from . import field as fieldmod
+
+
+def field_origin(rf, begin):
+    """Stitched-buffer index of a field read starting at raw sample `begin`:
+    DemodCache.read returns blocks from begin // S, so the field's buffer
+    starts at raw (begin // S) * S + blockcut, stitched index (begin // S) * S
+    (core.py:1371, :1409)."""
+    S = rf.block_stride
+    return (begin // S) * S
+
+
+class CaptureDecoder:
+    """Pre-allocated demod + hsync + TBC of `fields` over one capture.
+
+    data: int16 (or float32) device tensor holding the capture.
+    fields: list of dicts with demod_offset (stitched index of the field
+    buffer origin), demod_len, and nlines; linelocs1: per-field line
+    locations relative to each origin, concatenated (float64, host or device).
+    """
+
+    def __init__(self, rf, data, fields, linelocs1, lineoffset=0, coord_f32=True, smoothing=0.0, audio=False):
+        torch = rf.torch
+        self.rf = rf
+        self.data = data
+        N, S = rf.blocklen, rf.block_stride
+        self.nblocks = (data.numel() - N) // S + 1
+        self.total = self.nblocks * S
+        dev = rf.device
+        names = list(rf.video_fields) + ["rfhpf"]
+        self.buffers = {n: torch.empty(self.total, dtype=torch.float32, device=dev) for n in names}
+        self.audio = bool(audio)
+        if self.audio:
+            na = rf.filterbank.audio["left"]["nbins"]
+            alen = na - rf.blockcut // (N // na) - (-(-32 // (N // na)))
+            for n in ("audio_left", "audio_right"):
+                self.buffers[n] = torch.empty(self.nblocks * alen, dtype=torch.float32, device=dev)
+        W = rf.SysParams["outlinelen"]
+        linesout = rf.SysParams["frame_lines"] // 2 + 1
+        self.fields = []
+        off = 0
+        lo = 0
+        for f in fields:
+            fd = dict(f)
+            fd.setdefault("lineoffset", lineoffset)
+            fd.setdefault("linesout", linesout)
+            fd.setdefault("outwidth", W)
+            fd.setdefault("inlinelen", float(rf.linelen))
+            fd["lineloc_offset"] = lo
+            fd["out_offset"] = off
+            fd["demod_len"] = min(fd.get("demod_len", self.total), self.total - fd["demod_offset"])
+            lo += fd["nlines"]
+            off += fd["linesout"] * fd["outwidth"]
+            self.fields.append(fd)
+        self.npixels = off
+        ll = linelocs1 if isinstance(linelocs1, torch.Tensor) else torch.from_numpy(
+            np.ascontiguousarray(linelocs1, dtype=np.float64))
+        self.linelocs1 = ll.to(dev, torch.float64).contiguous()
+        assert self.linelocs1.numel() == lo, "line count mismatch"
+        self.linelocs2 = torch.empty_like(self.linelocs1)
+        self.linebad = torch.empty(lo, dtype=torch.uint8, device=dev)
+        self.out_u16 = torch.empty(self.npixels, dtype=torch.uint16, device=dev)
+        self.hs_ws = torch.empty(max(1, rf.lib.ld_hsync_workspace_bytes(rf.handle, len(self.fields))),
+                                 dtype=torch.uint8, device=dev)
+        self.tbc_ws = torch.empty(rf.lib.ld_tbc_workspace_bytes(rf.handle, len(self.fields)),
+                                  dtype=torch.uint8, device=dev)
+        rf._workspace(self.nblocks)
+        self.coord_f32 = coord_f32
+        self.smoothing = smoothing
+
+    def demod(self, mtf_level=0):
+        return self.rf.demod_blocks(self.data, 0, self.nblocks, mtf_level=mtf_level, cut=True, audio=self.audio,
+                                    out=self.buffers)
+
+    def refine(self):
+        return fieldmod.refine_linelocs_hsync_fields(
+            self.rf, self.buffers["demod_05"], self.fields, self.linelocs1, out=(self.linelocs2, self.linebad),
+            workspace=self.hs_ws)
+
+    def tbc(self):
+        return fieldmod.tbc_fields(self.rf, self.buffers["demod"], self.fields, self.linelocs2,
+                                   coord_f32=self.coord_f32, smoothing=self.smoothing, out_u16=self.out_u16,
+                                   final=True, workspace=self.tbc_ws)
+
+    def step(self, mtf_level=0):
+        """One pass of the hot path over the whole capture (no host sync)."""
+        self.demod(mtf_level)
+        self.refine()
+        self.tbc()
+        return self.out_u16
+
+
+def truth_fields(rf, cap, nfields, first_field=0, margin_lines=2, delay=13.0):
+    """Field plan from a synthetic capture's ground truth: lines sit on the
+    field's video-line grid (half-line f*hpf + first + 2*(l - L0), with line
+    1 the first line starting in the field's first two half-lines); the field
+    buffer begins at the block holding the start of line -margin_lines;
+    linelocs1 = rounded true starts + the typical filter delay of demod_05's
+    sync edge, relative to the buffer origin."""
+    from .synth import _field_layout
+    hpf, _, _, first = _field_layout(rf.system)
+    l0 = first // 2 + 1
+    proclines = rf.SysParams["frame_lines"] // 2 + 1 + 10 + (3 if rf.system == "PAL" else 0)
+    hs = cap.halfline_starts
+    fields, lls = [], []
+    for f in range(first_field, first_field + nfields):
+        idx = f * hpf + first + 2 * (np.arange(proclines) - l0)
+        if idx[0] < 2 * margin_lines or idx[-1] >= len(hs):
+            raise ValueError(f"field {f} does not fit the capture")
+        raw = hs[idx]
+        begin = int(hs[idx[0] - 2 * margin_lines]) - rf.blockcut
+        org = field_origin(rf, max(begin, 0))
+        fields.append(dict(demod_offset=org, nlines=proclines))
+        lls.append(np.round(raw - (org + rf.blockcut) + delay))
+    return fields, np.concatenate(lls)

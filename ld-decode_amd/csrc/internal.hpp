@@ -1,0 +1,44 @@
+// Handle layout and error plumbing shared by the library's translation units.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdarg.h>
+#include <stdio.h>
+
+#include "../../include/ldgpu.h"
+
+struct ld_handle {
+  int device = 0;
+  int blocklen = 0;          // N (real samples per block)
+  int cut_begin = 1024;      // RFDecode.blockcut        core.py:321
+  int cut_end = 32;          // RFDecode.blockcut_end    core.py:402 (F05_offset)
+  float2* twiddle = nullptr; // exp(-2*pi*i*t/N), t < N
+  float* sinc_lut = nullptr; // (65536+1) x 16 float32
+  float2* filt[LD_NFILT] = {};
+  double filt_dc[LD_NFILT] = {};
+  int filt_shift[LD_NFILT] = {};
+  float2* audio_filt[2] = {};
+  int audio_nbins = 0;
+  int audio_lowbin[2] = {};
+  double audio_rate[2] = {};
+  double audio_low_freq[2] = {};
+  double fs = 40e6;
+  double clip_lo = 1.5e6, clip_hi = 30e6, dc_offset = 0.0;
+  int video_rot = 0;
+  bool params_set = false;
+};
+
+namespace ldgpu {
+
+void set_error(const char* fmt, ...);
+int hip_fail(hipError_t e, const char* what);
+
+#define LD_HIP(call)                                    \
+  do {                                                  \
+    hipError_t e_ = (call);                             \
+    if (e_ != hipSuccess) return ldgpu::hip_fail(e_, #call); \
+  } while (0)
+
+constexpr int kSincTaps = 16;
+constexpr int kSincPhases = 1 << 16;
+
+}  // namespace ldgpu

@@ -1,0 +1,208 @@
+"""GPU field resampler (TBC) and FP64 hsync refinement: the host mirror of
+ld-decode's ``Field`` video path (core.py:1587-2722).
+
+``Field`` keeps the reference's attribute names (``linelocs``, ``lineoffset``,
+``inlinelen``, ``outlinelen``, ``outlinecount``, ``out_scale``) and
+``downscale(...) -> (dsout, dsaudio, efmout)``; ``tbc_fields`` is the batched
+form that resamples many fields in one launch.  Line locations come from the
+caller or from ``refine_linelocs_hsync`` (GPU, FP64).
+"""
+
+import ctypes
+
+import numpy as np
+from scipy.special import i0
+
+from . import _lib
+from .params import out_scale as _out_scale
+
+SINC_TAPS = 16
+SINC_PHASES = 1 << 16
+
+
+def build_sinc_lut(beta=5.0, taps=SINC_TAPS, phases=SINC_PHASES):
+    """(phases + 1) x taps float32 Kaiser-windowed sinc table (utils.py:73-124):
+    row p holds the weights for fractional phase p/phases of taps at distances
+    (taps/2 - 1 - t) + p/phases, each row normalised to unit sum (float32
+    weights divided by their float32 sum); the last row repeats row phases-1."""
+    half = taps // 2
+    dist = np.arange(half - 1, -half - 1, -1, dtype=np.float64)[None, :] + \
+        (np.arange(phases, dtype=np.float64) / phases)[:, None]
+    r = dist / half
+    window = np.where(np.abs(r) <= 1.0, i0(beta * np.sqrt(np.clip(1.0 - r * r, 0.0, None))) / i0(beta), 0.0)
+    w = np.sinc(dist) * window
+    lut = np.empty((phases + 1, taps), dtype=np.float32)
+    lut[:phases] = w.astype(np.float32) / w.sum(axis=1, keepdims=True).astype(np.float32)
+    lut[phases] = lut[phases - 1]
+    return lut
+
+
+def upload_sinc_lut(rf, lut=None):
+    lut = build_sinc_lut() if lut is None else np.ascontiguousarray(lut, dtype=np.float32)
+    ptr = lut.ctypes.data_as(ctypes.POINTER(ctypes.c_float))
+    _lib.check(rf.lib.ld_set_sinc_lut(rf.handle, ptr, lut.shape[0], lut.shape[1]))
+    rf.downscale_sinc_lut = lut
+
+
+def level_params(rf, system=None):
+    """hz_to_output constants (core.py:1715-1734, utils.py:1061-1075)."""
+    dp, sp = rf.DecoderParams, rf.SysParams
+    lp = _lib.LevelParams()
+    lp.ire0 = dp["ire0"]
+    lp.hz_ire = dp["hz_ire"]
+    lp.output_zero = sp["outputZero"]
+    lp.vsync_ire = dp["vsync_ire"]
+    lp.out_scale = _out_scale(system or rf.system, dp["vsync_ire"])
+    return lp
+
+
+def tbc_fields(rf, demod, fields, linelocs, coord_f32=True, smoothing=0.0, threshold=15,
+               out_f32=None, out_u16=None, final=True, stream=None, workspace=None):
+    """Resample several fields in one launch.
+
+    demod: float32 device tensor; fields: list of dicts with keys
+    demod_offset, demod_len, lineloc_offset, nlines, lineoffset, linesout,
+    outwidth, inlinelen, out_offset; linelocs: float64 device tensor.
+    Returns (out_f32, out_u16) device tensors (one may be None)."""
+    torch = rf.torch
+    if getattr(rf, "downscale_sinc_lut", None) is None:
+        upload_sinc_lut(rf)
+    nf = len(fields)
+    arr = _field_array(fields)
+    total = max(f["out_offset"] + f["linesout"] * f["outwidth"] for f in fields)
+    if final and out_u16 is None:
+        out_u16 = torch.empty(total, dtype=torch.uint16, device=rf.device)
+    if not final and out_f32 is None:
+        out_f32 = torch.empty(total, dtype=torch.float32, device=rf.device)
+    ws_bytes = rf.lib.ld_tbc_workspace_bytes(rf.handle, nf)
+    ws = workspace if workspace is not None else torch.empty(ws_bytes, dtype=torch.uint8, device=rf.device)
+    lp = level_params(rf)
+    out_len = (out_u16 if out_u16 is not None else out_f32).numel()
+    st = stream if stream is not None else rf._stream()
+    _lib.check(rf.lib.ld_tbc_fields(
+        rf.handle, demod.data_ptr(), demod.numel(), arr, nf, linelocs.data_ptr(), linelocs.numel(),
+        1 if coord_f32 else 0, float(smoothing), float(threshold), ctypes.byref(lp),
+        out_f32.data_ptr() if out_f32 is not None else None,
+        out_u16.data_ptr() if out_u16 is not None else None, out_len,
+        ws.data_ptr(), ws.numel(), ctypes.c_void_p(st)))
+    return out_f32, out_u16
+
+
+def _field_array(fields):
+    arr = (_lib.TbcField * len(fields))()
+    for i, f in enumerate(fields):
+        for k, _ in _lib.TbcField._fields_:
+            setattr(arr[i], k, f.get(k, 0))
+    return arr
+
+
+def refine_linelocs_hsync_fields(rf, demod_05, fields, linelocs1, linebad=None, stream=None, out=None,
+                                 workspace=None):
+    """Field.refine_linelocs_hsync (core.py:2484-2551) for many fields in one
+    launch, FP64 positions.  fields: dicts with demod_offset, demod_len,
+    lineloc_offset, nlines; linelocs1: float64 device tensor (per-field
+    relative positions).  Returns (linelocs2 float64, linebad uint8) device tensors."""
+    torch = rf.torch
+    n = linelocs1.numel()
+    bad = None
+    if linebad is not None:
+        bad = linebad.to(device=rf.device, dtype=torch.uint8).contiguous()
+    if out is not None:
+        ll2, bad2 = out
+    else:
+        ll2 = torch.empty_like(linelocs1)
+        bad2 = torch.empty(n, dtype=torch.uint8, device=rf.device)
+    arr = _field_array(fields)
+    ws = workspace
+    if ws is None:
+        ws = torch.empty(max(1, rf.lib.ld_hsync_workspace_bytes(rf.handle, len(fields))), dtype=torch.uint8,
+                         device=rf.device)
+    dp = rf.DecoderParams
+    st = stream if stream is not None else rf._stream()
+    _lib.check(rf.lib.ld_refine_hsync(
+        rf.handle, demod_05.data_ptr(), demod_05.numel(), arr, len(fields), linelocs1.data_ptr(),
+        bad.data_ptr() if bad is not None else None, n, float(rf.freq), rf.iretohz(dp["vsync_ire"] / 2),
+        rf.iretohz(-55), rf.iretohz(30), 1 if rf.system == "PAL" else 0, ll2.data_ptr(), bad2.data_ptr(),
+        ws.data_ptr(), ws.numel(), ctypes.c_void_p(st)))
+    return ll2, bad2
+
+
+def refine_linelocs_hsync(rf, demod_05, linelocs1, linebad=None, stream=None):
+    """Single-field form: linelocs1 (host or device) index demod_05 directly."""
+    torch = rf.torch
+    if not isinstance(linelocs1, torch.Tensor):
+        linelocs1 = torch.from_numpy(np.ascontiguousarray(linelocs1, dtype=np.float64))
+    ll1 = linelocs1.to(device=rf.device, dtype=torch.float64).contiguous()
+    if linebad is not None and not isinstance(linebad, torch.Tensor):
+        linebad = torch.from_numpy(np.ascontiguousarray(linebad, dtype=np.uint8))
+    fd = dict(demod_offset=0, demod_len=demod_05.numel(), lineloc_offset=0, nlines=ll1.numel())
+    ll2, bad2 = refine_linelocs_hsync_fields(rf, demod_05, [fd], ll1, linebad, stream)
+    return ll2, bad2.bool()
+
+
+def compute_deriv_error(linelocs, baserr):
+    """core.py:2553-2564 (host; a few hundred values)."""
+    ll = np.asarray(linelocs, dtype=np.float64)
+    dd = np.abs(np.diff(np.diff(ll))) > 4
+    d1 = np.zeros(len(ll), bool)
+    d1[1:-1] = dd
+    d2 = np.zeros(len(ll), bool)
+    d2[2:] = dd
+    return np.asarray(baserr, bool) | d1 | d2
+
+
+class Field:
+    """One field's TBC state (the video half of core.py:1587-2722).
+
+    decode: the dict returned by ``RFDecode.demod_blocks`` (stitched buffers).
+    linelocs: line locations in that buffer's sample index (FP64).
+    """
+
+    def __init__(self, rf, decode, linelocs=None, lineoffset=0, linebad=None,
+                 wow_level_adjust_smoothing=0, wow_interpolation_method="linear", coord_f32=True):
+        if wow_interpolation_method != "linear":
+            raise NotImplementedError("only the default linear wow interpolation is on the GPU path")
+        self.rf = rf
+        self.data = decode
+        self.inlinelen = rf.linelen
+        self.outlinelen = rf.SysParams["outlinelen"]
+        self.outlinecount = (rf.SysParams["frame_lines"] // 2) + 1
+        self.lineoffset = lineoffset
+        self.linelocs = linelocs
+        self.linebad = linebad
+        self.wow_level_adjust_smoothing = wow_level_adjust_smoothing
+        self.coord_f32 = coord_f32
+        self.out_scale = _out_scale(rf.system, rf.DecoderParams["vsync_ire"])
+        self.dspicture = None
+
+    def refine_linelocs_hsync(self, linelocs1=None, linebad=None):
+        ll1 = self.linelocs if linelocs1 is None else linelocs1
+        bad = self.linebad if linebad is None else linebad
+        ll2, bad2 = refine_linelocs_hsync(self.rf, self.data["video"]["demod_05"], ll1, bad)
+        self.linelocs2 = ll2
+        self.linebad = bad2
+        return ll2
+
+    def downscale(self, lineinfo=None, linesout=None, outwidth=None, channel="demod", audio=0,
+                  final=False, lastfieldwritten=None):
+        """core.py:2626-2722 (video): returns (dsout, None, None); dsout is the
+        float32 Hz field, or the uint16 codes when final=True."""
+        if audio:
+            raise NotImplementedError("analogue audio resampling is not part of the GPU path")
+        torch = self.rf.torch
+        ll = self.linelocs if lineinfo is None else lineinfo
+        if not isinstance(ll, torch.Tensor):
+            ll = torch.as_tensor(np.asarray(ll, dtype=np.float64))
+        ll = ll.to(self.rf.device, torch.float64).contiguous()
+        outwidth = self.outlinelen if outwidth is None else outwidth
+        linesout = self.outlinecount if linesout is None else linesout
+        src = self.data["video"][channel]
+        fd = dict(demod_offset=0, demod_len=src.numel(), lineloc_offset=0, nlines=ll.numel(),
+                  lineoffset=self.lineoffset, linesout=linesout, outwidth=outwidth,
+                  inlinelen=float(self.inlinelen), out_offset=0)
+        f32, u16 = tbc_fields(self.rf, src, [fd], ll, coord_f32=self.coord_f32,
+                              smoothing=self.wow_level_adjust_smoothing, final=final)
+        dsout = u16 if final else f32
+        if final:
+            self.dspicture = dsout
+        return dsout, None, None

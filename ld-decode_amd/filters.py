@@ -1,0 +1,158 @@
+"""Host-side FP64 filter design for the GPU demodulator.
+
+Every filter is a complex frequency response sampled at the ``blocklen`` bins
+of the block FFT, exactly as the reference builds them with
+``scipy.signal.freqz(b, a, blocklen, whole=True)`` (``lddecode/utils.py:782``).
+Because the GPU transforms real blocks, only bins ``0..blocklen/2`` are
+uploaded; for a real-coefficient filter the other half is the conjugate mirror.
+
+Design formulas follow ``RFDecode.computevideofilters`` (core.py:544-679),
+``build_groupdelay_equalizer`` (core.py:480-542), ``computeaudiofilters``
+(core.py:681-737) and the helpers ``emphasis_iir`` (utils.py:766-778),
+``build_hilbert`` (utils.py:882-890) and ``fft_determine_slices``
+(utils.py:929-950).  scipy.signal is the reference's own design library and is
+used here on the host only; no sample data ever passes through it.
+"""
+
+import math
+
+import numpy as np
+import scipy.signal as sps
+
+
+def freq_response(b, a, n):
+    """Whole-circle response of b/a at n bins (utils.py:782-783)."""
+    return sps.freqz(b, a, n, whole=True)[1]
+
+
+def emphasis(t_zero, t_pole, fs):
+    """First-order (de)emphasis: zero at 1/t_zero, pole at 1/t_pole rad/s,
+    bilinear-mapped with prewarping (utils.py:766-778)."""
+    wz = 2 * fs * np.tan((1 / t_zero) / (2 * fs))
+    wp = 2 * fs * np.tan((1 / t_pole) / (2 * fs))
+    b, a = sps.zpk2tf([-wz], [-wp], wp / wz)
+    return sps.bilinear(b, a, fs)
+
+
+def analytic_mask(n):
+    """Weights that keep DC and Nyquist once and double the positive bins
+    (utils.py:882-890)."""
+    if n % 2:
+        raise Exception("build_hilbert: must have even fft_size")
+    m = np.zeros(n)
+    m[0] = m[n // 2] = 1
+    m[1:n // 2] = 2
+    return m
+
+
+def audio_bins(center, min_bw, freq_hz, nbins_in):
+    """(lowbin, nbins, decimated rate) covering center +- min_bw (utils.py:929-950)."""
+    bw = freq_hz / nbins_in
+    cbin = int(np.round(center / bw))
+    half = int(np.round(min_bw / bw))
+    nbins = 2 * (2 ** math.ceil(math.log2(half * 2)))
+    return cbin - nbins // 4, nbins, bw * nbins
+
+
+def gd_equaliser(system, lpf, fs, lpf_freq):
+    """Unit-magnitude all-pass whose group delay is the IEC 60856/60857 target
+    curve minus the video LPF's own group delay (core.py:480-542)."""
+    n = len(lpf)
+    f = np.abs(np.fft.fftfreq(n, 1.0 / fs))
+    if system == "PAL":
+        knots_f = [0.0, 0.5e6, 2.0e6, 3.0e6, 4.0e6, 4.4336e6, 4.8e6, 5.5e6]
+        knots_t = [0.0, 0.0, 10e-9, 35e-9, 85e-9, 135e-9, 200e-9, 200e-9]
+    else:
+        knots_f = [0.0, 0.5e6, 2.0e6, 3.0e6, 3.58e6, 4.0e6, 4.2e6, 4.8e6]
+        knots_t = [0.0, 0.0, 15e-9, 45e-9, 80e-9, 135e-9, 200e-9, 200e-9]
+    target = np.interp(f, np.array(knots_f), np.array(knots_t))
+    gd = -np.gradient(np.unwrap(np.angle(lpf))) / (2 * np.pi * (fs / n))
+    ref_bin = np.argmin(np.abs(f - 0.5e6))
+    resid = target - (gd - gd[ref_bin])
+    lo, hi = lpf_freq + 0.3e6, lpf_freq + 1.3e6
+    resid = resid * np.clip((hi - f) / (hi - lo), 0.0, 1.0)
+    resid[f < 0.4e6] = 0.0
+    h = n // 2
+    phi = -2 * np.pi * np.cumsum(resid[:h + 1]) * (fs / n)
+    eq = np.ones(n, dtype=complex)
+    eq[:h + 1] = np.exp(1j * phi)
+    eq[h + 1:] = np.conj(eq[1:h][::-1])
+    eq[0] = 1.0
+    eq[h] = 1.0
+    return eq
+
+
+class FilterBank:
+    """All frequency-domain filters of one (system, blocklen, parameters) set.
+
+    Attributes are full-length complex128 arrays named as in the reference's
+    ``RFDecode.Filters`` dict (``Frfhpf``, ``RFVideo`` (analytic mask folded in),
+    ``MTF``, ``Fvideo_lpf``, ``Fdeemp``, ``Femp``, ``FVideo``, ``FVideo05``,
+    ``FVideoBurst``, ``FVideoPilot``) plus per-channel audio stage-1 data.
+    """
+
+    def __init__(self, system, blocklen, freq_hz, sysp, decp, analog_audio=True):
+        self.system = system
+        self.blocklen = n = blocklen
+        self.freq_hz = fs = freq_hz
+        nyq_mhz = fs / 2e6
+        nyq = fs / 2
+        F = self.filters = {}
+        F["Frfhpf"] = freq_response(*sps.butter(1, 10 / nyq_mhz, btype="highpass"), n)
+        # MTF: two poles mirrored about fs/4 (core.py:557-567)
+        p_lo = decp["MTF_freq"] / nyq_mhz
+        p_hi = (nyq_mhz + (nyq_mhz - decp["MTF_freq"])) / nyq_mhz
+        poles = [decp["MTF_poledist"] * np.exp(1j * np.pi * p) for p in (p_lo, p_hi)]
+        F["MTF"] = freq_response(*sps.zpk2tf([], poles, 1), n)
+        if "video_bpf_low_order" in decp:
+            hp = sps.butter(decp["video_bpf_low_order"], decp["video_bpf_low"] / nyq, btype="highpass")
+            lp = sps.butter(decp["video_bpf_high_order"], decp["video_bpf_high"] / nyq, btype="lowpass")
+            rfv = freq_response(*hp, n) * freq_response(*lp, n)
+        else:
+            bp = sps.butter(decp["video_bpf_order"],
+                            [decp["video_bpf_low"] / nyq, decp["video_bpf_high"] / nyq], btype="bandpass")
+            rfv = freq_response(*bp, n)
+        if sysp["analog_audio"] and system == "NTSC":
+            for key in ("audio_lfreq", "audio_rfreq"):
+                c, w = sysp[key], decp["audio_notchwidth"]
+                notch = sps.butter(decp["audio_notchorder"], [(c - w) / nyq, (c + w) / nyq], btype="bandstop")
+                rfv = rfv * freq_response(*notch, n)
+        F["RFVideo"] = rfv * analytic_mask(n)
+        F["Fvideo_lpf"] = freq_response(*sps.butter(decp["video_lpf_order"], decp["video_lpf_freq"] / nyq, "low"), n)
+        t1, t2 = decp["video_deemp"]
+        F["Fdeemp"] = freq_response(*emphasis(t1, t2, fs), n)
+        F["Femp"] = freq_response(*emphasis(t2, t1, fs), n)
+        strength = decp.get("video_deemp_strength", 1.0)
+        F["FVideo"] = (F["Fvideo_lpf"] * (F["Fdeemp"] ** strength)
+                       * gd_equaliser(system, F["Fvideo_lpf"], fs, decp["video_lpf_freq"]))
+        lpf_deemp = F["Fvideo_lpf"] * F["Fdeemp"]
+        fir05 = sps.firwin(65, [0.5 / nyq_mhz], pass_zero=True)
+        F["FVideo05"] = lpf_deemp * freq_response(fir05, [1.0], n)
+        fsc = sysp["fsc_mhz"]
+        firb = sps.firwin(81, [(fsc - 0.2) / nyq_mhz, (fsc + 0.2) / nyq_mhz], pass_zero=False)
+        F["FVideoBurst"] = lpf_deemp * freq_response(firb, [1.0], n)
+        if system == "PAL":
+            pil = sysp["pilot_mhz"]
+            bp = sps.butter(1, [(pil - 0.1) / nyq_mhz, (pil + 0.1) / nyq_mhz], btype="bandpass")
+            F["FVideoPilot"] = lpf_deemp * freq_response(*bp, n)
+        # sample shifts applied to the 0.5 MHz and burst outputs (core.py:659, :667)
+        self.offset05 = 32
+        self.offset_burst = 40
+
+        # analogue audio stage 1 (core.py:681-737)
+        self.audio = {}
+        if analog_audio:
+            for ch, key in (("left", "audio_lfreq"), ("right", "audio_rfreq")):
+                c, w = sysp[key], decp["audio_filterwidth"]
+                fir = sps.firwin(decp["audio_filterorder"], [(c - w) / nyq, (c + w) / nyq], pass_zero=False)
+                resp = freq_response(fir, 1.0, n)
+                lowbin, nb, rate = audio_bins(c, 200000, fs, n)
+                h = nb // 2
+                sl = np.concatenate([resp[lowbin:lowbin + h], resp[n - lowbin - h:n - lowbin]])
+                self.audio[ch] = {"lowbin": lowbin, "nbins": nb, "a1_freq": rate,
+                                  "low_freq": fs * (lowbin / n),
+                                  "filt1": sl * analytic_mask(nb)}
+
+    def half(self, name):
+        """Bins 0..blocklen/2 of a filter."""
+        return self.filters[name][: self.blocklen // 2 + 1]

@@ -1,0 +1,147 @@
+"""Seeded synthetic LaserDisc RF captures (NTSC / PAL), FP64.
+
+The signal model follows what the reference itself uses to exercise its
+decoder: an IRE raster with equalising / vertical / horizontal sync pulses
+(``tests/test_ac3rf.py:123-151``), IRE -> Hz with the system's ``ire0`` /
+``hz_ire`` (``core.py:739-741``), pre-emphasis by ``Fvideo_lpf * Femp``
+(``core.py:1007-1016``, ``scripts/bench_rf.py:58-61``) and phase-accumulated FM
+(``genwave``, ``utils.py:1009-1023``).  On top of that: 75% colour bars with a
+colour burst, the two analogue-audio FM carriers, per-line timing jitter,
+AWGN, and 10-bit quantisation stored as ``(code - 512) << 6`` int16
+(``utils.py:371-390``).
+
+The generator records the true (jittered) start of every line's sync pulse,
+in samples, as ground truth for line-location and TBC checks.
+"""
+
+import numpy as np
+import scipy.fft as sfft
+import scipy.signal as sps
+
+from .filters import emphasis
+from .params import system_tables
+
+FS = 40e6
+
+# 75% colour bars: (luma IRE above black, chroma peak IRE, chroma phase deg)
+_BARS = [(100, 0, 0), (89, 31, 167), (72, 44, 283), (62, 41, 241),
+         (48, 41, 61), (38, 44, 103), (21, 31, 347), (0, 0, 0)]
+
+
+class Capture:
+    """A synthetic capture: ``data`` int16 RF, ``line_starts`` float64 sample
+    positions of every sync-pulse leading edge (full lines only), and the
+    system parameters used."""
+
+    def __init__(self, data, line_starts, halfline_starts, system, sysp, fs, snr_db):
+        self.data = data
+        self.line_starts = line_starts
+        self.halfline_starts = halfline_starts
+        self.system = system
+        self.sysp = sysp
+        self.fs = fs
+        self.snr_db = snr_db
+
+
+def _field_layout(system):
+    """(halflines per field, eq pulses, vsync pulses, first video halfline)."""
+    if system == "NTSC":
+        return 525, 6, 6, 18
+    return 625, 5, 5, 15
+
+
+def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0):
+    """IRE value of every sample plus the line-start table.
+
+    Half-line h (global index) starts at h * H/2 * fs plus the jitter of its
+    full line.  Within a field (hpf half-lines), half-lines [0, neq) carry
+    equalising pulses, [neq, neq+nvs) vertical sync pulses, [neq+nvs, 3*neq)
+    equalising pulses, then every other half-line starts a video line with an
+    hsync pulse, burst and colour bars."""
+    hpf, neq, nvs, first = _field_layout(system)
+    H = sysp["line_period"] * 1e-6 * fs          # samples per line
+    hh = H / 2
+    nhalf = int(np.ceil(nsamples / hh)) + 4
+    hidx = np.arange(nhalf) + start_halfline
+    jit = rng.uniform(-jitter_samples, jitter_samples, size=nhalf // 2 + 4)
+    hstart = (hidx - start_halfline) * hh + jit[(hidx - start_halfline) // 2]
+    n = np.arange(nsamples, dtype=np.float64)
+    h_of_n = np.searchsorted(hstart, n, side="right") - 1
+    h_of_n = np.clip(h_of_n, 0, nhalf - 1)
+    t_us = (n - hstart[h_of_n]) / fs * 1e6      # time since the half-line start
+    hf = (h_of_n + start_halfline) % hpf          # half-line within field
+    sync = sysp["vsync_ire"]
+    setup = 7.5 if system == "NTSC" else 0.0
+    ire = np.zeros(nsamples)
+    eq = (hf < neq) | ((hf >= neq + nvs) & (hf < 3 * neq))
+    vs = (hf >= neq) & (hf < neq + nvs)
+    ire[eq & (t_us < sysp["eqPulseUS"])] = sync
+    ire[vs & (t_us < sysp["vsyncPulseUS"])] = sync
+    video = hf >= first
+    second_half = ((hf - first) % 2) == 1
+    # time since the line's hsync for video half-lines
+    tl = t_us + np.where(second_half, sysp["line_period"] / 2, 0.0)
+    vline = video & ~second_half
+    ire[vline & (t_us < sysp["hsyncPulseUS"])] = sync
+    fsc = sysp["fsc_mhz"]
+    t_abs_us = n / fs * 1e6
+    b0, b1 = sysp["colorBurstUS"]
+    burst = video & (tl >= b0) & (tl < b1)
+    burst_phase = np.pi if system == "NTSC" else 3 * np.pi / 4
+    ire[burst] = 20 * np.sin(2 * np.pi * fsc * t_abs_us[burst] + burst_phase)
+    a0, a1 = sysp["activeVideoUS"]
+    act = video & (tl >= a0) & (tl < a1)
+    bar = np.clip(((tl[act] - a0) / (a1 - a0) * 8).astype(int), 0, 7)
+    lum = np.array([b[0] for b in _BARS], float) * 0.75
+    cam = np.array([b[1] for b in _BARS], float)
+    cph = np.deg2rad(np.array([b[2] for b in _BARS], float))
+    ire[act] = (setup + lum[bar] * (1 - setup / 100)
+                + cam[bar] * np.sin(2 * np.pi * fsc * t_abs_us[act] + cph[bar]))
+    # full-line starts: video half-lines at even offset from `first`
+    hf_all = (hidx) % hpf
+    is_line = (hf_all >= first) & (((hf_all - first) % 2) == 0)
+    is_line |= hf_all < 3 * neq  # vblank half-lines also begin with a pulse
+    line_starts = hstart[is_line & (hstart < nsamples)]
+    return ire, line_starts, hstart[hstart < nsamples]
+
+
+def make_capture(system="NTSC", nsamples=700_000, seed=1, snr_db=40.0, jitter_ns=50.0,
+                 audio=True, fs=FS, start_halfline=0):
+    """Generate one capture.  Returns :class:`Capture`."""
+    rng = np.random.default_rng(seed)
+    sysp, decp = system_tables(system)
+    jitter = jitter_ns * 1e-9 * fs
+    ire, line_starts, hstarts = ire_raster(system, sysp, nsamples, fs, jitter, rng, start_halfline)
+    hz = sysp["ire0"] + sysp["hz_ire"] * ire
+    # pre-emphasis: Fvideo_lpf * Femp applied in the frequency domain over the
+    # whole capture (the reference applies the same product per block)
+    lpf_b, lpf_a = sps.butter(decp["video_lpf_order"], decp["video_lpf_freq"] / (fs / 2), "low")
+    t1, t2 = decp["video_deemp"]
+    emp_b, emp_a = emphasis(t2, t1, fs)
+    nf = sfft.next_fast_len(nsamples, real=True)
+    w = np.linspace(0, np.pi, nf // 2 + 1)
+    H = sps.freqz(lpf_b, lpf_a, worN=w)[1] * sps.freqz(emp_b, emp_a, worN=w)[1]
+    mean = hz.mean()
+    spec = sfft.rfft(hz - mean, nf) * H
+    hz = sfft.irfft(spec, nf)[:nsamples] + mean * H[0].real
+    # phase-accumulated FM: phase_i = 2*pi*sum_{m<i} f_m / fs (genwave model)
+    phase = np.empty(nsamples)
+    phase[0] = 0.0
+    np.cumsum(hz[:-1] * (2 * np.pi / fs), out=phase[1:])
+    sig = np.sin(phase)
+    del phase
+    if audio:
+        t = np.arange(nsamples) / fs
+        if system == "NTSC":
+            carriers = [(sysp["audio_lfreq"], 1000.0, 100e3), (sysp["audio_rfreq"], 3000.0, 100e3)]
+        else:
+            carriers = [(sysp["audio_lfreq"], 1000.0, 50e3), (sysp["audio_rfreq"], 3000.0, 50e3)]
+        for fc, fa, dev in carriers:
+            sig += 0.1 * np.sin(2 * np.pi * fc * t - (dev / fa) * np.cos(2 * np.pi * fa * t))
+        del t
+    if snr_db is not None:
+        sigma = np.sqrt(0.5 / 10 ** (snr_db / 10))
+        sig += rng.normal(0.0, sigma, nsamples)
+    code = np.clip(np.round(512 + 511 * sig / np.max(np.abs(sig))), 0, 1023).astype(np.int32)
+    data = ((code - 512) << 6).astype(np.int16)
+    return Capture(data, line_starts, hstarts, system, sysp, fs, snr_db)

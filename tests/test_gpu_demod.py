@@ -1,0 +1,116 @@
+"""GPU parity of the demodulator (HIP kernels through the C ABI) against the
+FP64 oracle on seeded synthetic RF.
+
+Tolerances (FP32 arithmetic vs FP64, stated per channel): video Hz channels
+max |err| <= 25 Hz (1 output LSB is 33.9 Hz NTSC / 21.3 Hz PAL) and relative
+error <= 1e-4 of the FP64 value; rfhpf max |err| <= 0.05 (units of input
+codes); audio stage-1 max |err| <= 10 Hz.
+"""
+
+import numpy as np
+import pytest
+
+from lddecode_amd import _lib, synth
+from oracle import dsp as O
+
+pytestmark = pytest.mark.gpu
+
+VIDEO_TOL_HZ = 25.0
+REL_TOL = 1e-4
+RFHPF_TOL = 0.05
+AUDIO_TOL_HZ = 10.0
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+def _check_video(got, ref, fields):
+    for k in fields:
+        g, r = _np(got[k]), ref[k]
+        assert g.shape == r.shape, k
+        err = np.abs(g - r)
+        assert err.max() <= VIDEO_TOL_HZ, (k, err.max())
+        assert (err / np.maximum(np.abs(r), 1e6)).max() <= REL_TOL, k
+
+
+@pytest.mark.parametrize("system,seed,cut,mtf", [("NTSC", 1, True, 0), ("NTSC", 1, False, 0),
+                                                 ("NTSC", 11, True, 1.0), ("PAL", 3, True, 0),
+                                                 ("PAL", 3, False, 0.5)])
+def test_demodblock_matches_oracle(gpu_rf, oracle_rf, system, seed, cut, mtf):
+    """Config C1: one 32768-sample int16 block, audio carriers, 40 dB AWGN."""
+    rf, orf = gpu_rf(system), oracle_rf(system)
+    cap = synth.make_capture(system, 32768, seed=seed, snr_db=40.0 if system == "NTSC" else 36.0)
+    got = rf.demodblock(data=cap.data, mtf_level=mtf, cut=cut)
+    ref = orf.demodblock(data=cap.data.astype(np.float64), mtf_level=mtf, cut=cut)
+    _check_video(got["video"], ref["video"], rf.video_fields)
+    d = np.abs(_np(got["rfhpf"]) - ref["rfhpf"])
+    assert d.max() <= RFHPF_TOL, d.max()
+    for k in ("audio_left", "audio_right"):
+        g, r = _np(got["audio"][k]), ref["audio"][k]
+        assert g.shape == r.shape
+        assert np.abs(g - r).max() <= AUDIO_TOL_HZ, (k, np.abs(g - r).max())
+
+
+def test_float32_input_and_delays(gpu_rf, oracle_rf):
+    """computedelays runs the GPU demod on a float signal; its delays equal the
+    oracle's (video_rot exactly)."""
+    rf, orf = gpu_rf("NTSC"), oracle_rf("NTSC")
+    assert rf.delays["video_rot"] == orf.delays["video_rot"]
+    for k in ("video_sync", "video_white"):
+        assert abs(rf.delays[k] - orf.delays[k]) < 1e-3
+    x = orf.fake_signal.astype(np.float32)
+    got = rf.demodblock(data=x, cut=True)
+    ref = orf.demodblock(data=x.astype(np.float64), cut=True)
+    _check_video(got["video"], ref["video"], rf.video_fields)
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_stitched_blocks_match_oracle(gpu_rf, oracle_rf, system):
+    """DemodCache stitching: 4 blocks at stride 31712 written straight into
+    contiguous buffers equal the oracle's concatenated cut blocks."""
+    rf, orf = gpu_rf(system), oracle_rf(system)
+    S = rf.block_stride
+    cap = synth.make_capture(system, 3 * S + 32768, seed=21, snr_db=40.0)
+    got = rf.demod_blocks(cap.data, 0, 4)
+    ref, start = O.demod_span(orf, cap.data.astype(np.float64), 0, 4 * S - 1)
+    assert start == 0
+    _check_video(got["video"], {k: ref[k] for k in rf.video_fields}, rf.video_fields)
+    assert np.abs(_np(got["rfhpf"]) - ref["rfhpf"]).max() <= RFHPF_TOL
+    for k in ("audio_left", "audio_right"):
+        assert np.abs(_np(got["audio"][k]) - ref[k]).max() <= AUDIO_TOL_HZ
+
+
+def test_constant_carrier_known_answer(gpu_rf):
+    """A bin-centred FM carrier demodulates to its own frequency."""
+    rf = gpu_rf("NTSC")
+    f = round(rf.iretohz(30) * 32768 / 40e6) * 40e6 / 32768
+    x = (8000 * np.cos(2 * np.pi * f * np.arange(32768) / 40e6)).astype(np.float32)
+    v = rf.demodblock(data=x, cut=True)["video"]
+    np.testing.assert_allclose(_np(v["demod_raw"]), f, atol=2.0)
+    np.testing.assert_allclose(_np(v["demod"])[:-600], f, atol=2.0)
+
+
+def test_blocks_are_independent_of_batching(gpu_rf):
+    """Size-independent property: demodulating 40 blocks in one launch equals
+    two launches of 20 (same kernels per block: bit-identical)."""
+    rf = gpu_rf("NTSC", audio=False)
+    S = rf.block_stride
+    cap = synth.make_capture("NTSC", 39 * S + 32768, seed=9, snr_db=40.0)
+    import torch
+    x = torch.from_numpy(cap.data).cuda()
+    a = rf.demod_blocks(x, 0, 40)["video"]["demod"]
+    b1 = rf.demod_blocks(x, 0, 20)["video"]["demod"]
+    b2 = rf.demod_blocks(x, 20 * S, 20)["video"]["demod"]
+    assert torch.equal(a, torch.cat([b1, b2]))
+
+
+def test_out_of_range_block_is_an_error(gpu_rf):
+    rf = gpu_rf("NTSC")
+    with pytest.raises(_lib.LdGpuError) as e:
+        rf.demod_blocks(np.zeros(40000, np.int16), 0, 2)
+    assert e.value.code == _lib.LD_ERANGE
+    with pytest.raises(ValueError):
+        rf.demodblock(data=np.zeros(1000, np.int16))
+    with pytest.raises(Exception):
+        rf.demodblock()
