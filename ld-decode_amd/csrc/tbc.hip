@@ -15,6 +15,11 @@
 
 #include "internal.hpp"
 
+// Positions, level adjust and the output code map follow the reference's
+// operation order with one rounding per operation; the compiler must not fuse
+// multiplies and adds here (the sinc weight interpolation uses an explicit fmaf).
+#pragma clang fp contract(off)
+
 namespace ldgpu {
 
 constexpr int kLinesMax = 1024;  // line locations per field handled by k_tbc_lines
