@@ -1,21 +1,23 @@
-// RF demodulation kernels (FP32) and the handle / demod part of the C ABI.
+// RF demodulation kernel (FP32) and the handle / demod part of the C ABI.
 //
-// Per block of N = 32768 real samples (M = N/2 = 16384 complex points), the
-// reference's RFDecode.demodblock_cpu (core.py:755-879) is computed as:
+// k_demod_block: one workgroup of 512 threads per block of N = 32768 real
+// samples (M = 16384 complex points) computes the reference's
+// RFDecode.demodblock_cpu (core.py:755-879) end to end:
 //
-//   k_fwd        int16/f32 block -> X = rFFT(x), bins 0..M          (core.py:761)
-//   k_c2r        X * Frfhpf -> irFFT -> cut/roll -> rfhpf            (core.py:769-772)
-//   k_analytic   X * RFVideo(*MTF^l)/N -> analytic z, even and odd   (core.py:800-805)
-//                samples as two M-point inverse transforms
-//   k_demod      z -> instantaneous frequency (conjugate product,    (utils.py:898-926,
-//                atan2) -> demod_raw; clip, remove DC, rFFT -> D      core.py:806-809)
-//   k_c2r        D * {FVideo, FVideo05, FVideoBurst, FVideoPilot}    (core.py:811-845)
-//                -> irFFT -> + dc*H(0) -> roll -> cut
-//   k_audio      sliced X bins * filt1 -> 1024-pt iFFT -> inst. freq (core.py:854-875)
+//   1. X = rFFT(block)                                        (core.py:761)
+//   2. rfhpf = irFFT(X * Frfhpf), cut/rolled by video_rot     (core.py:769-772)
+//   3. analytic z = iFFT(X * RFVideo * MTF^l), even and odd   (core.py:800-805)
+//      samples as two M-point inverse transforms
+//   4. demod_raw = arg(z[n] conj z[n-1]) * fs / 2pi, wrapped  (utils.py:898-926)
+//   5. D = rFFT(clip(demod_raw) - dc)                         (core.py:809)
+//   6. demod, demod_05, demod_burst[, demod_pilot] =          (core.py:811-845)
+//      irFFT(D * H) + dc * H(0), rolled and cut
+//   7. audio stage 1: sliced X bins * filt1 -> 1024-pt iFFT   (core.py:854-875)
+//      -> instantaneous frequency, one wavefront per channel
 //
 // Real transforms of length N use one M-point complex transform plus the
-// standard split (even/odd packing) step.  Spectra live in a per-block
-// workspace in HBM between kernels.
+// even/odd split step.  Every intermediate stays in registers and LDS: HBM
+// sees the input block and the requested outputs only.
 #include <math.h>
 #include <stdlib.h>
 #include <string.h>
@@ -49,244 +51,369 @@ constexpr int kN = 32768;      // block length handled by these kernels
 constexpr int kM = kN / 2;     // complex points per transform
 constexpr int kT = 512;        // threads per workgroup for the M-point FFT
 constexpr int kP = kM / kT;    // points per thread (32)
-constexpr int kLdx = kM + 64;  // workspace row pitch for half spectra (float2)
 constexpr int kNA = 1024;      // audio slice bins at N = 32768
 constexpr float kTwoPi = 6.283185307179586f;
 
-using FwdM = WgFft<kM, -1, kN>;
-using InvM = WgFft<kM, +1, kN>;
-using InvA = WgFft<kNA, +1, kN>;
+using FwdM = WgFft<kM, -1>;
+using InvM = WgFft<kM, +1>;
+using InvA = WgFft<kNA, +1, kSyncWave>;
 
 struct Lds {
   float re[LdsSize<kM>::floats];
   float im[LdsSize<kM>::floats];
 };
 
+// Twiddles of the M-point transform and of the real-transform split, kept in
+// LDS for the whole kernel (copied once from the global table):
+//   p2[k][i] = exp(-2 pi i 2^i k / 1024), k < 32, i < 5   (second pass)
+//   a[h][i]  = exp(-2 pi i 2^i 32 h / 16384), h < 32, i < 4 (third pass,
+//   b[l][i]  = exp(-2 pi i 2^i l / 16384),    l < 32       k = 32 h + l)
+//   split[t] = exp(-2 pi i t / N), t < T: W_N^(t + T m) = split[t] * exp(-2 pi i m / 64)
+struct TwLds {
+  float2 p2[32 * 5];
+  float2 a[32 * 4];
+  float2 b[32 * 4];
+  float2 split[kT];
+};
+constexpr int kTwLdsCount = 32 * 5 + 32 * 4 + 32 * 4 + kT;
+
+struct TwSrcLds {
+  const TwLds* t;
+  template <int NS>
+  __device__ __forceinline__ float2 get(int k, int i) const {
+    if constexpr (NS == 32) {
+      return t->p2[k * 5 + i];
+    } else {
+      static_assert(NS == 1024, "M-point transform passes");
+      return cmul(t->a[(k >> 5) * 4 + i], t->b[(k & 31) * 4 + i]);
+    }
+  }
+};
+
+// W_N^(tid + T m) from the thread's split[tid] and a compile-time constant
+template <int M_>
+__device__ __forceinline__ float2 split_tw(float2 wt) {
+  return cmul(wt, make_float2(Tw<64, M_, -1>::re, Tw<64, M_, -1>::im));
+}
+
+// Everything one fused launch needs.  Filters are half spectra (bins 0..M)
+// with the inverse-transform scale folded in.  The c2r outputs (rfhpf and the
+// post filters) are described uniformly.
+struct C2rOut {
+  const float2* h;
+  float* out;
+  float dc;
+  int shift, cut, len;
+};
+
+struct DemodArgs {
+  const void* in;
+  int64_t first, step;
+  int in_bytes;            // bytes per input sample (2: int16, 4: float32)
+  const float2* tw_rows;   // Stockham pass twiddles of the audio transform (TwLayout)
+  const float2* tw_lds;    // TwLds image, copied to LDS by every workgroup
+  const float2* h_rfv;
+  C2rOut hpf;              // out == nullptr: skip
+  C2rOut post[4];
+  int npost;
+  float* demod_raw;
+  int vcut, vlen;
+  int64_t vstride;
+  float2* ws_even;         // per block M float2: even analytic samples
+  const float2* h_audio[2];
+  int lowbin[2];
+  float rate[2], low_freq[2];
+  float* out_audio[2];
+  int acut, alen;
+  int64_t astride;
+  float fs, clip_lo, clip_hi, dc;
+};
+
+constexpr uint32_t kHalfBytes = (kM + 1) * 8;  // a half-spectrum table
+constexpr uint32_t kStrideB = kT * 8;           // bytes between a thread's elements m and m+1
+
 // ---------------------------------------------------------------- helpers
-// Split step after an M-point FFT of y[m] = x[2m] + i x[2m+1]: thread's
-// v[m] = Y[k], k = tid + T*m, becomes X[k] = E + W_N^k O.  Writes X[0..M].
-__device__ __forceinline__ void r2c_split_store(float2 (&v)[kP], Lds& s, const float2* __restrict__ tw,
-                                                float2* __restrict__ X) {
+// After a forward M-point FFT of y[m] = x[2m] + i x[2m+1] (v[m] = Y[k],
+// k = tid+T*m) form the half spectrum X[k] = E + W_N^k O of the real
+// sequence x in x[m], and X[M] (real) in xM on the thread holding k = 0.
+__device__ __forceinline__ void r2c_split(const float2 (&v)[kP], float2 (&x)[kP], float& xM, Lds& s, float2 wt) {
   const int tid = threadIdx.x;
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int k = tid + kT * m;
-    s.re[pidx(k)] = v[m].x;
-    s.im[pidx(k)] = v[m].y;
+  const LdsPair own(s.re, s.im, pidx(tid)), mir(s.re, s.im, pidx(-tid));
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    own.template st<POff<kT * m>::value>(v[m]);
+  });
+  if (tid == 0) {  // Y[M] = Y[0] (periodic), read by k = 0
+    s.re[pidx(kM)] = v[0].x;
+    s.im[pidx(kM)] = v[0].y;
   }
   __syncthreads();
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int k = tid + kT * m;
-    int km = (kM - k) & (kM - 1);
-    float2 a = v[m];
-    float2 b = make_float2(s.re[pidx(km)], s.im[pidx(km)]);
-    float2 E = make_float2(0.5f * (a.x + b.x), 0.5f * (a.y - b.y));
-    float2 O = make_float2(0.5f * (a.y + b.y), -0.5f * (a.x - b.x));
-    float2 w = tw[k];
-    X[k] = cadd(E, cmul(w, O));
-    if (k == 0) X[kM] = make_float2(a.x - a.y, 0.0f);
-  }
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    float2 p = v[m];
+    float2 q = mir.template ld<POff<kM - kT * m>::value>();
+    float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+    float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+    x[m] = cadd(E, cmul(split_tw<m>(wt), O));
+    if (m == 0 && tid == 0) xM = p.x - p.y;
+  });
   __syncthreads();
 }
 
-// Pre-step of an inverse real transform: from Z = X*H (bins 0..M, Hermitian
-// extension implied) build Y[k] = E + i O in the thread's natural layout.
-__device__ __forceinline__ void c2r_pre(float2 (&v)[kP], const float2* __restrict__ X,
-                                        const float2* __restrict__ H, const float2* __restrict__ tw) {
+// Input of a real inverse transform of Z = X * H (Hermitian half spectrum, H
+// scaled by 1/M), conjugated so that a forward FFT produces the conjugate
+// of the inverse: v[m] = conj(E + i O).
+__device__ __forceinline__ void c2r_pre(const float2 (&x)[kP], float xM, rsrc_t H, float2 (&v)[kP], Lds& s,
+                                        float2 wt) {
   const int tid = threadIdx.x;
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int k = tid + kT * m;
-    int km = kM - k;
-    float2 A = cmul(X[k], H[k]);
-    float2 B = cconj(cmul(X[km], H[km]));
+  const LdsPair own(s.re, s.im, pidx(tid)), mir(s.re, s.im, pidx(-tid));
+  const uint32_t vo = tid * 8u;
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    float2 z = cmul(x[m], bload2(H, vo, m * kStrideB));
+    v[m] = z;
+    own.template st<POff<kT * m>::value>(z);
+  });
+  if (tid == 0) {  // Z[M], read by k = 0
+    float2 zM = cscale(bload2(H, 0u, kM * 8u), xM);
+    s.re[pidx(kM)] = zM.x;
+    s.im[pidx(kM)] = zM.y;
+  }
+  __syncthreads();
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    float2 B = cconj(mir.template ld<POff<kM - kT * m>::value>());
+    float2 A = v[m];
     float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
-    float2 Dd = make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y));
-    float2 O = cmulc(Dd, tw[k]);  // * W_N^{-k}
-    v[m] = make_float2(E.x - O.y, E.y + O.x);
+    float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), split_tw<m>(wt));
+    v[m] = make_float2(E.x - O.y, -(E.y + O.x));
+  });
+  // the FFT's first pass synchronises before it writes LDS
+}
+
+// Write y (from a forward FFT of conjugated input: y[2m] = v.x, y[2m+1] =
+// -v.y) as out[i] = y[(i + cut + shift) mod N] + dc for i < len; stores
+// with i >= len fall outside the descriptor and are dropped.
+__device__ __forceinline__ void c2r_store(const float2 (&v)[kP], const C2rOut& o, int64_t base, int tid) {
+  const rsrc_t r = make_rsrc(o.out + base, (uint32_t)o.len * 4u);
+  const int off = o.cut + o.shift;
+  const float dc = o.dc;
+#pragma unroll
+  for (int m = 0; m < kP; ++m) {
+    int n = 2 * (tid + kT * m);
+    uint32_t i0 = (uint32_t)((n - off) & (kN - 1));
+    uint32_t i1 = (uint32_t)((n + 1 - off) & (kN - 1));
+    bstore1(r, v[m].x + dc, i0 * 4u, 0u);
+    bstore1(r, dc - v[m].y, i1 * 4u, 0u);
   }
 }
 
-// Hz of the phase increment d (radians, from atan2 in (-pi, pi]) wrapped to
-// [0, 2pi): utils.py:921-926
+// Hz of the phase increment d (radians, in (-pi, pi]) wrapped to [0, 2pi):
+// utils.py:921-926
 __device__ __forceinline__ float inc_to_hz(float d, float hz_per_rad, float fs) {
   float hz = d * hz_per_rad;
   return d < 0.0f ? hz + fs : hz;
 }
 
-// ---------------------------------------------------------------- kernels
-template <int DT>
-__global__ __launch_bounds__(kT) void k_fwd(const void* __restrict__ in, int64_t first, int64_t step,
-                                           float2* __restrict__ ws_x, const float2* __restrict__ tw) {
-  __shared__ Lds s;
-  const int tid = threadIdx.x;
-  const int64_t s0 = first + step * blockIdx.x;
-  float2 v[kP];
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int64_t i = s0 + 2 * (int64_t)(tid + kT * m);
-    if constexpr (DT == LD_IN_INT16) {
-      const int16_t* p = static_cast<const int16_t*>(in);
-      v[m] = make_float2((float)p[i], (float)p[i + 1]);
-    } else {
-      const float* p = static_cast<const float*>(in);
-      v[m] = make_float2(p[i], p[i + 1]);
-    }
-  }
-  FwdM::run(v, s.re, s.im, tw);
-  r2c_split_store(v, s, tw, ws_x + (int64_t)blockIdx.x * kLdx);
-}
-
-// Inverse real transform of (spectrum row * H) with output roll/cut:
-// out[i] = y[(i + cut_begin + shift) mod N] + dc, for i < outlen.
-struct C2rArgs {
-  const float2* H[4];
-  float dc[4];
-  int shift[4];
-  float* out[4];
-};
-
-__global__ __launch_bounds__(kT) void k_c2r(const float2* __restrict__ ws_spec, C2rArgs a, int cut_begin,
-                                           int outlen, int64_t out_stride, const float2* __restrict__ tw) {
-  __shared__ Lds s;
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x, f = blockIdx.y;
-  float* out = a.out[f];
-  if (out == nullptr) return;
-  const float2* X = ws_spec + (int64_t)b * kLdx;
-  float2 v[kP];
-  c2r_pre(v, X, a.H[f], tw);
-  InvM::run(v, s.re, s.im, tw);
-  const float dc = a.dc[f];
-  const int off = cut_begin + a.shift[f];
-  out += (int64_t)b * out_stride;
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int n = 2 * (tid + kT * m);
-    int i0 = (n - off) & (kN - 1);
-    int i1 = (n + 1 - off) & (kN - 1);
-    if (i0 < outlen) out[i0] = v[m].x + dc;
-    if (i1 < outlen) out[i1] = v[m].y + dc;
-  }
-}
-
-// Analytic signal: z[n] = (1/N) sum_{k=0..M} Z[k] e^{+2 pi i k n / N} with
-// Z = X * H (H carries the 0/1/2 analytic mask and 1/N).  Even samples:
-// M-point inverse transform of Z[k] (+Z[M] folded into bin 0); odd samples:
-// of Z[k] W_N^{-k} (-Z[M] folded into bin 0).  blockIdx.y selects the half.
-__global__ __launch_bounds__(kT) void k_analytic(const float2* __restrict__ ws_x, const float2* __restrict__ H,
-                                                float2* __restrict__ ws_z, const float2* __restrict__ tw) {
-  __shared__ Lds s;
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x, half = blockIdx.y;
-  const float2* X = ws_x + (int64_t)b * kLdx;
-  float2 v[kP];
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int k = tid + kT * m;
-    float2 z = cmul(X[k], H[k]);
-    if (k == 0) {
-      float2 zm = cmul(X[kM], H[kM]);
-      z = half ? csub(z, zm) : cadd(z, zm);
-    } else if (half) {
-      z = cmulc(z, tw[k]);
-    }
-    v[m] = z;
-  }
-  InvM::run(v, s.re, s.im, tw);
-  float2* Z = ws_z + ((int64_t)b * 2 + half) * kM;
-#pragma unroll
-  for (int m = 0; m < kP; ++m) Z[tid + kT * m] = v[m];
-}
-
-// Instantaneous frequency + clip + DC removal + forward rFFT of the result.
-__global__ __launch_bounds__(kT) void k_demod(const float2* __restrict__ ws_z, float2* __restrict__ ws_d,
-                                             float* __restrict__ demod_raw, int cut_begin, int outlen,
-                                             int64_t out_stride, float fs, float clip_lo, float clip_hi,
-                                             float dc, const float2* __restrict__ tw) {
-  __shared__ Lds s;
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x;
-  const float2* ze = ws_z + (int64_t)b * 2 * kM;
-  const float2* zo = ze + kM;
-  const float hz_per_rad = fs / kTwoPi;
-  float* raw = demod_raw ? demod_raw + (int64_t)b * out_stride : nullptr;
-  float2 v[kP];
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
-    int j = tid + kT * m;
-    float2 e = ze[j], o = zo[j];
-    float h0 = 0.0f;
-    if (j > 0) {
-      float2 p = cmulc(e, zo[j - 1]);
-      h0 = inc_to_hz(atan2f(p.y, p.x), hz_per_rad, fs);
-    }
-    float2 q = cmulc(o, e);
-    float h1 = inc_to_hz(atan2f(q.y, q.x), hz_per_rad, fs);
-    if (raw) {
-      int i0 = 2 * j - cut_begin, i1 = i0 + 1;
-      if (i0 >= 0 && i0 < outlen) raw[i0] = h0;
-      if (i1 >= 0 && i1 < outlen) raw[i1] = h1;
-    }
-    v[m] = make_float2(fminf(fmaxf(h0, clip_lo), clip_hi) - dc, fminf(fmaxf(h1, clip_lo), clip_hi) - dc);
-  }
-  FwdM::run(v, s.re, s.im, tw);
-  r2c_split_store(v, s, tw, ws_d + (int64_t)b * kLdx);
-}
-
-// Analogue audio stage 1: one wavefront per (block, channel).
-struct AudioArgs {
-  const float2* F[2];
-  int lowbin[2];
-  float rate[2];
-  float low_freq[2];
-  float* out[2];
-};
-
-__global__ __launch_bounds__(64) void k_audio(const float2* __restrict__ ws_x, AudioArgs a, int cut_begin,
-                                             int outlen, int64_t out_stride, const float2* __restrict__ tw) {
-  __shared__ float sre[LdsSize<kNA>::floats], sim[LdsSize<kNA>::floats];
-  const int tid = threadIdx.x;
-  const int b = blockIdx.x, ch = blockIdx.y;
-  float* out = a.out[ch];
-  if (out == nullptr) return;
-  const float2* X = ws_x + (int64_t)b * kLdx;
-  const float2* F = a.F[ch];
-  const int lb = a.lowbin[ch];
+// Analogue audio stage 1 for one channel by one wavefront (utils.py:898-926,
+// core.py:857-866): slice bins, 1024-point inverse FFT, instantaneous
+// frequency at the decimated rate plus the slice's low frequency.
+__device__ void audio_wave(const DemodArgs& a, int c, const float2* __restrict__ slice, float* lre, float* lim,
+                           int b, const TwGlobal& twr) {
+  const int lane = threadIdx.x & 63;
+  const rsrc_t F = make_rsrc(a.h_audio[c], kNA * 8u);
   float2 v[kNA / 64];
 #pragma unroll
   for (int m = 0; m < kNA / 64; ++m) {
-    int k = tid + 64 * m;
+    int k = lane + 64 * m;
+    float2 f = bload2(F, lane * 8u, m * 512u);
     float2 z = make_float2(0.0f, 0.0f);
     if (k < kNA / 2)
-      z = cmul(X[lb + k], F[k]);
+      z = cmul(slice[k], f);
     else if (k == kNA / 2)  // bin N - lowbin - nbins/2 of the full spectrum
-      z = cmul(cconj(X[lb + kNA / 2]), F[k]);
+      z = cmul(cconj(slice[kNA / 2]), f);
     v[m] = z;
   }
-  InvA::run(v, sre, sim, tw);
+  InvA::run(v, lre, lim, twr);
 #pragma unroll
   for (int m = 0; m < kNA / 64; ++m) {
-    int n = tid + 64 * m;
-    sre[pidx(n)] = v[m].x;
-    sim[pidx(n)] = v[m].y;
+    int n = lane + 64 * m;
+    lre[pidx(n)] = v[m].x;
+    lim[pidx(n)] = v[m].y;
   }
-  __syncthreads();
-  const float rate = a.rate[ch], lf = a.low_freq[ch];
+  lds_barrier<kSyncWave>();
+  const float rate = a.rate[c], lf = a.low_freq[c];
   const float hz_per_rad = rate / kTwoPi;
-  out += (int64_t)b * out_stride;
+  const rsrc_t out = make_rsrc(a.out_audio[c] + (int64_t)b * a.astride, (uint32_t)a.alen * 4u);
 #pragma unroll
   for (int m = 0; m < kNA / 64; ++m) {
-    int n = tid + 64 * m;
+    int n = lane + 64 * m;
     float hz = 0.0f;
     if (n > 0) {
       int pn = n - 1;
-      float2 p = cmulc(v[m], make_float2(sre[pidx(pn)], sim[pidx(pn)]));
+      float2 p = cmulc(v[m], make_float2(lre[pidx(pn)], lim[pidx(pn)]));
       hz = inc_to_hz(atan2f(p.y, p.x), hz_per_rad, rate);
     }
-    int i = n - cut_begin;
-    if (i >= 0 && i < outlen) out[i] = hz + lf;
+    int i = n - a.acut;  // i < 0 wraps to a huge offset: dropped
+    bstore1(out, hz + lf, (uint32_t)i * 4u, 0u);
+  }
+}
+
+// ---------------------------------------------------------------- kernel
+// One workgroup demodulates one block end to end (RFDecode.demodblock_cpu,
+// core.py:755-879); only the input and the requested outputs touch HBM
+// (plus the even analytic samples, which wait in a per-block workspace).
+// The work is a sequence of phases around ONE forward FFT instance (inverse
+// transforms run on conjugated data), which keeps the kernel's code small
+// enough for the instruction cache:
+//   0        block -> X                         (rFFT)
+//   1        X * Frfhpf -> rfhpf                (c2r, optional)
+//   2, 3     X * RFVideo -> analytic even / odd (inverse)
+//   4        inst. frequency, clip -> D         (rFFT)
+//   5..8     D * post filter -> video outputs   (c2r)
+enum { kPhIn = 0, kPhHpf = 1, kPhEven = 2, kPhOdd = 3, kPhDemod = 4, kPhPost = 5 };
+
+__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
+  __shared__ Lds s;
+  __shared__ float2 aslice[2][kNA / 2 + 1];
+  __shared__ TwLds twl;
+  const int tid = threadIdx.x;
+  const int b = blockIdx.x;
+  const int64_t s0 = a.first + a.step * b;
+  const rsrc_t hrv = make_rsrc(a.h_rfv, kHalfBytes);
+  const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)b * kM, kM * 8u);
+  const uint32_t vo = tid * 8u;
+  {  // twiddle tables into LDS
+    const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
+    float2* dst = &twl.p2[0];
+    for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
+  }
+  __syncthreads();
+  const TwSrcLds twf{&twl};
+  const float2 wt0 = twl.split[tid];
+  float2 v[kP], x[kP];
+  float xM = 0.0f;
+  const int last = kPhPost + a.npost;
+
+  for (int ph = 0; ph < last; ++ph) {
+    // the split twiddles derived from wt are loop-invariant; keep the
+    // compiler from hoisting (and spilling) all 32 of them out of the loop
+    float2 wt = wt0;
+    int tidv = tid;  // opaque copy for per-element store offsets (same reason)
+    asm volatile("" : "+v"(wt.x), "+v"(wt.y), "+v"(tidv));
+    // ---------------------------------------------------------- pre
+    if (ph == kPhIn) {
+      // spectrum of the block (core.py:761): y[j] = x[2j] + i x[2j+1]
+      const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(kN * a.in_bytes));
+      if (a.in_bytes == 4) {
+#pragma unroll
+        for (int m = 0; m < kP; ++m) v[m] = bload2(in, vo, m * kStrideB);
+      } else if ((s0 & 1) == 0) {
+#pragma unroll
+        for (int m = 0; m < kP; ++m) {
+          uint32_t w = __builtin_bit_cast(uint32_t, bload1(in, tid * 4u, m * (kT * 4u)));
+          v[m] = make_float2((float)(int16_t)(w & 0xffffu), (float)(int16_t)(w >> 16));
+        }
+      } else {  // odd start: 16-bit loads
+#pragma unroll
+        for (int m = 0; m < kP; ++m) {
+          uint32_t o = tid * 4u + m * (kT * 4u);
+          int16_t lo = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o, 0, 0);
+          int16_t hi = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o + 2, 0, 0);
+          v[m] = make_float2((float)lo, (float)hi);
+        }
+      }
+    } else if (ph == kPhHpf) {
+      // RF high-pass branch (core.py:769-772)
+      if (a.hpf.out == nullptr) continue;
+      c2r_pre(x, xM, make_rsrc(a.hpf.h, kHalfBytes), v, s, wt);
+    } else if (ph == kPhEven || ph == kPhOdd) {
+      // analytic signal (core.py:800-805), conjugated: even samples from
+      // Z[k] (+ Z[M] folded into bin 0), odd samples from Z[k] W_N^-k
+      // (- Z[M] folded into bin 0)
+      const float2 hM = cscale(bload2(hrv, 0u, kM * 8u), xM);
+      const bool odd = (ph == kPhOdd);
+      static_for<0, kP>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int k = tid + kT * m;
+        float2 z = cmul(x[m], bload2(hrv, vo, m * kStrideB));
+        if (k == 0)
+          z = odd ? csub(z, hM) : cadd(z, hM);
+        else if (odd)
+          z = cmulc(z, split_tw<m>(wt));
+        v[m] = cconj(z);
+      });
+    } else if (ph == kPhDemod) {
+      // instantaneous frequency (utils.py:898-926), clip (core.py:809).
+      // The workspace (even) and v (odd) hold conj(z): arg(a b*) of the
+      // conjugates is minus the wanted increment.  Odd samples go to LDS
+      // shifted by one so each thread reads z[2j-1] at its own index j.
+      const LdsPair sh(s.re, s.im, pidx(tid + 1)), own(s.re, s.im, pidx(tid));
+      static_for<0, kP>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        sh.template st<POff<kT * m>::value>(v[m]);
+      });
+      __syncthreads();
+      const float hz_per_rad = a.fs / kTwoPi;
+      const bool want_raw = a.demod_raw != nullptr;
+      const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)b * a.vstride, (uint32_t)a.vlen * 4u);
+      static_for<0, kP>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int j = tidv + kT * m;
+        float2 e = bload2(zev, vo, m * kStrideB), od = v[m];
+        float2 prev = own.template ld<POff<kT * m>::value>();
+        float2 p = cmulc(e, prev);
+        float h0 = (j > 0) ? inc_to_hz(-atan2f(p.y, p.x), hz_per_rad, a.fs) : 0.0f;
+        float2 q = cmulc(od, e);
+        float h1 = inc_to_hz(-atan2f(q.y, q.x), hz_per_rad, a.fs);
+        if (want_raw) {  // i < 0 wraps to a huge offset: dropped
+          uint32_t i0 = (uint32_t)(2 * j - a.vcut);
+          bstore1(raw, h0, i0 * 4u, 0u);
+          bstore1(raw, h1, i0 * 4u + 4u, 0u);
+        }
+        v[m] = make_float2(fminf(fmaxf(h0, a.clip_lo), a.clip_hi) - a.dc,
+                           fminf(fmaxf(h1, a.clip_lo), a.clip_hi) - a.dc);
+      });
+    } else {
+      // video post filters (core.py:811-820)
+      c2r_pre(x, xM, make_rsrc(a.post[ph - kPhPost].h, kHalfBytes), v, s, wt);
+    }
+
+    // ---------------------------------------------------------- transform
+    FwdM::run(v, s.re, s.im, twf);
+
+    // ---------------------------------------------------------- post
+    if (ph == kPhIn || ph == kPhDemod) {
+      r2c_split(v, x, xM, s, wt);
+      if (ph == kPhIn) {
+        static_for<0, 2>([&](auto cc) {
+          constexpr int c = decltype(cc)::value;
+          if (a.out_audio[c] != nullptr) {
+#pragma unroll
+            for (int m = 0; m < kP; ++m) {
+              int i = tidv + kT * m - a.lowbin[c];
+              if (i >= 0 && i <= kNA / 2) aslice[c][i] = x[m];
+            }
+          }
+        });
+      }
+    } else if (ph == kPhEven) {
+#pragma unroll
+      for (int m = 0; m < kP; ++m) bstore2(zev, v[m], vo, m * kStrideB);
+    } else if (ph == kPhHpf) {
+      c2r_store(v, a.hpf, (int64_t)b * a.vstride, tidv);
+    } else if (ph >= kPhPost) {
+      c2r_store(v, a.post[ph - kPhPost], (int64_t)b * a.vstride, tidv);
+    }
+  }
+
+  // analogue audio stage 1, one wavefront per channel (core.py:854-875)
+  __syncthreads();
+  if (tid < 128) {
+    const int c = tid >> 6;
+    const TwGlobal tg{make_rsrc(a.tw_rows, TwLayout::total * 8u), TwLayout::k1k_p2, TwLayout::k1k_p3, 16, 4, 2};
+    if (a.out_audio[c] != nullptr)
+      audio_wave(a, c, aslice[c], s.re + c * LdsSize<kNA>::floats, s.im + c * LdsSize<kNA>::floats, b, tg);
   }
 }
 
@@ -315,13 +442,36 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   ld_handle* h = new ld_handle();
   h->device = device;
   h->blocklen = blocklen;
-  std::vector<float2> tw(blocklen);
-  for (int t = 0; t < blocklen; ++t) {
-    double a = -2.0 * M_PI * (double)t / (double)blocklen;
-    tw[t] = make_float2((float)cos(a), (float)sin(a));
+  // [0, TwLayout::total): Stockham pass rows (TwLayout), then the TwLds
+  // image (second-pass rows, third-pass factor tables, split twiddles)
+  const int M = 0;
+  std::vector<float2> tw(TwLayout::total + kTwLdsCount);
+  auto cexp = [](double num, double den) {
+    double a = -2.0 * M_PI * num / den;
+    return make_float2((float)cos(a), (float)sin(a));
+  };
+  auto rows = [&](int off, int R, int NS) {
+    int lr = 0;
+    while ((2 << lr) <= R) ++lr;
+    for (int k = 0; k < NS; ++k)
+      for (int i = 0; i < lr; ++i) tw[M + off + k * lr + i] = cexp((double)(1 << i) * k, (double)NS * R);
+  };
+  rows(TwLayout::k16k_p2, 32, 32);
+  rows(TwLayout::k16k_p3, 16, 1024);
+  rows(TwLayout::k1k_p2, 16, 16);
+  rows(TwLayout::k1k_p3, 4, 256);
+  {
+    float2* t = tw.data() + TwLayout::total;
+    for (int k = 0; k < 32; ++k)
+      for (int i = 0; i < 5; ++i) *t++ = cexp((double)(1 << i) * k, 1024);
+    for (int hh = 0; hh < 32; ++hh)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 32 * hh, 16384);
+    for (int l = 0; l < 32; ++l)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * l, 16384);
+    for (int t2 = 0; t2 < kT; ++t2) *t++ = cexp(t2, blocklen);
   }
-  hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * blocklen);
-  if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * blocklen, hipMemcpyHostToDevice);
+  hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * tw.size());
+  if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     int rc = hip_fail(e, "ld_create twiddle upload");
     ld_destroy(h);
@@ -381,7 +531,7 @@ int ld_set_audio(ld_handle* h, int ch, const double* reim, int nbins, int lowbin
     set_error("ld_set_audio: nbins %d not supported (only %d at blocklen %d)", nbins, kNA, kN);
     return LD_EINVAL;
   }
-  if (lowbin < 1 || lowbin + nbins / 2 > h->blocklen / 2) {
+  if (lowbin < 1 || lowbin + nbins / 2 >= h->blocklen / 2) {
     set_error("ld_set_audio: lowbin %d outside the half spectrum", lowbin);
     return LD_EINVAL;
   }
@@ -415,14 +565,14 @@ int ld_set_demod_params(ld_handle* h, double fs, double clip_lo, double clip_hi,
 
 size_t ld_demod_workspace_bytes(const ld_handle* h, int nblocks) {
   if (!h || nblocks <= 0) return 0;
-  // X and D half spectra (kLdx each) + analytic signal (2 * kM)
-  return (size_t)nblocks * sizeof(float2) * (2 * (size_t)kLdx + 2 * (size_t)kM);
+  // even analytic samples of every block (M float2)
+  return (size_t)nblocks * (size_t)(h->blocklen / 2) * sizeof(float2);
 }
 
 int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_len, int64_t first,
                     int64_t step, int nblocks, int cut, const ld_demod_outputs* out, void* workspace,
                     size_t ws_bytes, void* stream) {
-  if (!h || !input || !out || !workspace || nblocks <= 0) {
+  if (!h || !input || !out || nblocks <= 0 || !workspace) {
     set_error("ld_demod_blocks: null argument or nblocks <= 0");
     return LD_EINVAL;
   }
@@ -449,9 +599,8 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
               nblocks - 1, (long long)step, N, (long long)input_len);
     return LD_ERANGE;
   }
-  size_t need = ld_demod_workspace_bytes(h, nblocks);
-  if (ws_bytes < need) {
-    set_error("ld_demod_blocks: workspace %zu bytes < %zu needed", ws_bytes, need);
+  if (ws_bytes < ld_demod_workspace_bytes(h, nblocks)) {
+    set_error("ld_demod_blocks: workspace %zu bytes < %zu needed", ws_bytes, ld_demod_workspace_bytes(h, nblocks));
     return LD_EINVAL;
   }
   const int S = N - h->cut_begin - h->cut_end;
@@ -461,68 +610,61 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   const int fdiv = N / NA;
   const int acut = cut ? h->cut_begin / fdiv : 0;
   const int alen = cut ? NA - acut - (h->cut_end + fdiv - 1) / fdiv : NA;
-  if (out->video_stride < 0 || out->audio_stride < 0 || (nblocks > 1 && out->video_stride < vlen &&
-      out->video_stride != 0)) {
-    set_error("ld_demod_blocks: bad output strides");
+  if (out->video_stride < 0 || out->audio_stride < 0 ||
+      (nblocks > 1 && ((out->video_stride < vlen) || ((out->audio_left || out->audio_right) && out->audio_stride < alen)))) {
+    set_error("ld_demod_blocks: output strides shorter than one block's output");
     return LD_EINVAL;
   }
-  hipStream_t st = (hipStream_t)stream;
-  LD_HIP(hipSetDevice(h->device));
-  float2* ws_x = (float2*)workspace;
-  float2* ws_d = ws_x + (size_t)nblocks * kLdx;
-  float2* ws_z = ws_d + (size_t)nblocks * kLdx;
-  const float2* tw = h->twiddle;
-  if (dtype == LD_IN_INT16)
-    hipLaunchKernelGGL(k_fwd<LD_IN_INT16>, dim3(nblocks), dim3(kT), 0, st, input, first, step, ws_x, tw);
-  else
-    hipLaunchKernelGGL(k_fwd<LD_IN_FLOAT32>, dim3(nblocks), dim3(kT), 0, st, input, first, step, ws_x, tw);
-  LD_HIP(hipGetLastError());
-  if (out->rfhpf) {
-    C2rArgs a = {};
-    a.H[0] = h->filt[LD_FILT_RFHPF];
-    a.dc[0] = 0.0f;
-    a.shift[0] = -h->video_rot;
-    a.out[0] = out->rfhpf;
-    hipLaunchKernelGGL(k_c2r, dim3(nblocks, 1), dim3(kT), 0, st, ws_x, a, h->cut_begin, S, out->video_stride, tw);
-    LD_HIP(hipGetLastError());
-  }
-  if (out->audio_left || out->audio_right) {
-    AudioArgs a = {};
-    for (int c = 0; c < 2; ++c) {
-      a.F[c] = h->audio_filt[c];
-      a.lowbin[c] = h->audio_lowbin[c];
-      a.rate[c] = (float)h->audio_rate[c];
-      a.low_freq[c] = (float)h->audio_low_freq[c];
-    }
-    a.out[0] = out->audio_left;
-    a.out[1] = out->audio_right;
-    hipLaunchKernelGGL(k_audio, dim3(nblocks, 2), dim3(64), 0, st, ws_x, a, acut, alen, out->audio_stride, tw);
-    LD_HIP(hipGetLastError());
-  }
-  const bool need_video = out->demod || out->demod_raw || out->demod_05 || out->demod_burst || out->demod_pilot;
-  if (!need_video) return LD_OK;
-  hipLaunchKernelGGL(k_analytic, dim3(nblocks, 2), dim3(kT), 0, st, ws_x, (const float2*)h->filt[LD_FILT_RFVIDEO],
-                     ws_z, tw);
-  LD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_demod, dim3(nblocks), dim3(kT), 0, st, ws_z, ws_d, out->demod_raw, vcut, vlen,
-                     out->video_stride, (float)h->fs, (float)h->clip_lo, (float)h->clip_hi, (float)h->dc_offset, tw);
-  LD_HIP(hipGetLastError());
-  C2rArgs a = {};
+  DemodArgs a = {};
+  a.in = input;
+  a.first = first;
+  a.step = step;
+  a.tw_rows = h->twiddle;
+  a.tw_lds = h->twiddle + TwLayout::total;
+  a.h_rfv = h->filt[LD_FILT_RFVIDEO];
+  a.hpf.h = h->filt[LD_FILT_RFHPF];
+  a.hpf.out = out->rfhpf;
+  a.hpf.dc = 0.0f;
+  a.hpf.shift = -h->video_rot;
+  a.hpf.cut = h->cut_begin;
+  a.hpf.len = S;
   const int ids[4] = {LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT};
   float* outs[4] = {out->demod, out->demod_05, out->demod_burst, out->demod_pilot};
-  int nf = 0;
   for (int i = 0; i < 4; ++i) {
     if (!outs[i]) continue;
-    a.H[nf] = h->filt[ids[i]];
-    a.dc[nf] = (float)(h->dc_offset * h->filt_dc[ids[i]]);
-    a.shift[nf] = h->filt_shift[ids[i]];
-    a.out[nf] = outs[i];
-    ++nf;
+    C2rOut& o = a.post[a.npost++];
+    o.h = h->filt[ids[i]];
+    o.out = outs[i];
+    o.dc = (float)(h->dc_offset * h->filt_dc[ids[i]]);
+    o.shift = h->filt_shift[ids[i]];
+    o.cut = vcut;
+    o.len = vlen;
   }
-  if (nf) {
-    hipLaunchKernelGGL(k_c2r, dim3(nblocks, nf), dim3(kT), 0, st, ws_d, a, vcut, vlen, out->video_stride, tw);
-    LD_HIP(hipGetLastError());
+  a.demod_raw = out->demod_raw;
+  a.ws_even = (float2*)workspace;
+  a.vcut = vcut;
+  a.vlen = vlen;
+  a.vstride = out->video_stride;
+  float* aouts[2] = {out->audio_left, out->audio_right};
+  for (int c = 0; c < 2; ++c) {
+    a.h_audio[c] = h->audio_filt[c];
+    a.lowbin[c] = h->audio_lowbin[c];
+    a.rate[c] = (float)h->audio_rate[c];
+    a.low_freq[c] = (float)h->audio_low_freq[c];
+    a.out_audio[c] = aouts[c];
   }
+  a.acut = acut;
+  a.alen = alen;
+  a.astride = out->audio_stride;
+  a.fs = (float)h->fs;
+  a.clip_lo = (float)h->clip_lo;
+  a.clip_hi = (float)h->clip_hi;
+  a.dc = (float)h->dc_offset;
+  a.in_bytes = dtype == LD_IN_INT16 ? 2 : 4;
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  hipLaunchKernelGGL(k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
+  LD_HIP(hipGetLastError());
   return LD_OK;
 }
 

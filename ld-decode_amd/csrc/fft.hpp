@@ -19,6 +19,29 @@
 
 namespace ldgpu {
 
+// ------------------------------------------------------------ buffer access
+// Raw buffer descriptors (gfx9 family word 3 = 0x00020000): one SGPR
+// descriptor per array, a per-thread VGPR byte offset and a scalar byte
+// offset per element group.  Out-of-range loads return 0 and out-of-range
+// stores are dropped by the hardware range check (num_records).
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ rsrc_t make_rsrc(const void* base, uint32_t bytes) {
+  return __builtin_amdgcn_make_buffer_rsrc(const_cast<void*>(base), (short)0, (int)bytes, 0x00020000);
+}
+__device__ __forceinline__ float2 bload2(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float2, __builtin_amdgcn_raw_buffer_load_b64(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ float bload1(rsrc_t r, uint32_t voff, uint32_t soff) {
+  return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
+}
+__device__ __forceinline__ void bstore1(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+}
+__device__ __forceinline__ void bstore2(rsrc_t r, float2 v, uint32_t voff, uint32_t soff) {
+  __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
+                                        r, (int)voff, (int)soff, 0);
+}
+
 // ---------------------------------------------------------------- complex ops
 __device__ __forceinline__ float2 cadd(float2 a, float2 b) { return make_float2(a.x + b.x, a.y + b.y); }
 __device__ __forceinline__ float2 csub(float2 a, float2 b) { return make_float2(a.x - b.x, a.y - b.y); }
@@ -99,6 +122,36 @@ __device__ __forceinline__ void dif_rec(float2* v) {
     dif_rec<N / 2, DIR>(v + N / 2);
   }
 }
+// compile-time loop: f(std::integral_constant<int, I>) for I in [B, E)
+template <int I>
+struct IntC {
+  static constexpr int value = I;
+};
+template <int B, int E, class F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(IntC<B>{});
+    static_for<B + 1, E>(f);
+  }
+}
+
+// static_for with a scheduling fence after every G iterations: bounds how far
+// the scheduler hoists independent loads (register pressure) in long
+// element-wise loops
+template <int B, int E, int G, class F>
+__device__ __forceinline__ void static_for_chunked(F&& f) {
+  if constexpr (B < E) {
+    f(IntC<B>{});
+    if constexpr ((B + 1) % G == 0) __builtin_amdgcn_sched_barrier(0);
+    static_for_chunked<B + 1, E, G>(f);
+  }
+}
+
+constexpr int ilog2(int x) {
+  int r = 0;
+  while ((1 << (r + 1)) <= x) ++r;
+  return r;
+}
 constexpr int bitrev(int x, int n) {
   int r = 0;
   for (int b = 1; b < n; b <<= 1) {
@@ -110,37 +163,91 @@ constexpr int bitrev(int x, int n) {
 
 // --------------------------------------------------------------- LDS layout
 __device__ __forceinline__ int pidx(int i) { return i + (i >> 5); }
+// pidx(base + C) for a compile-time C, written so that the per-thread part
+// (pidx(base)) is shared and C contributes an immediate offset whenever
+// (base & 31) + (C & 31) < 32 is provable (always when C % 32 == 0).
+template <int C>
+__device__ __forceinline__ int pidx_off(int pbase, int base) {
+  return pbase + C + (C >> 5) + (((base & 31) + (C & 31)) >> 5);
+}
 template <int N>
 struct LdsSize {
-  static constexpr int floats = N + N / 32;  // per real/imag array
+  static constexpr int floats = N + N / 32 + 32;  // per real/imag array (+ one wrap slot at index N)
 };
 
+// Per-thread LDS base for a (real, imaginary) pair of float arrays, kept in a
+// register the optimiser cannot see through, so every access is the base
+// plus an immediate offset (< 64 KiB) instead of a separately materialised
+// address.  Element offsets passed to ld/st are compile-time constants.
+typedef __attribute__((address_space(3))) float lds_float;
+__device__ __forceinline__ uint32_t lds_addr(const float* p) {
+  return (uint32_t)(uintptr_t)(const lds_float*)p;
+}
+struct LdsPair {
+  uint32_t re, im;
+  __device__ __forceinline__ LdsPair(const float* lre, const float* lim, int elem) {
+    re = lds_addr(lre) + 4u * (uint32_t)elem;
+    im = lds_addr(lim) + 4u * (uint32_t)elem;
+    asm volatile("" : "+v"(re), "+v"(im));
+  }
+  template <int C>
+  __device__ __forceinline__ float2 ld() const {
+    return make_float2(((const lds_float*)(uintptr_t)re)[C], ((const lds_float*)(uintptr_t)im)[C]);
+  }
+  template <int C>
+  __device__ __forceinline__ void st(float2 v) const {
+    ((lds_float*)(uintptr_t)re)[C] = v.x;
+    ((lds_float*)(uintptr_t)im)[C] = v.y;
+  }
+};
+// padded offset of element (base + C) relative to padded base, when
+// (base & 31) + (C & 31) < 32 (true for every access pattern used here)
+template <int C>
+struct POff {
+  static constexpr int value = C + (C >> 5);
+};
+
+// ------------------------------------------------------------ barriers
+// Block-wide (all T threads of the workgroup take part) or wavefront-local
+// (T == 64 and other waves may be doing something else).
+enum { kSyncBlock = 0, kSyncWave = 1 };
+template <int SYNC>
+__device__ __forceinline__ void lds_barrier() {
+  if constexpr (SYNC == kSyncBlock) {
+    __syncthreads();
+  } else {
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  }
+}
+
 // ------------------------------------------------------------ Stockham pass
-// Pass of radix R after NS points have been combined.  TW points at a table
-// of exp(-2*pi*i*t/L) for t < L (L a multiple of N); DIR = -1 forward, +1
-// inverse (conjugated twiddles).  FROM_LDS: inputs come from LDS (else from
-// registers in natural layout).  TO_LDS: outputs go to LDS (else, only legal
-// when NS*R == N, to registers in natural layout).
-template <int N, int T, int R, int NS, int DIR, bool FROM_LDS, bool TO_LDS, int L>
+// Pass of radix R after NS points have been combined.  TW is this pass's
+// twiddle rows: TW[k*log2(R) + i] = exp(-2*pi*i*2^i*k/(NS*R)) for k < NS
+// (unused when NS == 1); DIR = -1 forward, +1 inverse (conjugated twiddles).  FROM_LDS: inputs come from LDS (else from registers in natural
+// layout).  TO_LDS: outputs go to LDS (else, only legal when NS*R == N, to
+// registers in natural layout).
+template <int N, int T, int R, int NS, int DIR, bool FROM_LDS, bool TO_LDS, int SYNC, class TWF>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[N / T], float* __restrict__ lre,
-                                              float* __restrict__ lim,
-                                              const float2* __restrict__ tw) {
+                                              float* __restrict__ lim, const TWF& twf) {
   constexpr int P = N / T;
   constexpr int Q = P / R;  // butterflies per thread
   static_assert(P % R == 0, "radix must divide points per thread");
   static_assert(TO_LDS || NS * R == N, "only the last pass may end in registers");
-  const int tid = threadIdx.x;
+  const int tid = threadIdx.x & (T - 1);
+  static_assert(T % 32 == 0 && (N / R) % 32 == 0, "LDS offsets assume 32-aligned strides");
   if constexpr (FROM_LDS) {
-#pragma unroll
-    for (int q = 0; q < Q; ++q)
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        int i = tid + T * q + r * (N / R);
-        v[q + r * Q] = make_float2(lre[pidx(i)], lim[pidx(i)]);
-      }
-    __syncthreads();
+    const LdsPair b(lre, lim, pidx(tid));
+    static_for<0, Q>([&](auto qc) {
+      static_for<0, R>([&](auto rc) {
+        constexpr int q = decltype(qc)::value, r = decltype(rc)::value;
+        v[q + r * Q] = b.template ld<POff<T * q + r * (N / R)>::value>();
+      });
+    });
+    lds_barrier<SYNC>();
   } else if constexpr (TO_LDS) {
-    __syncthreads();  // LDS may still be read by a previous stage
+    lds_barrier<SYNC>();  // LDS may still be read by a previous stage
   }
 #pragma unroll
   for (int q = 0; q < Q; ++q) {
@@ -150,70 +257,96 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[N / T], float* __restr
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
     if constexpr (NS > 1) {
-      constexpr int step = L / (NS * R);
+      // w_r = W^{r k}: the log2(R) powers W^{2^i k} come from the table, the
+      // others are products over the set bits of r (at most log2(R)-1
+      // multiplies deep: a few float32 ulp)
+      constexpr int LR = ilog2(R);
+      float2 base[LR];
 #pragma unroll
-      for (int r = 1; r < R; ++r) {
-        float2 w = tw[r * k * step];
-        if constexpr (DIR > 0) w.y = -w.y;
-        u[r] = cmul(u[r], w);
+      for (int i = 0; i < LR; ++i) {
+        base[i] = twf.template get<NS>(k, i);
+        if constexpr (DIR > 0) base[i].y = -base[i].y;
       }
+      float2 w[R];
+      static_for_chunked<1, R, 8>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        constexpr int hb = ilog2(r);
+        if constexpr (r == (1 << hb))
+          w[r] = base[hb];
+        else
+          w[r] = cmul(w[r - (1 << hb)], base[hb]);
+        u[r] = cmul(u[r], w[r]);
+      });
     }
     dif_rec<R, DIR>(u);
     if constexpr (TO_LDS) {
       const int base = (j / NS) * NS * R + k;
-#pragma unroll
-      for (int r = 0; r < R; ++r) {
-        int o = base + r * NS;
-        float2 x = u[bitrev(r, R)];
-        lre[pidx(o)] = x.x;
-        lim[pidx(o)] = x.y;
-      }
+      const LdsPair w(lre, lim, pidx(base));
+      static_for<0, R>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        w.template st<POff<r * NS>::value>(u[bitrev(r, R)]);
+      });
     } else {
 #pragma unroll
       for (int r = 0; r < R; ++r) v[q + r * Q] = u[bitrev(r, R)];
     }
   }
-  if constexpr (TO_LDS) __syncthreads();
+  if constexpr (TO_LDS) lds_barrier<SYNC>();
 }
 
-// Full transforms used by the demodulator.  On entry v[m] = x[tid + T*m];
-// on exit v[m] = X[tid + T*m] (unnormalised).  LDS is free again on return.
-template <int N, int DIR, int L>
-struct WgFft;
+// Twiddle rows of every pass with NS > 1, concatenated (global table).
+// Offsets (in float2) of each transform's passes; the host builds the table
+// with the same layout: row k of a pass holds exp(-2 pi i 2^i k / (NS R)),
+// i < log2(R).
+struct TwLayout {
+  // 16384 points: passes (R=32, NS=32) and (R=16, NS=1024)
+  static constexpr int k16k_p2 = 0;
+  static constexpr int k16k_p3 = k16k_p2 + 32 * 5;
+  // 1024 points: passes (R=16, NS=16) and (R=4, NS=256)
+  static constexpr int k1k_p2 = k16k_p3 + 1024 * 4;
+  static constexpr int k1k_p3 = k1k_p2 + 16 * 4;
+  static constexpr int total = k1k_p3 + 256 * 2;
+};
 
-// 16384 points, 512 threads: radix 32 x 32 x 16
-template <int DIR, int L>
-struct WgFft<16384, DIR, L> {
-  static constexpr int T = 512;
-  static __device__ __forceinline__ void run(float2 (&v)[32], float* lre, float* lim,
-                                             const float2* tw) {
-    stockham_pass<16384, 512, 32, 1, DIR, false, true, L>(v, lre, lim, tw);
-    stockham_pass<16384, 512, 32, 32, DIR, true, true, L>(v, lre, lim, tw);
-    stockham_pass<16384, 512, 16, 1024, DIR, true, false, L>(v, lre, lim, tw);
+// Twiddle source reading the global rows through a buffer descriptor.
+struct TwGlobal {
+  rsrc_t tw;
+  int off_p2, off_p3;  // row tables of the second and third pass
+  int ns_p2, lr_p2, lr_p3;
+  template <int NS>
+  __device__ __forceinline__ float2 get(int k, int i) const {
+    const int off = (NS == 16 || NS == 32) ? off_p2 : off_p3;
+    const int lr = (NS == 16 || NS == 32) ? lr_p2 : lr_p3;
+    return bload2(tw, (uint32_t)(off + k * lr + i) * 8u, 0u);
   }
 };
 
-// 8192 points, 256 threads: radix 32 x 16 x 16
-template <int DIR, int L>
-struct WgFft<8192, DIR, L> {
-  static constexpr int T = 256;
-  static __device__ __forceinline__ void run(float2 (&v)[32], float* lre, float* lim,
-                                             const float2* tw) {
-    stockham_pass<8192, 256, 32, 1, DIR, false, true, L>(v, lre, lim, tw);
-    stockham_pass<8192, 256, 16, 32, DIR, true, true, L>(v, lre, lim, tw);
-    stockham_pass<8192, 256, 16, 512, DIR, true, false, L>(v, lre, lim, tw);
+// Full transforms used by the demodulator.  On entry v[m] = x[tid + T*m];
+// on exit v[m] = X[tid + T*m] (unnormalised).  LDS is free again on return.
+template <int N, int DIR, int SYNC = kSyncBlock>
+struct WgFft;
+
+// 16384 points, 512 threads: radix 32 x 32 x 16
+template <int DIR, int SYNC>
+struct WgFft<16384, DIR, SYNC> {
+  static constexpr int T = 512;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[32], float* lre, float* lim, const TWF& twf) {
+    stockham_pass<16384, 512, 32, 1, DIR, false, true, SYNC>(v, lre, lim, twf);
+    stockham_pass<16384, 512, 32, 32, DIR, true, true, SYNC>(v, lre, lim, twf);
+    stockham_pass<16384, 512, 16, 1024, DIR, true, false, SYNC>(v, lre, lim, twf);
   }
 };
 
 // 1024 points, 64 threads (one wavefront): radix 16 x 16 x 4
-template <int DIR, int L>
-struct WgFft<1024, DIR, L> {
+template <int DIR, int SYNC>
+struct WgFft<1024, DIR, SYNC> {
   static constexpr int T = 64;
-  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim,
-                                             const float2* tw) {
-    stockham_pass<1024, 64, 16, 1, DIR, false, true, L>(v, lre, lim, tw);
-    stockham_pass<1024, 64, 16, 16, DIR, true, true, L>(v, lre, lim, tw);
-    stockham_pass<1024, 64, 4, 256, DIR, true, false, L>(v, lre, lim, tw);
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf) {
+    stockham_pass<1024, 64, 16, 1, DIR, false, true, SYNC>(v, lre, lim, twf);
+    stockham_pass<1024, 64, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf);
+    stockham_pass<1024, 64, 4, 256, DIR, true, false, SYNC>(v, lre, lim, twf);
   }
 };
 
