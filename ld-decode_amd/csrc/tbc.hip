@@ -11,6 +11,7 @@
 //                 demod_05 (core.py:2484-2551, calczc utils.py:795-854)
 #include <math.h>
 
+#include <cstring>
 #include <vector>
 
 #include "internal.hpp"
@@ -32,6 +33,20 @@ struct LineRow {
   long long first_px[kLinesMax + 1];
 };
 
+// Everything the pixel kernel needs for one output line, written by
+// k_tbc_lines so that a line's workgroups start with a single record load.
+struct LineRec {
+  double ll[4];          // knots jB .. jB+3
+  double lev[3], ema[3]; // per-segment level and EMA start, segments jB .. jB+2
+  double w[3];           // 1 / (t_{j+1} - t_j) of those segments
+  long long first[3];    // first pixel of those segments
+  long long p0;          // pixel index of the line's first output pixel
+  long long demod_offset, demod_len, out_offset;
+  long long lineloc_offset;
+  double inl, rinl, outscale;
+  int jB, nseg, W, field;
+};
+
 // Segment of the k=1 spline holding x: knots at j*inlinelen, right-continuous
 // (scipy's interval search picks t[j] <= x < t[j+1]).
 __device__ __forceinline__ int segment_of(double x, double inl, int nseg_max) {
@@ -41,69 +56,81 @@ __device__ __forceinline__ int segment_of(double x, double inl, int nseg_max) {
   return j < 0 ? 0 : (j > nseg_max ? nseg_max : j);
 }
 
-// Bitonic sort of n <= kLinesMax (key, weight) pairs in shared memory.
-__device__ void block_sort(double* key, int* wgt, int n) {
-  int npow = 1;
-  while (npow < n) npow <<= 1;
-  for (int i = n + threadIdx.x; i < npow; i += blockDim.x) {
-    key[i] = INFINITY;
-    wgt[i] = 0;
+// Sort n <= kLinesMax (key, weight) pairs in shared memory by rank: each
+// element's position is the number of elements ordered before it (ties by
+// index), so one pass of n^2/blockDim comparisons and two barriers.
+__device__ void block_sort(double* key, int* wgt, double* tkey, int* twgt, int n) {
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    const double ki = key[i];
+    int r = 0;
+    for (int j = 0; j < n; ++j) {
+      const double kj = key[j];
+      r += (kj < ki) || (kj == ki && j < i);
+    }
+    tkey[r] = ki;
+    twgt[r] = wgt[i];
   }
   __syncthreads();
-  for (int k = 2; k <= npow; k <<= 1) {
-    for (int j = k >> 1; j > 0; j >>= 1) {
-      for (int i = threadIdx.x; i < npow; i += blockDim.x) {
-        int ixj = i ^ j;
-        if (ixj > i) {
-          bool up = (i & k) == 0;
-          double a = key[i], b = key[ixj];
-          if ((a > b) == up) {
-            key[i] = b;
-            key[ixj] = a;
-            int t = wgt[i];
-            wgt[i] = wgt[ixj];
-            wgt[ixj] = t;
-          }
-        }
-      }
-      __syncthreads();
-    }
+  for (int i = threadIdx.x; i < n; i += blockDim.x) {
+    key[i] = tkey[i];
+    wgt[i] = twgt[i];
   }
+  __syncthreads();
 }
 
-// Median of the multiset {key[i] repeated wgt[i] times} (sorted keys): the
-// mean of the elements at ranks (L-1)/2 and L/2, as numpy's median.
-__device__ double weighted_median(const double* key, const int* wgt, int n) {
-  long long L = 0;
-  for (int i = 0; i < n; ++i) L += wgt[i];
-  long long r0 = (L - 1) / 2, r1 = L / 2;
-  double v0 = 0, v1 = 0;
-  long long c = 0;
-  bool got0 = false;
-  for (int i = 0; i < n; ++i) {
-    long long nc = c + wgt[i];
-    if (!got0 && r0 < nc) {
-      v0 = key[i];
-      got0 = true;
-    }
-    if (r1 < nc) {
-      v1 = key[i];
-      break;
-    }
-    c = nc;
+// Median of the multiset {key[i] repeated wgt[i] times} over sorted keys:
+// the mean of the elements at ranks (L-1)/2 and L/2 (numpy's median), with
+// L the total weight.  Inclusive prefix sums of the weights (block scan in
+// `scan`) locate both ranks in parallel.  All threads return the result.
+__device__ double weighted_median(const double* key, const int* wgt, int* scan, int n, double* out) {
+  constexpr int PER = kLinesMax / kLineThreads;  // elements per thread
+  const int t = threadIdx.x;
+  int local[PER];
+  int run = 0;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = t * PER + e;
+    run += (i < n) ? wgt[i] : 0;
+    local[e] = run;
   }
-  return (L % 2) ? v1 : (v0 + v1) / 2.0;
+  scan[t] = run;
+  __syncthreads();
+  for (int d = 1; d < kLineThreads; d <<= 1) {  // Hillis-Steele over thread totals
+    int v = (t >= d) ? scan[t - d] : 0;
+    __syncthreads();
+    scan[t] += v;
+    __syncthreads();
+  }
+  const int before = (t > 0) ? scan[t - 1] : 0;
+  const long long L = scan[kLineThreads - 1];
+  const long long r0 = (L - 1) / 2, r1 = L / 2;
+#pragma unroll
+  for (int e = 0; e < PER; ++e) {
+    const int i = t * PER + e;
+    if (i >= n) break;
+    const long long hi = before + local[e], lo = hi - wgt[i];
+    if (r0 >= lo && r0 < hi) out[0] = key[i];
+    if (r1 >= lo && r1 < hi) out[1] = key[i];
+  }
+  __syncthreads();
+  const double m = (L % 2) ? out[1] : (out[0] + out[1]) / 2.0;
+  __syncthreads();
+  return m;
 }
 
 __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* __restrict__ fields,
                                                            const double* __restrict__ linelocs,
                                                            LineRow* __restrict__ rows, double smoothing,
-                                                           double threshold) {
+                                                           double threshold, LineRec* __restrict__ recs,
+                                                           const int* __restrict__ rec_base) {
   __shared__ double key[kLinesMax];
   __shared__ int wgt[kLinesMax];
   __shared__ double slope[kLinesMax];
   __shared__ int cnt[kLinesMax];
-  __shared__ double med_s, mad_s;
+  __shared__ double tkey[kLinesMax];
+  __shared__ int twgt[kLinesMax];
+  __shared__ int scan[kLineThreads];
+  __shared__ double sel[2];
   const ld_tbc_field fd = fields[blockIdx.x];
   const double* ll = linelocs + fd.lineloc_offset;
   LineRow* row = rows + blockIdx.x;
@@ -138,19 +165,15 @@ __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* 
     wgt[j] = c;
   }
   __syncthreads();
-  block_sort(key, wgt, nseg);
-  if (threadIdx.x == 0) med_s = weighted_median(key, wgt, nseg);
-  __syncthreads();
-  const double med = med_s;
+  block_sort(key, wgt, tkey, twgt, nseg);
+  const double med = weighted_median(key, wgt, scan, nseg, sel);
   for (int j = threadIdx.x; j < nseg; j += blockDim.x) {
     key[j] = fabs(slope[j] - med);
     wgt[j] = cnt[j];
   }
   __syncthreads();
-  block_sort(key, wgt, nseg);
-  if (threadIdx.x == 0) mad_s = weighted_median(key, wgt, nseg);
-  __syncthreads();
-  const double mad = mad_s;
+  block_sort(key, wgt, tkey, twgt, nseg);
+  const double mad = weighted_median(key, wgt, scan, nseg, sel);
   const double thr = mad > 0 ? threshold * mad : 0.001;
   for (int j = threadIdx.x; j < nseg; j += blockDim.x)
     row->level[j] = fabs(slope[j] - med) > thr ? med : slope[j];
@@ -171,11 +194,42 @@ __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* 
       }
     }
   }
+  __syncthreads();
+  // one record per output line
+  LineRec* out = recs + rec_base[blockIdx.x];
+  for (int line = threadIdx.x; line < fd.linesout; line += blockDim.x) {
+    LineRec r;
+    const long long p0 = (long long)(line + fd.lineoffset + 1) * W;
+    int jB = (int)(p0 / W) - 1;
+    jB = jB > nseg - 3 ? nseg - 3 : jB;
+    jB = jB < 0 ? 0 : jB;
+    for (int i = 0; i < 4; ++i) r.ll[i] = ll[jB + i < nseg ? jB + i : nseg];
+    for (int i = 0; i < 3; ++i) {
+      const int js = jB + i < nseg ? jB + i : nseg - 1;
+      r.lev[i] = row->level[js];
+      r.ema[i] = row->ema_start[js];
+      r.first[i] = row->first_px[js];
+      r.w[i] = 1.0 / (((double)(js + 1) * inl) - ((double)js * inl));
+    }
+    r.p0 = p0;
+    r.demod_offset = fd.demod_offset;
+    r.demod_len = fd.demod_len;
+    r.out_offset = fd.out_offset + (long long)line * W;
+    r.lineloc_offset = fd.lineloc_offset;
+    r.inl = inl;
+    r.rinl = 1.0 / inl;
+    r.outscale = outscale;
+    r.jB = jB;
+    r.nseg = nseg;
+    r.W = W;
+    r.field = blockIdx.x;
+    out[line] = r;
+  }
 }
 
 struct TbcPixelArgs {
+  const LineRec* recs;
   const float* demod;
-  const ld_tbc_field* fields;
   const double* linelocs;
   const LineRow* rows;
   const float* lut;
@@ -184,82 +238,221 @@ struct TbcPixelArgs {
   double smoothing;
   double lscale, loffset;  // hz_to_output: x*lscale + loffset
   int coord_f32;
+  int lut_folded;          // read rows > 32768 as mirrored rows of the lower half
+  int chunks;              // workgroups per output line
 };
 
-__global__ __launch_bounds__(256) void k_tbc_pixels(TbcPixelArgs a) {
-  const ld_tbc_field fd = a.fields[blockIdx.y];
-  const int W = fd.outwidth;
-  const long long q = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  const long long nout = (long long)fd.linesout * W;
-  if (q >= nout) return;
-  const LineRow* row = a.rows + blockIdx.y;
-  const double* ll = a.linelocs + fd.lineloc_offset;
-  const long long p = q + (long long)W * (fd.lineoffset + 1);
-  const double inl = fd.inlinelen;
-  const double x = (double)p * (inl / (double)W);
-  const int nseg = fd.nlines - 1;
-  const int j = segment_of(x, inl, nseg - 1);
+constexpr int kTbcThreads = 256;       // pixels per workgroup (one chunk of a line)
+constexpr int kTbcMaxWidth = 2048;
+constexpr int kChunkStage = 2048;      // staged source samples per chunk
+constexpr int kStagePerThread = kChunkStage / kTbcThreads;
+
+typedef __amdgpu_buffer_rsrc_t rsrc_t;
+__device__ __forceinline__ float4 lut_load(rsrc_t r, uint32_t off) {
+  return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
+}
+
+// Per-pixel state between the LUT loads and the tap sum.
+struct PixelPrep {
+  long long start;  // first tap's sample index (field-relative)
+  double level;
+  float alpha;
+  bool fold;
+  float4 u[4], v[4];  // LUT groups: ws taps (u) and we taps (v), see pixel_prep
+};
+
+// Source position, level factor and LUT rows of pixel t of the line.
+__device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec& rc, rsrc_t lut, int t,
+                                           PixelPrep& q) {
+  const int W = rc.W;
+  const long long p = rc.p0 + t;
+  const double x = (double)p * rc.outscale;
+  // segment of the k=1 spline holding x: knots at j*inlinelen, right-
+  // continuous; the guess x*(1/inl) is within one of it
+  int j = (int)(x * rc.rinl);
+  if ((double)(j + 1) * rc.inl <= x) ++j;
+  if ((double)j * rc.inl > x) --j;
+  j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
+  const int jl = j - rc.jB;
+  double llj, llj1, lev, ema, w;
+  long long first;
+  if (jl >= 0 && jl <= 2) {
+    llj = rc.ll[jl];
+    llj1 = rc.ll[jl + 1];
+    lev = rc.lev[jl];
+    ema = rc.ema[jl];
+    first = rc.first[jl];
+    w = rc.w[jl];
+  } else {
+    const double* ll = a.linelocs + rc.lineloc_offset;
+    const LineRow* row = a.rows + rc.field;
+    llj = ll[j];
+    llj1 = ll[j + 1];
+    lev = row->level[j];
+    ema = row->ema_start[j];
+    first = row->first_px[j];
+    w = 1.0 / (((double)(j + 1) * rc.inl) - ((double)j * rc.inl));
+  }
   // value of the k=1 B-spline (scipy's de Boor order of operations)
-  const double t0 = (double)j * inl, t1 = (double)(j + 1) * inl;
-  const double w = 1.0 / (t1 - t0);
-  const double loc = ll[j] * (w * (t1 - x)) + ll[j + 1] * (w * (x - t0));
-  double level = row->level[j];
-  if (a.smoothing > 0) {
-    const long long t = p - row->first_px[j];
-    if (!(j == 0)) {
-      const double keep = 1.0 - 1.0 / (a.smoothing * W);
-      level = level + pow(keep, (double)(t + 1)) * (row->ema_start[j] - level);
-    }
+  const double t0 = (double)j * rc.inl, t1 = (double)(j + 1) * rc.inl;
+  const double loc = llj * (w * (t1 - x)) + llj1 * (w * (x - t0));
+  q.level = lev;
+  if (a.smoothing > 0 && j != 0) {
+    const double keep = 1.0 - 1.0 / (a.smoothing * W);
+    q.level = lev + pow(keep, (double)(p - first + 1)) * (ema - lev);
   }
   long long ci;
   int ph0;
   float alpha;
   if (a.coord_f32) {
     const float c = (float)loc;
-    ci = (long long)c;  // truncation toward zero, as int()
+    ci = fabsf(c) < 2.0e9f ? (long long)(int)c : (long long)c;  // truncation toward zero, as int()
     const float frac = c - (float)ci;
     const float pp = frac * 65536.0f;
     ph0 = (int)pp;
     alpha = pp - (float)ph0;
   } else {
-    ci = (long long)loc;
+    ci = fabs(loc) < 2.0e9 ? (long long)(int)loc : (long long)loc;
     const double frac = loc - (double)ci;
     const double pp = frac * 65536.0;
     ph0 = (int)pp;
     alpha = (float)(pp - (double)ph0);
   }
-  // negative coordinates have no LUT row (the reference reads out of bounds):
-  // use phase 0; their taps fall outside the buffer and read 0
+  // negative coordinates have no LUT row (the reference reads out of
+  // bounds): use phase 0; their taps fall outside the buffer and read 0
   if (ph0 < 0 || ph0 >= kSincPhases) {
     ph0 = 0;
     alpha = 0.0f;
   }
-  const float4* w0 = reinterpret_cast<const float4*>(a.lut + (size_t)ph0 * kSincTaps);
-  const float4* w1 = reinterpret_cast<const float4*>(a.lut + (size_t)(ph0 + 1) * kSincTaps);
-  const float* src = a.demod + fd.demod_offset;
-  const long long start = ci - (kSincTaps / 2 - 1);
-  double acc = 0.0;
+  q.alpha = alpha;
+  q.start = ci - (kSincTaps / 2 - 1);
+  // Weights ws = row ph0, we = row ph0+1.  In a folded table rows above
+  // 32768 are read as tap-reversed rows of the lower half: ws = reversed row
+  // 65536-ph0, we = reversed row 65535-ph0 (for ph0 = 65535 the last row
+  // repeats row 65535, i.e. ws).  The float4 groups are fetched in reversed
+  // group order here and their components reversed in pixel_finish.
+  const bool fold = a.lut_folded && ph0 > kSincPhases / 2 - 1;
+  const int b = fold ? kSincPhases - 1 - ph0 : ph0;
+  const uint32_t rowA = (uint32_t)b * (kSincTaps * 4u), rowB = rowA + kSincTaps * 4u;
+  const bool last = ph0 == kSincPhases - 1;
+  const uint32_t ou = fold ? rowB + 48u : rowA;
+  const uint32_t ov = fold ? (last ? rowB : rowA) + 48u : rowB;
+  const int step = fold ? -16 : 16;
 #pragma unroll
-  for (int g = 0; g < kSincTaps / 4; ++g) {
-    float4 u = w0[g], v = w1[g];
-    float ws[4] = {u.x, u.y, u.z, u.w};
-    float we[4] = {v.x, v.y, v.z, v.w};
+  for (int g = 0; g < 4; ++g) {
+    q.u[g] = lut_load(lut, ou + g * step);
+    q.v[g] = lut_load(lut, ov + g * step);
+  }
+  q.fold = fold;
+}
+
+__device__ __forceinline__ float comp(const float4& f, int c) {
+  return c == 0 ? f.x : (c == 1 ? f.y : (c == 2 ? f.z : f.w));
+}
+
+// 16-tap sum, level factor and output codes of pixel t.
+__device__ __forceinline__ void pixel_finish(const TbcPixelArgs& a, const LineRec& rc, const float* stage,
+                                             const float* src, long long slo, int nst, int t, const PixelPrep& q) {
+  const long long rel = q.start - slo;
+  float sv[kSincTaps];
+  if (rel >= 0 && rel + kSincTaps <= nst) {
+    const int r0 = (int)rel;
 #pragma unroll
-    for (int e = 0; e < 4; ++e) {
-      long long idx = start + 4 * g + e;
-      float s = (idx >= 0 && idx < fd.demod_len) ? src[idx] : 0.0f;
-      float wt = fmaf(alpha, we[e] - ws[e], ws[e]);
-      acc += (double)(s * wt);
+    for (int e = 0; e < kSincTaps; ++e) sv[e] = stage[r0 + e];
+  } else {
+#pragma unroll
+    for (int e = 0; e < kSincTaps; ++e) {
+      const long long idx = q.start + e;
+      sv[e] = (idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
     }
   }
-  const float r = (float)(level * acc);
-  const long long o = fd.out_offset + q;
+  double acc = 0.0;
+#pragma unroll
+  for (int e = 0; e < kSincTaps; ++e) {
+    const int g = e >> 2, c = e & 3;
+    const float ws = q.fold ? comp(q.u[g], 3 - c) : comp(q.u[g], c);
+    const float we = q.fold ? comp(q.v[g], 3 - c) : comp(q.v[g], c);
+    const float wt = fmaf(q.alpha, we - ws, ws);
+    acc += (double)(sv[e] * wt);
+  }
+  const float r = (float)(q.level * acc);
+  const long long o = rc.out_offset + t;
   if (a.out_f32) a.out_f32[o] = r;
   if (a.out_u16) {
     double vv = (double)r * a.lscale + a.loffset + 0.5;
     vv = vv < 0.0 ? 0.0 : (vv > 65535.0 ? 65535.0 : vv);
     a.out_u16[o] = (uint16_t)vv;
   }
+}
+
+// Source position of pixel p of a line (the spline value of pixel_prep).
+__device__ __forceinline__ double line_loc(const TbcPixelArgs& a, const LineRec& rc, long long p) {
+  const double x = (double)p * rc.outscale;
+  int j = (int)(x * rc.rinl);
+  if ((double)(j + 1) * rc.inl <= x) ++j;
+  if ((double)j * rc.inl > x) --j;
+  j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
+  const int jl = j - rc.jB;
+  double llj, llj1, w;
+  if (jl >= 0 && jl <= 2) {
+    llj = rc.ll[jl];
+    llj1 = rc.ll[jl + 1];
+    w = rc.w[jl];
+  } else {
+    const double* ll = a.linelocs + rc.lineloc_offset;
+    llj = ll[j];
+    llj1 = ll[j + 1];
+    w = 1.0 / (((double)(j + 1) * rc.inl) - ((double)j * rc.inl));
+  }
+  const double t0 = (double)j * rc.inl, t1 = (double)(j + 1) * rc.inl;
+  return llj * (w * (t1 - x)) + llj1 * (w * (x - t0));
+}
+
+// One workgroup per 256-pixel chunk of an output line (blockIdx.x = line *
+// chunks + chunk over the flat line records).  The chunk's source span
+// follows from the positions of its first and last pixel; the LUT loads of
+// every pixel and the staging loads of the span are in flight together, and
+// after one barrier every thread sums its taps from LDS.  Pixels whose taps
+// leave the span (decreasing line locations, spans over kChunkStage) read
+// them from global memory.
+__global__ __launch_bounds__(kTbcThreads) void k_tbc_pixels(TbcPixelArgs a) {
+  __shared__ float stage[kChunkStage];
+  const int line = blockIdx.x / a.chunks, chunk = blockIdx.x - line * a.chunks;
+  const LineRec& rc = a.recs[line];
+  const int W = rc.W;
+  const int c0 = chunk * kTbcThreads;
+  if (c0 >= W) return;
+  const int c1 = (W - c0 < kTbcThreads ? W : c0 + kTbcThreads) - 1;
+  const float* src = a.demod + rc.demod_offset;
+  const int tid = threadIdx.x;
+  const int t = c0 + tid;
+  const rsrc_t lut = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.lut), (short)0,
+                                                       (int)((kSincPhases + 1) * kSincTaps * 4), 0x00020000);
+  // staging span of the chunk (uniform)
+  const double la = line_loc(a, rc, rc.p0 + c0), lb = line_loc(a, rc, rc.p0 + c1);
+  const double lo = fmin(la, lb), hi = fmax(la, lb);
+  const bool finite = lo == lo && hi == hi && fabs(lo) < 1e15 && fabs(hi) < 1e15;
+  const long long slo = finite ? (long long)floor(lo) - (kSincTaps / 2 + 2) : 0;
+  const long long shi = finite ? (long long)floor(hi) + (kSincTaps / 2 + 4) : 0;
+  const int nst = (finite && shi - slo <= kChunkStage) ? (int)(shi - slo) : 0;
+  // LUT loads first: the in-order load counter then lets the barrier wait
+  // cover both the LUT rows and the staging loads issued after them
+  PixelPrep q;
+  if (t <= c1) pixel_prep(a, rc, lut, t, q);
+  float sreg[kStagePerThread];
+#pragma unroll
+  for (int k = 0; k < kStagePerThread; ++k) {
+    const int i = tid + k * kTbcThreads;
+    const long long idx = slo + i;
+    sreg[k] = (i < nst && idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
+  }
+#pragma unroll
+  for (int k = 0; k < kStagePerThread; ++k) {
+    const int i = tid + k * kTbcThreads;
+    if (i < nst) stage[i] = sreg[k];
+  }
+  __syncthreads();
+  if (t <= c1) pixel_finish(a, rc, stage, src, slo, nst, t, q);
 }
 
 // ------------------------------------------------------------- hsync refine
@@ -393,16 +586,36 @@ int ld_set_sinc_lut(ld_handle* h, const float* host_lut, int rows, int taps) {
   size_t bytes = sizeof(float) * (size_t)rows * taps;
   if (!h->sinc_lut) LD_HIP(hipMalloc(&h->sinc_lut, bytes));
   LD_HIP(hipMemcpy(h->sinc_lut, host_lut, bytes, hipMemcpyHostToDevice));
+  // A Kaiser-windowed sinc table is mirror-symmetric: the weights of phase
+  // p/65536 at distances (7-t)+p/65536 equal those of phase 1-p/65536 at the
+  // negated distances.  When every row holds that bitwise (and the last row
+  // repeats row 65535), the pixel kernel reads only rows 0..32768, which
+  // halves the table's cache footprint.
+  int folded = 1;
+  const uint32_t* u = reinterpret_cast<const uint32_t*>(host_lut);
+  for (int p = 1; p < kSincPhases && folded; ++p)
+    for (int t = 0; t < kSincTaps; ++t)
+      if (u[(size_t)p * kSincTaps + t] != u[(size_t)(kSincPhases - p) * kSincTaps + (kSincTaps - 1 - t)]) {
+        folded = 0;
+        break;
+      }
+  if (std::memcmp(u + (size_t)kSincPhases * kSincTaps, u + (size_t)(kSincPhases - 1) * kSincTaps,
+                  sizeof(float) * kSincTaps) != 0)
+    folded = 0;
+  h->sinc_folded = folded;
   return LD_OK;
 }
 
 static size_t tbc_fields_bytes(int nfields) {
-  return ((sizeof(ld_tbc_field) * (size_t)nfields + 255) / 256) * 256;
+  // field descriptors followed by the per-field line-record base indices
+  return ((sizeof(ld_tbc_field) * (size_t)nfields + sizeof(int) * ((size_t)nfields + 1) + 255) / 256) * 256;
 }
+constexpr int kMaxLinesOut = 1024;  // output lines per field
 
 size_t ld_tbc_workspace_bytes(const ld_handle* h, int nfields) {
   if (!h || nfields <= 0) return 0;
-  return sizeof(LineRow) * (size_t)nfields + tbc_fields_bytes(nfields);
+  return tbc_fields_bytes(nfields) + sizeof(LineRow) * (size_t)nfields +
+         sizeof(LineRec) * (size_t)nfields * kMaxLinesOut;
 }
 
 int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const ld_tbc_field* fields,
@@ -413,10 +626,10 @@ int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const l
     set_error("ld_tbc_fields: null argument or empty batch");
     return LD_EINVAL;
   }
-  int max_pixels_per_field = 0;
   for (int f = 0; f < nfields; ++f) {
     const ld_tbc_field& fd = fields[f];
-    if (fd.nlines < 2 || fd.nlines > kLinesMax || fd.outwidth <= 0 || fd.linesout <= 0 || fd.lineoffset < 0 ||
+    if (fd.nlines < 2 || fd.nlines > kLinesMax || fd.outwidth <= 0 || fd.outwidth > kTbcMaxWidth ||
+        fd.linesout <= 0 || fd.linesout > kMaxLinesOut || fd.lineoffset < 0 ||
         !(fd.inlinelen > 0) || fd.lineloc_offset < 0 || fd.lineloc_offset + fd.nlines > nlinelocs) {
       set_error("ld_tbc_fields: field %d: bad geometry (nlines %d, max %d) or line range", f, fd.nlines, kLinesMax);
       return LD_EINVAL;
@@ -430,7 +643,6 @@ int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const l
       set_error("ld_tbc_fields: field %d: output span outside the buffer", f);
       return LD_ERANGE;
     }
-    if (npx > max_pixels_per_field) max_pixels_per_field = (int)npx;
   }
   if (!out_f32 && !out_u16) {
     set_error("ld_tbc_fields: no output requested");
@@ -450,18 +662,30 @@ int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const l
   }
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
+  // field descriptors and each field's first line-record index, one upload
+  std::vector<char> hdr(tbc_fields_bytes(nfields), 0);
+  std::memcpy(hdr.data(), fields, sizeof(ld_tbc_field) * (size_t)nfields);
+  int* base = (int*)(hdr.data() + sizeof(ld_tbc_field) * (size_t)nfields);
+  base[0] = 0;
+  for (int f = 0; f < nfields; ++f) base[f + 1] = base[f] + fields[f].linesout;
+  const int total_lines = base[nfields];
   ld_tbc_field* dfields = (ld_tbc_field*)workspace;
+  const int* dbase = (const int*)((char*)workspace + sizeof(ld_tbc_field) * (size_t)nfields);
   LineRow* rows = (LineRow*)((char*)workspace + tbc_fields_bytes(nfields));
-  LD_HIP(hipMemcpyAsync(dfields, fields, sizeof(ld_tbc_field) * (size_t)nfields, hipMemcpyHostToDevice, st));
+  LineRec* recs = (LineRec*)((char*)rows + sizeof(LineRow) * (size_t)nfields);
+  // a pageable source is staged before hipMemcpyAsync returns, so the
+  // temporary host buffer may go out of scope
+  LD_HIP(hipMemcpyAsync(dfields, hdr.data(), hdr.size(), hipMemcpyHostToDevice, st));
   hipLaunchKernelGGL(k_tbc_lines, dim3(nfields), dim3(kLineThreads), 0, st, dfields, linelocs, rows, smoothing,
-                     threshold);
+                     threshold, recs, dbase);
   LD_HIP(hipGetLastError());
   TbcPixelArgs a;
+  a.recs = recs;
   a.demod = demod;
-  a.fields = dfields;
   a.linelocs = linelocs;
   a.rows = rows;
   a.lut = h->sinc_lut;
+  a.lut_folded = h->sinc_folded;
   a.out_f32 = out_f32;
   a.out_u16 = out_u16;
   a.smoothing = smoothing;
@@ -472,8 +696,10 @@ int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const l
   } else {
     a.lscale = a.loffset = 0.0;
   }
-  dim3 grid((max_pixels_per_field + 255) / 256, nfields);
-  hipLaunchKernelGGL(k_tbc_pixels, grid, dim3(256), 0, st, a);
+  int max_width = 0;
+  for (int f = 0; f < nfields; ++f) max_width = fields[f].outwidth > max_width ? fields[f].outwidth : max_width;
+  a.chunks = (max_width + kTbcThreads - 1) / kTbcThreads;
+  hipLaunchKernelGGL(k_tbc_pixels, dim3(total_lines * a.chunks), dim3(kTbcThreads), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

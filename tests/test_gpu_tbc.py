@@ -25,11 +25,12 @@ def _np(t):
     return t.detach().cpu().numpy()
 
 
-def _oracle_field(orf, demod, ll, lineoffset, coord_f32, smoothing=0.0):
+def _oracle_field(orf, demod, ll, lineoffset, coord_f32, smoothing=0.0, lut=None):
     sp = orf.SysParams
     W, lines = sp["outlinelen"], sp["frame_lines"] // 2 + 1
     locs, wow = T.computewow_scaled(ll, orf.linelen, W, lines, lineoffset)
-    ds = T.scale_field(np.asarray(demod, np.float32), locs, wow, T.build_kaiser_lut(), lineoffset, W, lines * W,
+    lut = T.build_kaiser_lut() if lut is None else lut
+    ds = T.scale_field(np.asarray(demod, np.float32), locs, wow, lut, lineoffset, W, lines * W,
                        smoothing=smoothing, coord_f32=coord_f32)
     dp = orf.DecoderParams
     u16 = T.hz_to_output_array(ds.astype(np.float32), dp["ire0"], dp["hz_ire"], sp["outputZero"], dp["vsync_ire"],
@@ -92,6 +93,30 @@ def test_tbc_smoothing_and_float_output(gpu_rf, oracle_rf):
         assert np.abs(ds - ds_o).max() <= 1.0
         u16 = _np(f.downscale(final=True)[0]).astype(np.int64)
         assert np.abs(u16 - u16_o.astype(np.int64)).max() <= 1
+
+
+def test_tbc_asymmetric_lut(gpu_rf, oracle_rf):
+    """A sinc table without the mirror symmetry (rows above 32768 perturbed)
+    is read in full: float32 output equals the oracle's with the same table,
+    bit for bit, in both coordinate modes.  The symmetric table is restored
+    afterwards."""
+    import torch
+    rf, orf = gpu_rf("NTSC", audio=False), oracle_rf("NTSC", audio=False)
+    rng = np.random.default_rng(21)
+    demod = (8.1e6 + rng.normal(0, 3e5, 800_000)).astype(np.float32)
+    ll = 5000 + np.arange(273) * 2542.3 + rng.uniform(-2, 2, 273)
+    lut = T.build_kaiser_lut()
+    lut[32769::7] *= np.float32(1.0 + 2.0 ** -8)
+    try:
+        fieldmod.upload_sinc_lut(rf, lut)
+        for coord_f32 in (True, False):
+            ds_o, _ = _oracle_field(orf, demod, ll, 0, coord_f32, lut=lut)
+            f = fieldmod.Field(rf, {"video": {"demod": torch.from_numpy(demod).cuda()}}, linelocs=ll,
+                               coord_f32=coord_f32)
+            ds = _np(f.downscale(final=False)[0])
+            np.testing.assert_array_equal(ds, ds_o.astype(np.float32))
+    finally:
+        fieldmod.upload_sinc_lut(rf)
 
 
 def test_batched_fields_equal_single(gpu_rf):
