@@ -17,6 +17,7 @@ Prints one JSON line.  value = usable RF samples demodulated per second
 import argparse
 import json
 import os
+import pathlib
 import sys
 import time
 
@@ -83,35 +84,165 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=False):
     W, lines = rf.SysParams["outlinelen"], rf.SysParams["frame_lines"] // 2 + 1
     db, tb, U = algorithmic_bytes(dec.nblocks, rf.block_stride, nfields, W * lines, NTSC_FIELD_SAMPLES, audio)
     return dict(wall_s=wall, step_ms=step_ms, demod_ms=demod_ms, hsync_ms=hsync_ms, tbc_ms=tbc_ms,
-                demod_bytes=db, tbc_bytes=tb, usable=U, nblocks=dec.nblocks,
+                demod_bytes=db, tbc_bytes=tb, usable=U, nblocks=dec.nblocks, fields=fields, ll1=ll1,
                 bad_lines=float(dec.linebad.float().mean().item()))
 
 
-def cpu_baseline(cap, nblocks, workers):
-    """FP64 numpy/scipy oracle on the host: `nblocks` blocks demodulated by a
-    thread pool (one block per task, as DemodCache's workers) plus the TBC of
-    one field.  Returns (Msamples/s, seconds)."""
+def cpu_baseline(cap, fields, ll1, nsample, workers, min_seconds, max_seconds):
+    """The same step on the host: the FP64 numpy/scipy oracle demodulates the
+    blocks of the first `nsample` fields of the batch on a thread pool (one
+    block per task, as DemodCache's workers), then refines every field's
+    hsync line locations and resamples it to uint16 (one field per task; the
+    resampler loop is the oracle's compiled C restatement, as the reference
+    runs it compiled with numba).
+    Passes over this fixed sample repeat until `min_seconds` have elapsed
+    (stopping early past `max_seconds`).  Returns (Msamples/s, seconds,
+    passes, blocks per pass)."""
     from concurrent.futures import ThreadPoolExecutor
 
+    from oracle import ctbc
     from oracle import dsp as O
     from oracle import tbc as T
+    from oracle.params import out_scale
+    ctbc.lib()
     orf = O.OracleRF("NTSC", decode_analog_audio=True)
     S, N = O.block_stride(orf), orf.blocklen
-    data = cap.data.astype(np.float64)
+    sp, dp = orf.SysParams, orf.DecoderParams
+    W, lines = sp["outlinelen"], sp["frame_lines"] // 2 + 1
+    vs = dp["vsync_ire"]
+    osc = out_scale("NTSC", vs)
     lut = T.build_kaiser_lut()
+    nl = fields[0]["nlines"]
+    plan = []
+    for f in range(nsample):
+        ll = np.asarray(ll1[f * nl:(f + 1) * nl])
+        plan.append((fields[f]["demod_offset"], ll))
+    b0 = plan[0][0] // S
+    b1 = max((org + int(ll.max()) + 3000) // S + 1 for org, ll in plan)
+    data = cap.data[b0 * S: b1 * S + N].astype(np.float64)
+    blocks = list(range(b1 - b0 + 1))
 
-    def one(b):
-        return orf.demodblock(data=data[b * S:b * S + N], cut=True)
+    def demod(b):
+        r = orf.demodblock(data=data[b * S:b * S + N], cut=True)
+        return r["video"]["demod"], r["video"]["demod_05"]
 
+    def tbc(args):
+        off, ll, demod_all, d05_all = args
+        ll2, _ = T.refine_linelocs_hsync(d05_all[off:], ll, np.zeros(len(ll), bool), "NTSC", 40.0,
+                                         orf.iretohz, vs)
+        locs, wow = T.computewow_scaled(ll2, orf.linelen, W, lines, 0)
+        ds = ctbc.scale_field(demod_all[off:], locs, wow, lut, 0, W, lines * W)
+        return T.hz_to_output_array(ds.astype(np.float32), dp["ire0"], dp["hz_ire"], sp["outputZero"], vs, osc)
+
+    passes = 0
     t0 = time.perf_counter()
     with ThreadPoolExecutor(max_workers=workers) as ex:
-        outs = list(ex.map(one, range(nblocks)))
-    demod = np.concatenate([o["video"]["demod"] for o in outs])
-    ll = 2000 + np.arange(273) * 2542.22          # one field's line map
-    locs, wow = T.computewow_scaled(ll, orf.linelen, 910, 263, 0)
-    T.scale_field(demod.astype(np.float32), locs, wow, lut, 0, 910, 263 * 910)
-    dt = time.perf_counter() - t0
-    return nblocks * S / dt / 1e6, dt
+        while True:
+            outs = list(ex.map(demod, blocks))
+            demod_all = np.concatenate([o[0] for o in outs]).astype(np.float32)
+            d05_all = np.concatenate([o[1] for o in outs]).astype(np.float32)
+            list(ex.map(tbc, [(org - b0 * S, ll, demod_all, d05_all) for org, ll in plan]))
+            passes += 1
+            dt = time.perf_counter() - t0
+            if dt >= min_seconds or dt >= max_seconds:
+                break
+    return passes * len(blocks) * S / dt / 1e6, dt, passes, len(blocks)
+
+
+TRAFFIC_FILE = pathlib.Path(__file__).resolve().parent / "profiles" / "traffic_ntsc64.json"
+COPY_ELEMS = 64 << 20          # calibration copy: 256 MiB read + 256 MiB written
+
+
+def profile_step(args):
+    """Profiling target: the batched step only, then one plain copy kernel of
+    known size (torch.neg over COPY_ELEMS float32) that calibrates the HBM
+    byte counters."""
+    __graft_entry__.load_package()
+    import torch
+    from lddecode_amd import capture, synth
+    from lddecode_amd.rfdecode import RFDecode
+    rf = RFDecode(system="NTSC", decode_analog_audio=44100, device=0)
+    nf = args.fields
+    cap = synth.make_capture("NTSC", (nf + 2) * NTSC_FIELD_SAMPLES + 100_000, seed=args.seed, snr_db=40.0)
+    fields, ll1 = capture.truth_fields(rf, cap, nf, first_field=1)
+    dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).to(rf.device), fields, ll1, audio=True)
+    for _ in range(args.warmup + args.steps):
+        dec.demod()
+        dec.refine()
+        dec.tbc()
+    torch.cuda.synchronize()
+    x = torch.ones(COPY_ELEMS, dtype=torch.float32, device=rf.device)
+    y = torch.empty_like(x)
+    torch.neg(x, out=y)
+    torch.cuda.synchronize()
+    print(f"# profile step done: {dec.nblocks} blocks", file=sys.stderr, flush=True)
+
+
+def _pmc_pass(counter, args, workdir):
+    """One rocprofv3 counter pass over `bench.py --profile-step` (a child
+    process); returns {kernel key: [values per dispatch]}."""
+    import sqlite3
+    import subprocess
+    out = pathlib.Path(workdir) / counter.lower()
+    cmd = ["rocprofv3", "--pmc", counter, "-d", str(out), "-o", "run", "--", sys.executable, "-u",
+           str(pathlib.Path(__file__).resolve()), "--profile-step", "--fields", str(args.fields),
+           "--steps", "2", "--warmup", "1"]
+    print("# " + " ".join(cmd), file=sys.stderr, flush=True)
+    env = dict(os.environ, TMPDIR="/tmp")
+    subprocess.run(["timeout", "-k", "10", "300"] + cmd, check=True, cwd="/tmp", env=env)
+    dbs = sorted(out.rglob("*_results.db"))
+    if not dbs:
+        raise RuntimeError(f"rocprofv3 wrote no results under {out}")
+    con = sqlite3.connect(str(dbs[0]))
+    rows = con.execute("select dispatch_id, kernel_name, grid_size, sum(value) from counters_collection "
+                       "where counter_name = ? group by dispatch_id", (counter,)).fetchall()
+    got = {}
+    for _, name, grid, v in rows:
+        short = name.split("(")[0].replace("void ", "")
+        if "neg_kernel" in name:
+            short = "copy"
+        got.setdefault((short, int(grid)), []).append(float(v))
+    return got
+
+
+def measure_traffic(args):
+    """HBM traffic per launch of the step's kernels: FETCH_SIZE and
+    WRITE_SIZE (KiB per dispatch) from two counter passes, each scaled by the
+    ratio of known bytes to the counter's reading for the calibration copy."""
+    import tempfile
+    work = tempfile.mkdtemp(prefix="ld_traffic_", dir="/tmp")
+    fetch = _pmc_pass("FETCH_SIZE", args, work)
+    write = _pmc_pass("WRITE_SIZE", args, work)
+    copy_kib = COPY_ELEMS * 4 / 1024
+    fc = max(v for k, vs in fetch.items() if k[0] == "copy" for v in vs)
+    wc = max(v for k, vs in write.items() if k[0] == "copy" for v in vs)
+    fscale, wscale = copy_kib / fc, copy_kib / wc
+    kernels = {}
+    for key in sorted(set(fetch) & set(write)):
+        if not key[0].startswith("ldgpu::"):
+            continue
+        f = float(np.mean(fetch[key])) * fscale * 1024
+        w = float(np.mean(write[key])) * wscale * 1024
+        kernels[f"{key[0]}@{key[1]}"] = {"kernel": key[0], "grid_threads": key[1], "dispatches": len(fetch[key]),
+                                         "fetch_bytes": round(f), "write_bytes": round(w),
+                                         "traffic_bytes": round(f + w)}
+    res = {"fields": args.fields, "copy_bytes_each_way": COPY_ELEMS * 4,
+           "copy_fetch_kib": fc, "copy_write_kib": wc, "fetch_scale": fscale, "write_scale": wscale,
+           "command": f"python bench.py --measure-traffic --fields {args.fields}", "kernels": kernels}
+    dest = pathlib.Path(args.traffic_out) if args.traffic_out else TRAFFIC_FILE
+    dest.parent.mkdir(parents=True, exist_ok=True)
+    dest.write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res, indent=1))
+
+
+def load_traffic(kernel, grid_threads):
+    """Measured HBM bytes per launch for a kernel of the batched step, or None."""
+    try:
+        res = json.loads(TRAFFIC_FILE.read_text())
+    except (OSError, ValueError):
+        return None
+    k = res.get("kernels", {}).get(f"{kernel}@{grid_threads}")
+    return None if k is None else k["traffic_bytes"]
 
 
 def main():
@@ -121,8 +252,17 @@ def main():
     ap.add_argument("--warmup", type=int, default=3)
     ap.add_argument("--fields", type=int, default=64, help="NTSC fields per batched step")
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--cpu-blocks", type=int, default=64, help="blocks in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-fields", type=int, default=2, help="fields in the CPU baseline sample (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=10.0, help="minimum CPU baseline time")
+    ap.add_argument("--profile-step", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--measure-traffic", action="store_true",
+                    help="run the two rocprofv3 counter passes and write " + str(TRAFFIC_FILE.name))
+    ap.add_argument("--traffic-out", default=None, help="where --measure-traffic writes its JSON")
     args = ap.parse_args()
+    if args.profile_step:
+        return profile_step(args)
+    if args.measure_traffic:
+        return measure_traffic(args)
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py measures one GPU: this build has no multi-GPU path (captures shard across GPUs "
               "with no communication; run one process per GPU on separate captures).", file=sys.stderr)
@@ -150,13 +290,17 @@ def main():
     tbc_gbs = batched["tbc_bytes"] / (batched["tbc_ms"] / 1e3) / 1e9
     path_gbs = (batched["demod_bytes"] + batched["tbc_bytes"]) / (batched["step_ms"] / 1e3) / 1e9
 
+    demod_traffic = load_traffic("ldgpu::k_demod_block", batched["nblocks"] * 512)
+    tbc_traffic = load_traffic("ldgpu::k_tbc_pixels", nf * (rf.SysParams["frame_lines"] // 2 + 1) * 4 * 256)
     cpu = None
-    if args.cpu_blocks > 0:
+    if args.cpu_fields > 0:
         workers = min(os.cpu_count() or 1, 16)
-        rate, secs = cpu_baseline(cap, args.cpu_blocks, workers)
+        rate, secs, passes, nblk = cpu_baseline(cap, batched["fields"], batched["ll1"], args.cpu_fields, workers,
+                                                args.cpu_seconds, 3 * args.cpu_seconds)
         cpu = {"value": round(rate, 3), "unit": "Msamples/s", "cores": workers, "kind": "port",
-               "sample": f"FP64 numpy/scipy oracle: {args.cpu_blocks} NTSC blocks demodulated on a "
-                         f"{workers}-thread pool + TBC of one field, {secs:.1f} s wall"}
+               "sample": f"FP64 numpy/scipy oracle on a {workers}-thread pool, the same step on the first "
+                         f"{args.cpu_fields} fields of the batch ({nblk} blocks demodulated, hsync refine, "
+                         f"TBC to uint16 per field), {passes} passes in {secs:.1f} s wall"}
 
     line = {
         "metric": METRIC,
@@ -186,9 +330,12 @@ def main():
         "stages_ms": {"demod": round(batched["demod_ms"], 4), "hsync": round(batched["hsync_ms"], 4),
                       "tbc": round(batched["tbc_ms"], 4)},
         "roofline": {"bound": "hbm", "achieved": round(demod_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(demod_gbs / HBM_PEAK_GBS, 4), "traffic": None,
-                     "kernel": "ld_demod_blocks launch sequence (22.25 B per usable sample)"},
-        "roofline_tbc": {"achieved": round(tbc_gbs, 1), "frac": round(tbc_gbs / HBM_PEAK_GBS, 4)},
+                     "frac": round(demod_gbs / HBM_PEAK_GBS, 4), "traffic": demod_traffic,
+                     "algorithmic_bytes": batched["demod_bytes"],
+                     "kernel": "k_demod_block, one launch per step (22.25 B per usable sample)"},
+        "roofline_tbc": {"achieved": round(tbc_gbs, 1), "frac": round(tbc_gbs / HBM_PEAK_GBS, 4),
+                         "traffic": tbc_traffic, "algorithmic_bytes": batched["tbc_bytes"],
+                         "kernel": "k_tbc_lines + k_tbc_pixels (traffic: k_tbc_pixels)"},
         "roofline_path": {"achieved": round(path_gbs, 1), "frac": round(path_gbs / HBM_PEAK_GBS, 4)},
         "bad_line_fraction": round(batched["bad_lines"], 4),
         "cpu_baseline": cpu,
