@@ -90,6 +90,15 @@ int ld_set_audio(ld_handle* h, int channel, const double* host_reim, int nbins, 
 int ld_set_demod_params(ld_handle* h, double fs_hz, double clip_lo, double clip_hi,
                         double dc_offset, int video_rot);
 
+/* PAL LD-V4300D spur notch (core.py:774-798, extra_options
+ * "PAL_V4300D_NotchFilter"): within spectrum bins [lo, hi) of every block,
+ * bins whose magnitude exceeds the window's mean + 3 standard deviations are
+ * zeroed together with both neighbours before the analytic-signal and audio
+ * stage-1 filters (the rfhpf output keeps the unnotched spectrum).  The
+ * reference's window is [int(N*8.42/f_MHz), int(1 + N*8.6/f_MHz)).  lo = hi = 0
+ * disables it; otherwise 1 <= lo < hi < N/2 and hi - lo <= 508. */
+int ld_set_notch(ld_handle* h, int lo, int hi);
+
 /* Output channels of ld_demod_blocks.  A NULL pointer skips that channel.
  * Block b writes its samples at ptr + b * video_stride (video and rfhpf) or
  * ptr + b * audio_stride (audio). */

@@ -20,7 +20,7 @@ ABI_VERSION = 1
 EXPORTS = (
     "ld_abi_version", "ld_last_error", "ld_create", "ld_destroy", "ld_set_filter", "ld_set_audio",
     "ld_set_demod_params", "ld_demod_workspace_bytes", "ld_demod_blocks", "ld_set_sinc_lut",
-    "ld_tbc_workspace_bytes", "ld_tbc_fields", "ld_hsync_workspace_bytes", "ld_refine_hsync",
+    "ld_tbc_workspace_bytes", "ld_tbc_fields", "ld_hsync_workspace_bytes", "ld_refine_hsync", "ld_set_notch",
 )
 
 
@@ -75,6 +75,7 @@ def _declare(lib):
         "ld_set_filter": (c_int, [c_vp, c_int, P(c_dbl), c_int, c_dbl, c_dbl, c_int]),
         "ld_set_audio": (c_int, [c_vp, c_int, P(c_dbl), c_int, c_int, c_dbl, c_dbl]),
         "ld_set_demod_params": (c_int, [c_vp, c_dbl, c_dbl, c_dbl, c_dbl, c_int]),
+        "ld_set_notch": (c_int, [c_vp, c_int, c_int]),
         "ld_demod_workspace_bytes": (c_sz, [c_vp, c_int]),
         "ld_demod_blocks": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
                                     P(DemodOutputs), c_vp, c_sz, c_vp]),

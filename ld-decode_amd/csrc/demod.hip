@@ -127,7 +127,10 @@ struct DemodArgs {
   int acut, alen;
   int64_t astride;
   float fs, clip_lo, clip_hi, dc;
+  int notch_lo, notch_hi;  // spur notch window, empty: off
 };
+
+constexpr int kNotchMax = 508;  // bins in the notch window
 
 constexpr uint32_t kHalfBytes = (kM + 1) * 8;  // a half-spectrum table
 constexpr uint32_t kStrideB = kT * 8;           // bytes between a thread's elements m and m+1
@@ -259,6 +262,52 @@ __device__ void audio_wave(const DemodArgs& a, int c, const float2* __restrict__
   }
 }
 
+// PAL LD-V4300D spur notch (core.py:774-798), decision, by one wavefront:
+// from the magnitudes of window bins [lo, hi) in mag, the window's mean +
+// 3 std (population, in double) flags the bins above it; flag[i] belongs to
+// bin lo - 2 + i (two zero guards each side).  Bins that are flagged or next
+// to a flagged bin are zeroed in the analytic-signal and audio inputs (their
+// mirror bins are implied by the half-spectrum form).
+__device__ __forceinline__ void notch_decide(const float* mag, int* flag, double* red, int lo, int hi) {
+  int lane = threadIdx.x, n = hi - lo;
+  // opaque: keeps the compiler from hoisting this rarely used work into the
+  // kernel prologue, where its registers would be held (and spilled)
+  asm volatile("" : "+v"(lane), "+v"(n));
+  double sum = 0.0;
+  for (int i = lane; i < n; i += 64) sum += mag[i];
+  red[lane] = sum;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  sum = 0.0;
+  for (int l = 0; l < 64; ++l) sum += red[l];
+  const double mean = sum / (double)n;
+  double var = 0.0;
+  for (int i = lane; i < n; i += 64) {
+    const double d = mag[i] - mean;
+    var += d * d;
+  }
+  __builtin_amdgcn_wave_barrier();
+  red[64 + lane] = var;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup");
+  var = 0.0;
+  for (int l = 0; l < 64; ++l) var += red[64 + l];
+  const double thr = mean + 3.0 * sqrt(var / (double)n);
+  for (int i = lane; i < n + 4; i += 64) {
+    const int w = i - 2;
+    flag[i] = (w >= 0 && w < n) ? ((double)mag[w] > thr) : 0;
+  }
+}
+
+// Whether spectrum bin k is zeroed by the notch (flags from notch_decide).
+__device__ __forceinline__ bool notch_zero(const int* flag, int lo, int hi, int k) {
+  if (k < lo - 1 || k > hi) return false;
+  const int i = k - lo + 2;
+  return (flag[i - 1] | flag[i] | flag[i + 1]) != 0;
+}
+
 // ---------------------------------------------------------------- kernel
 // One workgroup demodulates one block end to end (RFDecode.demodblock_cpu,
 // core.py:755-879); only the input and the requested outputs touch HBM
@@ -277,6 +326,10 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
   __shared__ Lds s;
   __shared__ float2 aslice[2][kNA / 2 + 1];
   __shared__ TwLds twl;
+  __shared__ float notch_mag[kNotchMax];
+  __shared__ int notch_flag[kNotchMax + 4];
+  __shared__ double notch_red[128];
+  __shared__ uint32_t notch_mask[kT];  // bit m: bin tid + T m zeroed by the spur notch
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int64_t s0 = a.first + a.step * b;
@@ -333,10 +386,12 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
       // (- Z[M] folded into bin 0)
       const float2 hM = cscale(bload2(hrv, 0u, kM * 8u), xM);
       const bool odd = (ph == kPhOdd);
+      const uint32_t zmask = a.notch_hi > a.notch_lo ? notch_mask[tidv] : 0u;
       static_for<0, kP>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const int k = tid + kT * m;
         float2 z = cmul(x[m], bload2(hrv, vo, m * kStrideB));
+        if (zmask & (1u << m)) z = make_float2(0.0f, 0.0f);
         if (k == 0)
           z = odd ? csub(z, hM) : cadd(z, hM);
         else if (odd)
@@ -348,7 +403,7 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
       // The workspace (even) and v (odd) hold conj(z): arg(a b*) of the
       // conjugates is minus the wanted increment.  Odd samples go to LDS
       // shifted by one so each thread reads z[2j-1] at its own index j.
-      const LdsPair sh(s.re, s.im, pidx(tid + 1)), own(s.re, s.im, pidx(tid));
+      const LdsPair sh(s.re, s.im, pidx(tidv + 1)), own(s.re, s.im, pidx(tidv));
       static_for<0, kP>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         sh.template st<POff<kT * m>::value>(v[m]);
@@ -386,13 +441,31 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
     if (ph == kPhIn || ph == kPhDemod) {
       r2c_split(v, x, xM, s, wt);
       if (ph == kPhIn) {
+        if (a.notch_hi > a.notch_lo) {  // PAL spur notch decision (core.py:774-798)
+          static_for<0, kP>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            const int k = tid + kT * m;
+            if (k >= a.notch_lo && k < a.notch_hi)
+              notch_mag[k - a.notch_lo] = sqrtf(x[m].x * x[m].x + x[m].y * x[m].y);
+          });
+          __syncthreads();
+          if (tid < 64) notch_decide(notch_mag, notch_flag, notch_red, a.notch_lo, a.notch_hi);
+          __syncthreads();
+          uint32_t zm = 0;
+          static_for<0, kP>([&](auto mc) {
+            constexpr int m = decltype(mc)::value;
+            if (notch_zero(notch_flag, a.notch_lo, a.notch_hi, tid + kT * m)) zm |= 1u << m;
+          });
+          notch_mask[tid] = zm;
+        }
+        const uint32_t zmask = a.notch_hi > a.notch_lo ? notch_mask[tid] : 0u;
         static_for<0, 2>([&](auto cc) {
           constexpr int c = decltype(cc)::value;
           if (a.out_audio[c] != nullptr) {
 #pragma unroll
             for (int m = 0; m < kP; ++m) {
               int i = tidv + kT * m - a.lowbin[c];
-              if (i >= 0 && i <= kNA / 2) aslice[c][i] = x[m];
+              if (i >= 0 && i <= kNA / 2) aslice[c][i] = (zmask & (1u << m)) ? make_float2(0.0f, 0.0f) : x[m];
             }
           }
         });
@@ -563,6 +636,21 @@ int ld_set_demod_params(ld_handle* h, double fs, double clip_lo, double clip_hi,
   return LD_OK;
 }
 
+int ld_set_notch(ld_handle* h, int lo, int hi) {
+  if (!h) {
+    set_error("ld_set_notch: null handle");
+    return LD_EINVAL;
+  }
+  if (!(lo == 0 && hi == 0) && !(lo >= 1 && lo < hi && hi < h->blocklen / 2 && hi - lo <= kNotchMax)) {
+    set_error("ld_set_notch: window [%d, %d) outside [1, %d) or wider than %d bins", lo, hi, h->blocklen / 2,
+              kNotchMax);
+    return LD_EINVAL;
+  }
+  h->notch_lo = lo;
+  h->notch_hi = hi;
+  return LD_OK;
+}
+
 size_t ld_demod_workspace_bytes(const ld_handle* h, int nblocks) {
   if (!h || nblocks <= 0) return 0;
   // even analytic samples of every block (M float2)
@@ -660,6 +748,8 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.clip_lo = (float)h->clip_lo;
   a.clip_hi = (float)h->clip_hi;
   a.dc = (float)h->dc_offset;
+  a.notch_lo = h->notch_lo;
+  a.notch_hi = h->notch_hi;
   a.in_bytes = dtype == LD_IN_INT16 ? 2 : 4;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));

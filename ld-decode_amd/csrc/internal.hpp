@@ -13,7 +13,8 @@ struct ld_handle {
   int cut_end = 32;          // RFDecode.blockcut_end    core.py:402 (F05_offset)
   float2* twiddle = nullptr; // exp(-2*pi*i*t/N), t < N
   float* sinc_lut = nullptr; // (65536+1) x 16 float32
-  int sinc_folded = 0;        // row p == row (65536-p) reversed for 0 < p < 65536
+  int sinc_folded = 0;
+  int notch_lo = 0, notch_hi = 0;  // spur notch window (bins), lo == hi: off        // row p == row (65536-p) reversed for 0 < p < 65536
   float2* filt[LD_NFILT] = {};
   double filt_dc[LD_NFILT] = {};
   int filt_shift[LD_NFILT] = {};

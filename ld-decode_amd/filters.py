@@ -91,7 +91,7 @@ class FilterBank:
     ``FVideoBurst``, ``FVideoPilot``) plus per-channel audio stage-1 data.
     """
 
-    def __init__(self, system, blocklen, freq_hz, sysp, decp, analog_audio=True):
+    def __init__(self, system, blocklen, freq_hz, sysp, decp, analog_audio=True, color_notch=False):
         self.system = system
         self.blocklen = n = blocklen
         self.freq_hz = fs = freq_hz
@@ -119,6 +119,11 @@ class FilterBank:
                 rfv = rfv * freq_response(*notch, n)
         F["RFVideo"] = rfv * analytic_mask(n)
         F["Fvideo_lpf"] = freq_response(*sps.butter(decp["video_lpf_order"], decp["video_lpf_freq"] / nyq, "low"), n)
+        if system == "NTSC" and color_notch:
+            # NTSC_ColorNotchFilter (core.py:625-631): 3rd-order band-stop from
+            # the video LPF corner to 5 MHz, folded into every post-demod filter
+            stop = sps.butter(3, [decp["video_lpf_freq"] / 1e6 / nyq_mhz, 5.0 / nyq_mhz], "bandstop")
+            F["Fvideo_lpf"] = F["Fvideo_lpf"] * freq_response(*stop, n)
         t1, t2 = decp["video_deemp"]
         F["Fdeemp"] = freq_response(*emphasis(t1, t2, fs), n)
         F["Femp"] = freq_response(*emphasis(t2, t1, fs), n)

@@ -67,10 +67,9 @@ class RFDecode:
         self.device = _device(torch, device)
         if decode_digital_audio:
             raise NotImplementedError("EFM output (decode_digital_audio) is not part of the GPU path")
-        if extra_options.get("PAL_V4300D_NotchFilter", False):
-            raise NotImplementedError("PAL_V4300D_NotchFilter is not part of the GPU path")
-        if extra_options.get("NTSC_ColorNotchFilter", False):
-            raise NotImplementedError("NTSC_ColorNotchFilter is not part of the GPU path")
+        # core.py:328-329
+        self.NTSC_ColorNotchFilter = bool(extra_options.get("NTSC_ColorNotchFilter", False))
+        self.PAL_V4300D_NotchFilter = bool(extra_options.get("PAL_V4300D_NotchFilter", False))
         # core.py:319-403
         self.blocklen = blocklen
         self.blockcut = 1024
@@ -114,6 +113,10 @@ class RFDecode:
         h = ctypes.c_void_p()
         _lib.check(self.lib.ld_create(self.device.index or 0, blocklen, ctypes.byref(h)))
         self.handle = h
+        if self.system == "PAL" and self.PAL_V4300D_NotchFilter:
+            # the spur window of core.py:785-788, applied per block in the kernel
+            lo, hi = int(blocklen * (8.42 / self.freq)), int(1 + (blocklen * (8.6 / self.freq)))
+            _lib.check(self.lib.ld_set_notch(h, lo, hi))
         self._ws = None
         self._mtf_uploaded = None
         self.downscale_sinc_lut = None
@@ -165,7 +168,7 @@ class RFDecode:
         """core.py:405-420: (re)build every table, upload, recalibrate delays."""
         self.setupcount += 1
         bank = FilterBank(self.system, self.blocklen, self.freq_hz, self.SysParams, self.DecoderParams,
-                          analog_audio=bool(self.decode_analog_audio))
+                          analog_audio=bool(self.decode_analog_audio), color_notch=self.NTSC_ColorNotchFilter)
         self.filterbank = bank
         self.Filters = dict(bank.filters)
         self.Filters["F05_offset"] = bank.offset05

@@ -144,9 +144,11 @@ class OracleRF:
     def __init__(self, system="NTSC", inputfreq=40, blocklen=BLOCKSIZE,
                  decode_analog_audio=True, has_analog_audio=True,
                  mtf_mult=1.0, mtf_offset=0.0, deemp_str=1.0,
-                 decoder_params_override=None):
+                 decoder_params_override=None, pal_v4300d_notch=False, ntsc_color_notch=False):
         # core.py:319-403
         self.system = system
+        self.PAL_V4300D_NotchFilter = pal_v4300d_notch     # extra_options, core.py:329
+        self.NTSC_ColorNotchFilter = ntsc_color_notch      # extra_options, core.py:328
         self.blocklen = blocklen
         self.blockcut = 1024
         self.blockcut_end = 0
@@ -246,6 +248,9 @@ class OracleRF:
         F["hilbert"] = build_hilbert(N)
         F["RFVideo"] = F["RFVideo"] * F["hilbert"]
         F["Fvideo_lpf"] = filtfft(sps.butter(DP["video_lpf_order"], DP["video_lpf_freq"] / self.freq_hz_half, "low"), N)
+        if self.system == "NTSC" and self.NTSC_ColorNotchFilter:   # core.py:625-631
+            notch = sps.butter(3, [DP["video_lpf_freq"] / 1000000 / self.freq_half, 5.0 / self.freq_half], "bandstop")
+            F["Fvideo_lpf"] = F["Fvideo_lpf"] * filtfft(notch, N)
         d1, d2 = DP["video_deemp"]
         F["Fdeemp"] = filtfft(emphasis_iir(d1, d2, self.freq_hz), N)
         F["Femp"] = filtfft(emphasis_iir(d2, d1, self.freq_hz), N)
@@ -299,6 +304,8 @@ class OracleRF:
         rot = self.delays["video_rot"] if getattr(self, "delays", None) is not None else 0
         hpf = sfft.ifft(X * F["Frfhpf"]).real
         rv["rfhpf"] = hpf[self.blockcut - rot: -self.blockcut_end - rot]
+        if self.system == "PAL" and self.PAL_V4300D_NotchFilter:
+            X = v4300d_notch(X, N, self.freq)
         Xf = X * F["RFVideo"]
         if mtf_level != 0:
             Xf = Xf * F["MTF"] ** mtf_level
@@ -365,6 +372,26 @@ class OracleRF:
         self.limits = {"sync": (np.min(vraw[1400:2800]), np.max(vraw[1400:2800])),
                        "viewable": (np.min(vraw[2900:6000]), np.max(vraw[2900:6000]))}
         return dec
+
+
+def v4300d_notch_window(blocklen, freq_mhz):
+    """core.py:785-788: the spectrum bins searched for the LD-V4300D spur."""
+    return int(blocklen * (8.42 / freq_mhz)), int(1 + (blocklen * (8.6 / freq_mhz)))
+
+
+def v4300d_notch(X, blocklen, freq_mhz):
+    """core.py:774-798 on a copy of the full spectrum X: bins of the window
+    whose magnitude exceeds mean + 3 std (population) of the window are
+    zeroed with both neighbours, and so are their mirror bins."""
+    X = np.array(X, copy=True)
+    lo, hi = v4300d_notch_window(blocklen, freq_mhz)
+    mag = np.sqrt(X[lo:hi].real ** 2 + X[lo:hi].imag ** 2)
+    m = np.mean(mag) + np.std(mag) * 3
+    for i in np.where(mag > m)[0]:
+        for k in (i - 1 + lo, i + lo, i + 1 + lo):
+            X[k] = 0
+            X[blocklen - k] = 0
+    return X
 
 
 # --------------------------------------------------------------------------

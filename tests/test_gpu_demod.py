@@ -52,6 +52,40 @@ def test_demodblock_matches_oracle(gpu_rf, oracle_rf, system, seed, cut, mtf):
         assert np.abs(g - r).max() <= AUDIO_TOL_HZ, (k, np.abs(g - r).max())
 
 
+def test_pal_v4300d_notch_matches_oracle(torch_cuda, oracle_rf):
+    """PAL_V4300D_NotchFilter (core.py:774-798): a PAL block carrying a strong
+    bin-centred spur at 8.4998 MHz; the GPU's per-block notch decision and
+    zeroing match the oracle on every output (the spur moves the unnotched
+    demod by far more than the tolerance, so the notch is exercised)."""
+    from lddecode_amd.rfdecode import RFDecode
+    rf = RFDecode(system="PAL", decode_analog_audio=44100, extra_options={"PAL_V4300D_NotchFilter": True}, device=0)
+    orf = O.OracleRF("PAL", decode_analog_audio=True, pal_v4300d_notch=True)
+    cap = synth.make_capture("PAL", 32768, seed=5, snr_db=36.0)
+    n = np.arange(32768)
+    x = (cap.data.astype(np.float64) + 0.3 * np.abs(cap.data).max() * np.cos(2 * np.pi * 6963 * n / 32768))
+    x = x.astype(np.float32)
+    got = rf.demodblock(data=x, cut=True)
+    ref = orf.demodblock(data=x.astype(np.float64), cut=True)
+    _check_video(got["video"], ref["video"], rf.video_fields)
+    assert np.abs(_np(got["rfhpf"]) - ref["rfhpf"]).max() <= RFHPF_TOL
+    for k in ("audio_left", "audio_right"):
+        assert np.abs(_np(got["audio"][k]) - ref["audio"][k]).max() <= AUDIO_TOL_HZ, k
+    ref0 = oracle_rf("PAL").demodblock(data=x.astype(np.float64), cut=True)
+    assert np.abs(ref0["video"]["demod"] - ref["video"]["demod"]).max() > 100 * VIDEO_TOL_HZ
+
+
+def test_ntsc_color_notch_matches_oracle(torch_cuda):
+    """NTSC_ColorNotchFilter (core.py:625-631) folded into the post-demod
+    filters: GPU outputs equal the oracle's with the same option."""
+    from lddecode_amd.rfdecode import RFDecode
+    rf = RFDecode(system="NTSC", decode_analog_audio=0, extra_options={"NTSC_ColorNotchFilter": True}, device=0)
+    orf = O.OracleRF("NTSC", decode_analog_audio=False, ntsc_color_notch=True)
+    cap = synth.make_capture("NTSC", 32768, seed=6, snr_db=40.0)
+    got = rf.demodblock(data=cap.data, cut=True)
+    ref = orf.demodblock(data=cap.data.astype(np.float64), cut=True)
+    _check_video(got["video"], ref["video"], rf.video_fields)
+
+
 def test_float32_input_and_delays(gpu_rf, oracle_rf):
     """computedelays runs the GPU demod on a float signal; its delays equal the
     oracle's (video_rot exactly)."""

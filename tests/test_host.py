@@ -45,6 +45,16 @@ def test_filterbank_matches_oracle(system):
         np.testing.assert_allclose(a["filt1"], b["filt1"], atol=1e-12)
 
 
+def test_filterbank_color_notch_matches_oracle():
+    orf = O.OracleRF("NTSC", ntsc_color_notch=True)
+    sp, dp = params.system_tables("NTSC")
+    for k in ("ire0", "hz_ire", "vsync_ire"):
+        dp[k] = sp[k]
+    bank = filters.FilterBank("NTSC", 32768, 40e6, sp, dp, color_notch=True)
+    for name in ("Fvideo_lpf", "FVideo", "FVideo05", "FVideoBurst"):
+        np.testing.assert_allclose(bank.filters[name], orf.Filters[name], rtol=0, atol=1e-12, err_msg=name)
+
+
 def test_sinc_lut_builder_matches_oracle():
     np.testing.assert_array_equal(fieldmod.build_sinc_lut(), T.build_kaiser_lut())
 
@@ -96,6 +106,7 @@ def test_abi_argument_errors_without_device():
     assert lib.ld_demod_workspace_bytes(None, 4) == 0
     with pytest.raises(_lib.LdGpuError):
         _lib.check(lib.ld_set_filter(None, 0, None, 0, 1.0, 0.0, 0))
+    assert lib.ld_set_notch(None, 6897, 7046) == _lib.LD_EINVAL
 
 
 def test_missing_library_fails_loudly(tmp_path):

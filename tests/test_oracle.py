@@ -236,6 +236,54 @@ def test_c_resampler_equals_numpy_oracle(coord_f32, smoothing):
     np.testing.assert_array_equal(a, b)
 
 
+def test_v4300d_notch_window():
+    """core.py:785-788 at 40 MSPS, N = 32768: int(6897.664) and int(1 + 7045.12)."""
+    assert O.v4300d_notch_window(32768, 40) == (6897, 7046)
+    assert O.v4300d_notch_window(65536, 40) == (13795, 14091)
+
+
+def test_v4300d_notch_closed_forms():
+    """core.py:774-798 on constructed spectra: a window of equal magnitudes
+    (std 0) zeroes nothing; one 100x spur among 148 unit bins (mean 1.66,
+    std 8.06: threshold 25.9) zeroes exactly that bin, both neighbours and
+    their three mirror bins; a spur on the window's first bin zeroes the bin
+    just below the window too."""
+    N = 32768
+    lo, hi = O.v4300d_notch_window(N, 40)
+    rng = np.random.default_rng(3)
+    X = np.exp(2j * np.pi * rng.random(N))
+    np.testing.assert_array_equal(O.v4300d_notch(X, N, 40), X)
+    for spur in (lo + 70, lo):
+        Y = X.copy()
+        Y[spur] *= 100
+        Z = O.v4300d_notch(Y, N, 40)
+        zero = {spur - 1, spur, spur + 1} | {N - k for k in (spur - 1, spur, spur + 1)}
+        assert set(np.flatnonzero(Z == 0)) == zero
+        keep = np.ones(N, bool)
+        keep[list(zero)] = False
+        np.testing.assert_array_equal(Z[keep], Y[keep])
+
+
+def test_ntsc_color_notch_filter():
+    """NTSC_ColorNotchFilter (core.py:625-631): a 3rd-order Butterworth band
+    stop from the video LPF corner to 5 MHz multiplies Fvideo_lpf: unity at
+    DC, a null inside the band, and every post-demod filter changes by the
+    same factor."""
+    a = O.OracleRF("NTSC", decode_analog_audio=False)
+    b = O.OracleRF("NTSC", decode_analog_audio=False, ntsc_color_notch=True)
+    N = a.blocklen
+    r = b.Filters["Fvideo_lpf"][: N // 2] / a.Filters["Fvideo_lpf"][: N // 2]
+    f = np.arange(N // 2) * 40e6 / N
+    lo = a.DecoderParams["video_lpf_freq"]
+    assert abs(r[0] - 1) < 1e-12
+    assert np.abs(r[(f > lo) & (f < 5e6)]).min() < 1e-3
+    assert np.abs(np.abs(r[f < 1e6]) - 1).max() < 1e-2
+    for k in ("FVideo05", "FVideoBurst"):
+        q = b.Filters[k][: N // 2] / a.Filters[k][: N // 2]
+        ok = np.abs(a.Filters[k][: N // 2]) > 1e-9
+        np.testing.assert_allclose(q[ok], r[ok], rtol=1e-9)
+
+
 def test_hz_to_output_closed_form():
     sp = sys_params("NTSC")
     osc = out_scale("NTSC", -40)
