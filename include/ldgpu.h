@@ -196,6 +196,71 @@ int ld_refine_hsync(ld_handle* h, const float* demod_05, int64_t demod_total,
                     double lo_hz, double hi_hz, int pal, double* linelocs2, uint8_t* linebad_out,
                     void* workspace, size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Line-location refinement after the hsync refine (FieldNTSC.process
+ * core.py:3472-3495, FieldPAL.process core.py:3212-3237).
+ * ------------------------------------------------------------------- */
+
+/* One field of a refinement batch.  Line locations of the field are entries
+ * [lineloc_offset, +nlines) of the DEVICE line arrays, in samples relative
+ * to the field's demod buffers at demod_offset (length demod_len).
+ * linecount is Field.linecount while the refinement runs (263 / 262 for a
+ * first / second field, core.py:2340); first_field is isFirstField. */
+typedef struct ld_line_field {
+  int64_t demod_offset;
+  int64_t demod_len;
+  int64_t lineloc_offset;
+  int32_t nlines;
+  int32_t linecount;
+  int32_t first_field;
+  int32_t reserved;
+} ld_line_field;
+
+/* The RFDecode / SysParams constants the refinements read: input rate in
+ * MHz (rf.freq), nominal samples per line (rf.linelen), fsc_mhz, pilot_mhz
+ * and DecoderParams hz_ire. */
+typedef struct ld_line_params {
+  double freq_mhz;
+  double linelen;
+  double fsc_mhz;
+  double pilot_mhz;
+  double hz_ire;
+} ld_line_params;
+
+/* Device workspace of ld_refine_burst_ntsc / ld_refine_pilot_pal. */
+size_t ld_refine_workspace_bytes(const ld_handle* h, int nfields, int64_t nlinelocs);
+
+/* NTSC colour-burst line alignment for consecutive fields: linebad |=
+ * compute_deriv_error(linelocs2) (core.py:1647), refine_linelocs_burst
+ * (core.py:3391-3440 with compute_burst_offsets :3361-3388,
+ * compute_line_bursts :2999-3047, clb_findbursts utils.py:1339-1368),
+ * fix_badlines with linelocs2 as backup (:2566-2589) and the 117.25 degree
+ * subcarrier shift (:3491-3495).  Each field's phase-adjust search starts
+ * from the previous field's median (core.py:3366-3369); the first field of
+ * the batch from prev_phase_adjust (0 for none).  `fields` is a HOST array;
+ * demod and demod_burst are the DEVICE video channels; linelocs2 /
+ * linebad_in are the hsync refine's DEVICE outputs.  Outputs (DEVICE):
+ * linelocs_out and linebad_out (Field.linelocs / linebad after process),
+ * phase_adjust_out[f] (phase_adjust_median, NaN without a valid burst) and
+ * field_phase_id[f] (fieldPhaseID).  nlines must be 267..512. */
+int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_burst, int64_t demod_total,
+                         const ld_line_field* fields, int nfields, const ld_line_params* params,
+                         const double* linelocs2, const uint8_t* linebad_in, int64_t nlinelocs,
+                         double prev_phase_adjust, double* linelocs_out, uint8_t* linebad_out,
+                         double* phase_adjust_out, int32_t* field_phase_id, void* workspace,
+                         size_t workspace_bytes, void* stream);
+
+/* PAL pilot line alignment: linebad |= compute_deriv_error(linelocs2), two
+ * refine_linelocs_pilot passes (core.py:3054-3093, :3221-3224) and
+ * fix_badlines (:3226) for every field of the batch (fields independent).
+ * Arguments as ld_refine_burst_ntsc; demod_pilot is the DEVICE pilot
+ * channel.  nlines must be 323..512. */
+int ld_refine_pilot_pal(ld_handle* h, const float* demod_pilot, int64_t demod_total,
+                        const ld_line_field* fields, int nfields, const ld_line_params* params,
+                        const double* linelocs2, const uint8_t* linebad_in, int64_t nlinelocs,
+                        double* linelocs_out, uint8_t* linebad_out, void* workspace,
+                        size_t workspace_bytes, void* stream);
+
 #ifdef __cplusplus
 }
 #endif

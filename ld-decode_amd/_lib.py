@@ -21,6 +21,7 @@ EXPORTS = (
     "ld_abi_version", "ld_last_error", "ld_create", "ld_destroy", "ld_set_filter", "ld_set_audio",
     "ld_set_demod_params", "ld_demod_workspace_bytes", "ld_demod_blocks", "ld_set_sinc_lut",
     "ld_tbc_workspace_bytes", "ld_tbc_fields", "ld_hsync_workspace_bytes", "ld_refine_hsync", "ld_set_notch",
+    "ld_refine_workspace_bytes", "ld_refine_burst_ntsc", "ld_refine_pilot_pal",
 )
 
 
@@ -60,6 +61,22 @@ class LevelParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("ire0", "hz_ire", "output_zero", "vsync_ire", "out_scale")]
 
 
+class LineField(ctypes.Structure):
+    _fields_ = [
+        ("demod_offset", ctypes.c_int64),
+        ("demod_len", ctypes.c_int64),
+        ("lineloc_offset", ctypes.c_int64),
+        ("nlines", ctypes.c_int32),
+        ("linecount", ctypes.c_int32),
+        ("first_field", ctypes.c_int32),
+        ("reserved", ctypes.c_int32),
+    ]
+
+
+class LineParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("freq_mhz", "linelen", "fsc_mhz", "pilot_mhz", "hz_ire")]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -86,6 +103,11 @@ def _declare(lib):
         "ld_hsync_workspace_bytes": (c_sz, [c_vp, c_int]),
         "ld_refine_hsync": (c_int, [c_vp, c_vp, c_i64, P(TbcField), c_int, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_dbl,
                                     c_dbl, c_int, c_vp, c_vp, c_vp, c_sz, c_vp]),
+        "ld_refine_workspace_bytes": (c_sz, [c_vp, c_int, c_i64]),
+        "ld_refine_burst_ntsc": (c_int, [c_vp, c_vp, c_vp, c_i64, P(LineField), c_int, P(LineParams), c_vp, c_vp,
+                                         c_i64, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+        "ld_refine_pilot_pal": (c_int, [c_vp, c_vp, c_i64, P(LineField), c_int, P(LineParams), c_vp, c_vp, c_i64,
+                                        c_vp, c_vp, c_vp, c_sz, c_vp]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

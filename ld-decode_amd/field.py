@@ -151,6 +151,61 @@ def compute_deriv_error(linelocs, baserr):
     return np.asarray(baserr, bool) | d1 | d2
 
 
+def line_params(rf):
+    """ld_line_params of an RFDecode: rf.freq, rf.linelen, fsc/pilot MHz, hz_ire."""
+    sp = rf.SysParams
+    return _lib.LineParams(float(rf.freq), float(rf.linelen), float(sp["fsc_mhz"]),
+                           float(sp.get("pilot_mhz", sp["fsc_mhz"])), float(rf.DecoderParams["hz_ire"]))
+
+
+def _line_fields(fields):
+    arr = (_lib.LineField * len(fields))()
+    for i, f in enumerate(fields):
+        for k in ("demod_offset", "demod_len", "lineloc_offset", "nlines", "linecount", "first_field"):
+            setattr(arr[i], k, int(f[k]))
+    return arr
+
+
+def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust=0.0, stream=None):
+    """Burst (NTSC) or pilot (PAL) line alignment + fix_badlines (+ NTSC
+    subcarrier shift) for a batch of fields (FieldNTSC.process
+    core.py:3472-3495, FieldPAL.process core.py:3212-3237).
+
+    video: dict of DEVICE float32 channels (``demod``/``demod_burst`` for
+    NTSC, ``demod_pilot`` for PAL); fields: dicts with demod_offset,
+    demod_len, lineloc_offset, nlines, linecount (263/262 for a first/second
+    field, as compute_linelocs sets it; the pilot pass does not read it) and
+    first_field;
+    linelocs2 / linebad: the hsync refine's outputs for the whole batch.
+    Returns (linelocs float64, linebad uint8, phase_adjust_median per field or
+    None, fieldPhaseID per field or None) as device tensors."""
+    torch = rf.torch
+    n = linelocs2.numel()
+    ll2 = linelocs2.to(rf.device, torch.float64).contiguous()
+    bad = linebad.to(rf.device, torch.uint8).contiguous()
+    ll = torch.empty(n, dtype=torch.float64, device=rf.device)
+    bad_out = torch.empty(n, dtype=torch.uint8, device=rf.device)
+    wsb = rf.lib.ld_refine_workspace_bytes(rf.handle, len(fields), n)
+    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=rf.device)
+    lf = _line_fields(fields)
+    lp = line_params(rf)
+    st = rf._stream() if stream is None else stream
+    if rf.system == "NTSC":
+        demod, burst = video["demod"], video["demod_burst"]
+        phase = torch.empty(len(fields), dtype=torch.float64, device=rf.device)
+        fpid = torch.empty(len(fields), dtype=torch.int32, device=rf.device)
+        _lib.check(rf.lib.ld_refine_burst_ntsc(
+            rf.handle, demod.data_ptr(), burst.data_ptr(), demod.numel(), lf, len(fields), ctypes.byref(lp),
+            ll2.data_ptr(), bad.data_ptr(), n, float(prev_phase_adjust), ll.data_ptr(), bad_out.data_ptr(),
+            phase.data_ptr(), fpid.data_ptr(), ws.data_ptr(), ws.numel(), st))
+        return ll, bad_out, phase, fpid
+    pilot = video["demod_pilot"]
+    _lib.check(rf.lib.ld_refine_pilot_pal(
+        rf.handle, pilot.data_ptr(), pilot.numel(), lf, len(fields), ctypes.byref(lp), ll2.data_ptr(),
+        bad.data_ptr(), n, ll.data_ptr(), bad_out.data_ptr(), ws.data_ptr(), ws.numel(), st))
+    return ll, bad_out, None, None
+
+
 class Field:
     """One field's TBC state (the video half of core.py:1587-2722).
 
@@ -182,6 +237,33 @@ class Field:
         self.linelocs2 = ll2
         self.linebad = bad2
         return ll2
+
+    def process_lines(self, linecount, first_field, prev_phase_adjust=0.0):
+        """The rest of FieldNTSC/FieldPAL.process after the hsync refine:
+        burst (NTSC) or pilot (PAL) alignment, fix_badlines and the NTSC
+        subcarrier shift.  Sets and returns self.linelocs; sets linebad,
+        and for NTSC phase_adjust_median and fieldPhaseID."""
+        torch = self.rf.torch
+        ll2 = self.linelocs2 if getattr(self, "linelocs2", None) is not None else self.linelocs
+        if not isinstance(ll2, torch.Tensor):
+            ll2 = torch.as_tensor(np.asarray(ll2, dtype=np.float64))
+        ll2 = ll2.to(self.rf.device, torch.float64)
+        bad = self.linebad
+        if bad is None:
+            bad = torch.zeros(ll2.numel(), dtype=torch.uint8)
+        elif not isinstance(bad, torch.Tensor):
+            bad = torch.as_tensor(np.asarray(bad, dtype=np.uint8))
+        video = self.data["video"]
+        n = video["demod"].numel()
+        fd = dict(demod_offset=0, demod_len=n, lineloc_offset=0, nlines=ll2.numel(), linecount=linecount,
+                  first_field=int(bool(first_field)))
+        ll, bad2, phase, fpid = refine_lines_fields(self.rf, video, [fd], ll2, bad, prev_phase_adjust)
+        self.linelocs = ll
+        self.linebad = bad2.bool()
+        if phase is not None:
+            self.phase_adjust_median = float(phase[0].item())
+            self.fieldPhaseID = int(fpid[0].item())
+        return ll
 
     def downscale(self, lineinfo=None, linesout=None, outwidth=None, channel="demod", audio=0,
                   final=False, lastfieldwritten=None):

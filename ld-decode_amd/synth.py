@@ -50,14 +50,16 @@ def _field_layout(system):
     return 625, 5, 5, 15
 
 
-def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0):
+def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0, pilot_ire=0.0):
     """IRE value of every sample plus the line-start table.
 
     Half-line h (global index) starts at h * H/2 * fs plus the jitter of its
     full line.  Within a field (hpf half-lines), half-lines [0, neq) carry
     equalising pulses, [neq, neq+nvs) vertical sync pulses, [neq+nvs, 3*neq)
     equalising pulses, then every other half-line starts a video line with an
-    hsync pulse, burst and colour bars."""
+    hsync pulse, burst and colour bars.  pilot_ire > 0 adds the PAL disc
+    pilot (pilot_mhz, continuous phase) of that peak amplitude on every
+    hsync pulse."""
     hpf, neq, nvs, first = _field_layout(system)
     H = sysp["line_period"] * 1e-6 * fs          # samples per line
     hh = H / 2
@@ -85,6 +87,9 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
     ire[vline & (t_us < sysp["hsyncPulseUS"])] = sync
     fsc = sysp["fsc_mhz"]
     t_abs_us = n / fs * 1e6
+    if pilot_ire > 0:
+        tip = vline & (t_us < sysp["hsyncPulseUS"])
+        ire[tip] += pilot_ire * np.sin(2 * np.pi * sysp["pilot_mhz"] * t_abs_us[tip])
     b0, b1 = sysp["colorBurstUS"]
     burst = video & (tl >= b0) & (tl < b1)
     burst_phase = np.pi if system == "NTSC" else 3 * np.pi / 4
@@ -106,12 +111,16 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
 
 
 def make_capture(system="NTSC", nsamples=700_000, seed=1, snr_db=40.0, jitter_ns=50.0,
-                 audio=True, fs=FS, start_halfline=0):
-    """Generate one capture.  Returns :class:`Capture`."""
+                 audio=True, fs=FS, start_halfline=0, pilot_ire=None):
+    """Generate one capture.  Returns :class:`Capture`.  pilot_ire: peak
+    amplitude of the PAL sync-tip pilot (default 10 IRE for PAL, none for
+    NTSC)."""
     rng = np.random.default_rng(seed)
     sysp, decp = system_tables(system)
     jitter = jitter_ns * 1e-9 * fs
-    ire, line_starts, hstarts = ire_raster(system, sysp, nsamples, fs, jitter, rng, start_halfline)
+    if pilot_ire is None:
+        pilot_ire = 10.0 if system == "PAL" else 0.0
+    ire, line_starts, hstarts = ire_raster(system, sysp, nsamples, fs, jitter, rng, start_halfline, pilot_ire)
     hz = sysp["ire0"] + sysp["hz_ire"] * ire
     # pre-emphasis: Fvideo_lpf * Femp applied in the frequency domain over the
     # whole capture (the reference applies the same product per block)
