@@ -27,11 +27,17 @@ class CaptureDecoder:
 
     data: int16 (or float32) device tensor holding the capture.
     fields: list of dicts with demod_offset (stitched index of the field
-    buffer origin), demod_len, and nlines; linelocs1: per-field line
-    locations relative to each origin, concatenated (float64, host or device).
+    buffer origin), demod_len, nlines and, for the burst / pilot alignment,
+    linecount and first_field; linelocs1: per-field line locations relative
+    to each origin, concatenated (float64, host or device).  lines=True runs
+    the burst (NTSC) / pilot (PAL) alignment, fix_badlines and the NTSC
+    subcarrier shift between the hsync refine and the TBC, as
+    FieldNTSC/FieldPAL.process do; prev_phase_adjust seeds the first NTSC
+    field's burst search.
     """
 
-    def __init__(self, rf, data, fields, linelocs1, lineoffset=0, coord_f32=True, smoothing=0.0, audio=False):
+    def __init__(self, rf, data, fields, linelocs1, lineoffset=0, coord_f32=True, smoothing=0.0, audio=False,
+                 lines=True, prev_phase_adjust=0.0):
         torch = rf.torch
         self.rf = rf
         self.data = data
@@ -71,6 +77,20 @@ class CaptureDecoder:
         assert self.linelocs1.numel() == lo, "line count mismatch"
         self.linelocs2 = torch.empty_like(self.linelocs1)
         self.linebad = torch.empty(lo, dtype=torch.uint8, device=dev)
+        self.lines_on = bool(lines)
+        self.prev_phase_adjust = prev_phase_adjust
+        if self.lines_on:
+            for fd in self.fields:
+                fd.setdefault("first_field", 1)
+                fd.setdefault("linecount", 263 if fd["first_field"] else 262)
+            self.linelocs = torch.empty_like(self.linelocs1)
+            self.linebad_final = torch.empty(lo, dtype=torch.uint8, device=dev)
+            self.phase_adjust = torch.empty(len(self.fields), dtype=torch.float64, device=dev)
+            self.field_phase_id = torch.empty(len(self.fields), dtype=torch.int32, device=dev)
+            self.ln_ws = torch.empty(max(1, rf.lib.ld_refine_workspace_bytes(rf.handle, len(self.fields), lo)),
+                                     dtype=torch.uint8, device=dev)
+        else:
+            self.linelocs = self.linelocs2
         self.out_u16 = torch.empty(self.npixels, dtype=torch.uint16, device=dev)
         self.hs_ws = torch.empty(max(1, rf.lib.ld_hsync_workspace_bytes(rf.handle, len(self.fields))),
                                  dtype=torch.uint8, device=dev)
@@ -89,8 +109,18 @@ class CaptureDecoder:
             self.rf, self.buffers["demod_05"], self.fields, self.linelocs1, out=(self.linelocs2, self.linebad),
             workspace=self.hs_ws)
 
+    def lines(self):
+        """Burst / pilot alignment + fix_badlines (+ NTSC shift) of every field."""
+        if not self.lines_on:
+            return self.linelocs
+        fieldmod.refine_lines_fields(self.rf, self.buffers, self.fields, self.linelocs2, self.linebad,
+                                     self.prev_phase_adjust,
+                                     out=(self.linelocs, self.linebad_final, self.phase_adjust, self.field_phase_id),
+                                     workspace=self.ln_ws)
+        return self.linelocs
+
     def tbc(self):
-        return fieldmod.tbc_fields(self.rf, self.buffers["demod"], self.fields, self.linelocs2,
+        return fieldmod.tbc_fields(self.rf, self.buffers["demod"], self.fields, self.linelocs,
                                    coord_f32=self.coord_f32, smoothing=self.smoothing, out_u16=self.out_u16,
                                    final=True, workspace=self.tbc_ws)
 
@@ -98,6 +128,7 @@ class CaptureDecoder:
         """One pass of the hot path over the whole capture (no host sync)."""
         self.demod(mtf_level)
         self.refine()
+        self.lines()
         self.tbc()
         return self.out_u16
 
@@ -122,6 +153,59 @@ def truth_fields(rf, cap, nfields, first_field=0, margin_lines=2, delay=13.0):
         raw = hs[idx]
         begin = int(hs[idx[0] - 2 * margin_lines]) - rf.blockcut
         org = field_origin(rf, max(begin, 0))
-        fields.append(dict(demod_offset=org, nlines=proclines))
+        is_first = (f % 2) == 0
+        fields.append(dict(demod_offset=org, nlines=proclines, first_field=int(is_first),
+                           linecount=263 if is_first else 262))
         lls.append(np.round(raw - (org + rf.blockcut) + delay))
     return fields, np.concatenate(lls)
+
+
+class PipelinedDecoder:
+    """Two CaptureDecoder buffer sets on two streams, so that consecutive
+    passes over a capture (or consecutive batches) overlap: pass k's demod
+    and hsync refine run on stream A while pass k-1's line alignment and
+    TBC run on stream B.  The NTSC burst alignment walks the fields in
+    order in one workgroup; overlapped this way it occupies one CU while
+    the next demod fills the others.  Both streams have the default
+    priority.  Arguments as CaptureDecoder; `step()` returns the uint16
+    output of the pass it queued (valid once stream B reaches it)."""
+
+    def __init__(self, rf, data, fields, linelocs1, **kw):
+        torch = rf.torch
+        self.rf = rf
+        self.torch = torch
+        self.sets = [CaptureDecoder(rf, data, fields, linelocs1, **kw) for _ in range(2)]
+        self.nblocks = self.sets[0].nblocks
+        self.stream_a = torch.cuda.Stream(device=rf.device)
+        self.stream_b = torch.cuda.Stream(device=rf.device)
+        self.free = [None, None]    # event: the set's previous TBC has finished
+        self.k = 0
+
+    def step(self, mtf_level=0):
+        torch = self.torch
+        i = self.k % 2
+        d = self.sets[i]
+        cur = torch.cuda.current_stream(self.rf.device)
+        with torch.cuda.stream(self.stream_a):
+            self.stream_a.wait_stream(cur)
+            if self.free[i] is not None:
+                self.stream_a.wait_event(self.free[i])
+            d.demod(mtf_level)
+            d.refine()
+            ready = torch.cuda.Event()
+            ready.record(self.stream_a)
+        with torch.cuda.stream(self.stream_b):
+            self.stream_b.wait_event(ready)
+            d.lines()
+            d.tbc()
+            done = torch.cuda.Event()
+            done.record(self.stream_b)
+        self.free[i] = done
+        self.k += 1
+        return d.out_u16
+
+    def join(self):
+        """Make torch's current stream wait for everything queued so far."""
+        cur = self.torch.cuda.current_stream(self.rf.device)
+        cur.wait_stream(self.stream_a)
+        cur.wait_stream(self.stream_b)

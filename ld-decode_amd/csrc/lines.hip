@@ -35,13 +35,26 @@ constexpr int kPilotLines = 323;  // refine_linelocs_pilot line range
 constexpr int kRefThreads = 512;  // lines per field handled by a workgroup
 constexpr int kZc = 16;           // clb_findbursts slots
 
+// Per (field, line) burst data.  The two clb_findbursts passes depend on
+// their starting phase_adjust p0 only through which integer each slot's
+// cycle position rounds to, so the result is a step function of p0 whose
+// steps sit where some base[k] + p0 + 0.5 is an integer, and it moves by q
+// when p0 moves by an integer q (the rounded integers all move by q; the
+// rising count then flips when q is odd).  k_burst_scan tabulates one
+// period: bp[] are the 16 step positions in [0, 1) sorted, tab[i] / rc[i]
+// the final phase_adjust and rising count on the i-th interval between them.
 struct BurstLine {
-  double base[kZc];  // (bstart + zcs[k] - s_rem) / zcburstdiv, all 16 slots
-  double adj;        // phase_adjust / 2 (k_burst_chain)
-  int count;         // zero crossings found
-  int rising;        // bit k: isrising[k]
-  int valid;         // compute_line_bursts returned a value
-  int rise;          // its `rising` result (k_burst_chain)
+  double base[kZc];    // (bstart + zcs[k] - s_rem) / zcburstdiv, all 16 slots
+  double bp[kZc];      // sorted step positions in [0, 1)
+  double tab[kZc + 1]; // phase_adjust after both passes, per interval
+  double adj;          // phase_adjust / 2 (k_burst_chain)
+  double minbase;      // smallest base[k]
+  unsigned char rc[kZc + 1];
+  unsigned char pad_[7];
+  int count;           // zero crossings found
+  int rising;          // bit k: isrising[k]
+  int valid;           // compute_line_bursts returned a value
+  int rise;            // its `rising` result (k_burst_chain)
 };
 
 struct FieldState {
@@ -69,8 +82,25 @@ __device__ double nb_mean_f32(F x, int n) {
   return (double)c / (double)n;
 }
 
+// The same over a 16-byte aligned LDS array: four values per read, the
+// additions still strictly in order.
+__device__ __forceinline__ double nb_mean_f32_lds(const float* x, int n) {
+  float c = 0.0f;
+  int i = 0;
+#pragma unroll 4
+  for (; i + 4 <= n; i += 4) {
+    const float4 q = *reinterpret_cast<const float4*>(x + i);
+    c = c + q.x;
+    c = c + q.y;
+    c = c + q.z;
+    c = c + q.w;
+  }
+  for (; i < n; ++i) c = c + x[i];
+  return (double)c / (double)n;
+}
+
 // numpy's pairwise float32 summation (8 accumulators per block of at most
-// 128, halves rounded down to a multiple of 8 above that).
+// 128 elements, halves rounded down to a multiple of 8 above that).
 template <class F>
 __device__ float np_block_sum(F x, int off, int n) {
   if (n < 8) {
@@ -91,67 +121,23 @@ __device__ float np_block_sum(F x, int off, int n) {
   return res;
 }
 
+// numpy's recursion for n <= 256 (two levels at most).
 template <class F>
 __device__ float np_pairwise_sum(F x, int n) {
   if (n <= 128) return np_block_sum(x, 0, n);
-  // explicit stack over the recursion of numpy's pairwise_sum
-  int so[24], sn[24], state[24];
-  float part[24];
-  int sp = 0;
-  so[0] = 0;
-  sn[0] = n;
-  state[0] = 0;
-  float ret = 0.0f;
-  while (sp >= 0) {
-    const int o = so[sp], m = sn[sp];
-    if (m <= 128) {
-      const float v = np_block_sum(x, o, m);
-      --sp;
-      if (sp < 0) {
-        ret = v;
-        break;
-      }
-      if (state[sp] == 1) {
-        part[sp] = v;
-        state[sp] = 2;
-        int n2 = sn[sp] / 2;
-        n2 -= n2 % 8;
-        ++sp;
-        so[sp] = so[sp - 1] + n2;
-        sn[sp] = sn[sp - 1] - n2;
-        state[sp] = 0;
-      } else {
-        // second half done: combine and pop until a parent needs its second half
-        float acc = part[sp] + v;
-        --sp;
-        while (sp >= 0 && state[sp] == 2) {
-          acc = part[sp] + acc;
-          --sp;
-        }
-        if (sp < 0) {
-          ret = acc;
-          break;
-        }
-        part[sp] = acc;
-        state[sp] = 2;
-        int n2 = sn[sp] / 2;
-        n2 -= n2 % 8;
-        ++sp;
-        so[sp] = so[sp - 1] + n2;
-        sn[sp] = sn[sp - 1] - n2;
-        state[sp] = 0;
-      }
-    } else {
-      int n2 = m / 2;
-      n2 -= n2 % 8;
-      state[sp] = 1;
-      ++sp;
-      so[sp] = o;
-      sn[sp] = n2;
-      state[sp] = 0;
-    }
+  int n2 = n / 2;
+  n2 -= n2 % 8;
+  const int n3 = n - n2;
+  const float left = np_block_sum(x, 0, n2);
+  float right;
+  if (n3 <= 128) {
+    right = np_block_sum(x, n2, n3);
+  } else {
+    int m2 = n3 / 2;
+    m2 -= m2 % 8;
+    right = np_block_sum(x, n2, m2) + np_block_sum(x, n2 + m2, n3 - m2);
   }
-  return ret;
+  return left + right;
 }
 
 // np.mean of a float32 array: pairwise float32 sum, float32 division.
@@ -211,33 +197,6 @@ __device__ __forceinline__ void sort16(double (&v)[kZc]) {
   }
 }
 
-// Median of the valid entries of val[0, n) (flag[i] != 0) held in LDS,
-// numpy's definition (mean of the two middle values for an even count);
-// NaN when none is valid.  All threads of the block call it.
-__device__ double block_median(const double* val, const int* flag, int n, int* cnt, double* mid) {
-  const int t = threadIdx.x;
-  if (t == 0) *cnt = 0;
-  __syncthreads();
-  if (t < n && flag[t]) atomicAdd(cnt, 1);
-  __syncthreads();
-  const int m = *cnt;
-  if (t < n && flag[t]) {
-    const double v = val[t];
-    int r = 0;
-    for (int j = 0; j < n; ++j) {
-      if (!flag[j]) continue;
-      const double w = val[j];
-      r += (w < v) || (w == v && j < t);
-    }
-    if (r == (m - 1) / 2) mid[0] = v;
-    if (r == m / 2) mid[1] = v;
-  }
-  __syncthreads();
-  const double med = m == 0 ? (double)NAN : ((m % 2) ? mid[1] : (mid[0] + mid[1]) / 2.0);
-  __syncthreads();
-  return med;
-}
-
 // Field.compute_deriv_error (core.py:2553-2564) of line l.
 __device__ __forceinline__ bool deriv_error(const double* ll, int l, int n) {
   bool e = false;
@@ -283,86 +242,45 @@ struct RefineArgs {
 };
 
 // ---------------------------------------------------------------- NTSC
-__global__ __launch_bounds__(64) void k_burst_scan(RefineArgs a, const float* __restrict__ demod,
-                                                   const float* __restrict__ dburst) {
-  const int l = blockIdx.x * 64 + threadIdx.x;
-  const int f = blockIdx.y;
-  if (l >= kBurstLines) return;
-  const ld_line_field fd = a.fields[f];
-  const double* ll = a.ll2 + fd.lineloc_offset;
-  BurstLine& out = a.lines[(size_t)f * kBurstLines + l];
-  out.valid = 0;
-  out.count = 0;
-  out.rising = 0;
+// Burst area of line l (compute_line_bursts, core.py:2999-3047): on return
+// s0 is its first sample and n its length (0: no area), bstart / s_rem /
+// zcburstdiv as the reference computes them.
+struct BurstArea {
+  long long s0, bstart;
+  int n;
+  double s_rem, zcburstdiv;
+};
+
+__device__ __forceinline__ bool burst_area(const RefineArgs& a, const ld_line_field& fd, const double* ll, int l,
+                                           BurstArea& ba) {
   const double loc = ll[l];
-  if (!(fabs(loc) < 9.0e15)) return;
+  if (!(fabs(loc) < 9.0e15)) return false;
   const long long s = (long long)loc;
-  const double s_rem = loc - (double)s;
+  ba.s_rem = loc - (double)s;
   const double lfreq = a.spl * get_linelen(ll, l, fd.linecount, a.linelen);
   const double fsc_inv = 1 / a.fsc_mhz;
-  const long long bstart = (long long)((21 * fsc_inv) * lfreq);
+  ba.bstart = (long long)((21 * fsc_inv) * lfreq);
   const long long bend = (long long)((28 * fsc_inv) * lfreq);
-  long long s0, s1;
-  np_slice(s + bstart, s + bend, fd.demod_len, s0, s1);
-  const int n = (int)(s1 - s0);
-  if (n == 0) return;
-  const float* pb = dburst + fd.demod_offset + s0;
-  const float* pd = demod + fd.demod_offset + s0;
-  const float mb = (float)nb_mean_f32([&](int i) { return pb[i]; }, n);
-  auto bf = [&](int i) { return pb[i] - mb; };
-  const float m2 = np_mean_f32(bf, n);
-  const float thr = sqrtf(np_mean_f32([&](int i) {
-    const float d = bf(i) - m2;
-    return d * d;
-  }, n));
-  const float md = (float)nb_mean_f32([&](int i) { return pd[i]; }, n);
-  float amax = 0.0f;
-  for (int i = 0; i < n; ++i) amax = fmaxf(amax, fabsf(pd[i] - md));
-  if ((double)amax > 30 * a.hz_ire) return;
-  const double zcburstdiv = (lfreq * fsc_inv) / 2;
-  float zcs[kZc];
-#pragma unroll
-  for (int k = 0; k < kZc; ++k) zcs[k] = 0.0f;
-  int count = 0, rising = 0;
-  int j = 0;
-  const int end = n - 1;
-  while (j < end && count < kZc) {
-    if (fabsf(bf(j)) > thr) {
-      double zc;
-      if (calczc0(bf, n, j, zc)) {
-        if (bf(j) < 0.0f) rising |= 1 << count;
-        // zcs is a float32 array in the reference
-#pragma unroll
-        for (int k = 0; k < kZc; ++k)
-          if (k == count) zcs[k] = (float)zc;
-        ++count;
-        j = (int)zc + 1;
-      } else {
-        break;
-      }
-    } else {
-      ++j;
-    }
-  }
-#pragma unroll
-  for (int k = 0; k < kZc; ++k) out.base[k] = (((double)bstart + (double)zcs[k]) - s_rem) / zcburstdiv;
-  out.count = count;
-  out.rising = rising;
-  out.valid = 1;
+  long long s1;
+  np_slice(s + ba.bstart, s + bend, fd.demod_len, ba.s0, s1);
+  ba.n = (int)(s1 - ba.s0);
+  ba.zcburstdiv = (lfreq * fsc_inv) / 2;
+  return ba.n > 0;
 }
 
-// The two clb_findbursts passes of one line from its stored cycle bases,
-// starting at phase_adjust = pa; returns the final phase_adjust and the
-// rising count of the second pass.
-__device__ double burst_passes(const BurstLine& bl, double pa, int& rising_count) {
+// The two clb_findbursts passes of one line from its cycle bases (all 16
+// slots), starting at phase_adjust = pa; returns the final phase_adjust and
+// the rising count of the second pass.
+__device__ double burst_passes(const double (&base)[kZc], int count, int rmask, double pa, int& rising_count) {
   rising_count = 0;
-  if (bl.count == 0) return pa;
+  if (count == 0) return pa;
+#pragma unroll
   for (int pass = 0; pass < 2; ++pass) {
     double d[kZc];
     int odd = 0;
 #pragma unroll
     for (int k = 0; k < kZc; ++k) {
-      const double cyc = bl.base[k] + pa;
+      const double cyc = base[k] + pa;
       const double rs = cyc + .5;
       const int r = (rs > -2147483648.0 && rs < 2147483648.0) ? (int)rs : 0;  // astype(np.int32)
       d[k] = (double)r - cyc;
@@ -370,102 +288,405 @@ __device__ double burst_passes(const BurstLine& bl, double pa, int& rising_count
     }
     sort16(d);
     pa += (d[7] + d[8]) / 2.0;
-    if (pass == 1) rising_count = __builtin_popcount((unsigned)(bl.rising ^ odd) & 0xffffu);
+    if (pass == 1) rising_count = __builtin_popcount((unsigned)(rmask ^ odd) & 0xffffu);
   }
   return pa;
 }
 
-__global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
-  __shared__ double adj[kRefThreads];
-  __shared__ int valid[kRefThreads];
-  __shared__ int cnt, rsum;
-  __shared__ double mid[2];
-  const int l = threadIdx.x;
-  double prev = a.prev_phase;
-  for (int f = 0; f < a.nfields; ++f) {
-    int v = 0, rise = 0;
-    double ad = 0.0;
-    if (l < kBurstLines) {
-      BurstLine& bl = a.lines[(size_t)f * kBurstLines + l];
-      v = bl.valid;
-      if (v) {
-        int rc;
-        const double pa = burst_passes(bl, -prev, rc);
-        rise = rc > (bl.count / 2.0);
-        ad = (-pa) / 2;
-        bl.adj = ad;
-        bl.rise = rise;
-      }
+// Step positions and per-interval results of one line (see BurstLine), by
+// one wavefront: lane k < 16 takes slot k's step position frac(-0.5 -
+// base[k]) and its rank among the 16; lanes i <= 16 then run both passes at
+// the middle of interval i.
+__device__ void burst_table(BurstLine& out, const double (&base)[kZc], int count, int rmask, double* sbp, int lane) {
+  double mine = 0.0;
+#pragma unroll
+  for (int k = 0; k < kZc; ++k)
+    if (k == lane) mine = base[k];
+  double t = -0.5 - mine;
+  t = t - floor(t);
+  if (lane < kZc) sbp[lane] = t;
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  int r = 0;
+#pragma unroll
+  for (int k = 0; k < kZc; ++k) {
+    const double o = sbp[k];
+    r += (o < t) || (o == t && k < lane);
+  }
+  __builtin_amdgcn_wave_barrier();
+  if (lane < kZc) {
+    sbp[kZc + r] = t;
+    out.bp[r] = t;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (lane <= kZc) {
+    const double lo = lane == 0 ? 0.0 : sbp[kZc + lane - 1];
+    const double hi = lane == kZc ? 1.0 : sbp[kZc + lane];
+    int rc;
+    const double pa = burst_passes(base, count, rmask, 0.5 * (lo + hi), rc);
+    out.tab[lane] = pa;
+    out.rc[lane] = (unsigned char)rc;
+  }
+}
+
+constexpr int kAreaMax = 256;  // burst-area samples a wavefront holds (4 per lane)
+constexpr int kAreaWords = kAreaMax / 64;
+
+// First set bit at or above position j of a kAreaMax-bit mask, or -1.
+__device__ __forceinline__ int next_bit(const unsigned long long (&m)[kAreaWords], int j) {
+#pragma unroll
+  for (int h = 0; h < kAreaWords; ++h) {
+    if (j < 64 * (h + 1)) {
+      const int k = j - 64 * h;
+      const unsigned long long t = k > 0 ? (m[h] >> k) << k : m[h];
+      if (t) return 64 * h + __builtin_ctzll(t);
+      j = 64 * (h + 1);
     }
-    adj[l] = ad;
-    valid[l] = v;
-    if (l == 0) rsum = 0;
+  }
+  return -1;
+}
+
+// Value i (uniform) of the area held as x[h] = value(lane + 64 h).
+__device__ __forceinline__ float area_at(const float (&x)[kAreaWords], int i) {
+  float v = 0.0f;
+#pragma unroll
+  for (int h = 0; h < kAreaWords; ++h)
+    if ((i >> 6) == h) v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x[h]), i & 63));
+  return v;
+}
+
+constexpr int kScanWaves = 4;  // lines per workgroup of k_burst_scan
+
+// One wavefront per (field, line): compute_line_bursts' statistics and the
+// clb_findbursts zero-crossing scan (core.py:2999-3047, utils.py:1339-1368),
+// reduced to the 16 cycle bases and the step table.  The statistics keep
+// the reference's summation orders (all lanes evaluate the sequential sums
+// redundantly from LDS); the scan walks ballot masks.  A burst area longer
+// than kAreaMax samples (input rates above about 130 MSPS) counts as no
+// burst.
+__global__ __launch_bounds__(64 * kScanWaves) void k_burst_scan(RefineArgs a, const float* __restrict__ demod,
+                                                               const float* __restrict__ dburst) {
+  __shared__ __align__(16) float sb[kScanWaves][kAreaMax];
+  __shared__ __align__(16) float sd[kScanWaves][kAreaMax];
+  __shared__ double sbp[kScanWaves][2 * kZc];
+  // the wavefront index is uniform: readfirstlane lets the line's scalars,
+  // loads and loop bounds live in scalar registers
+  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
+  const int l = blockIdx.x * kScanWaves + wv;
+  const int f = blockIdx.y;
+  if (l >= kBurstLines) return;
+  const ld_line_field fd = a.fields[f];
+  const double* ll = a.ll2 + fd.lineloc_offset;
+  BurstLine& out = a.lines[(size_t)f * kBurstLines + l];
+  if (lane == 0) {
+    out.valid = 0;
+    out.count = 0;
+    out.rising = 0;
+  }
+  BurstArea ba;
+  if (!burst_area(a, fd, ll, l, ba)) return;
+  const int n = ba.n;
+  if (n > kAreaMax) return;
+  float* b = sb[wv];
+  float* d = sd[wv];
+  const float* pb = dburst + fd.demod_offset + ba.s0;
+  const float* pd = demod + fd.demod_offset + ba.s0;
+#pragma unroll
+  for (int h = 0; h < kAreaWords; ++h) {
+    const int i = lane + 64 * h;
+    b[i] = i < n ? pb[i] : 0.0f;
+    d[i] = i < n ? pd[i] : 0.0f;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // burstarea - nb_mean(burstarea), rounded to float32 (core.py:3022)
+  const float mb = (float)nb_mean_f32_lds(b, n);
+  auto bf = [&](int i) { return b[i] - mb; };
+  const float m2 = np_mean_f32(bf, n);
+  const float thr = sqrtf(np_mean_f32([&](int i) {
+    const float e = bf(i) - m2;
+    return e * e;
+  }, n));
+  const float md = (float)nb_mean_f32_lds(d, n);
+  float amax = 0.0f;
+  float x[kAreaWords];
+#pragma unroll
+  for (int h = 0; h < kAreaWords; ++h) {
+    const int i = lane + 64 * h;
+    if (i < n) amax = fmaxf(amax, fabsf(d[i] - md));
+    x[h] = i < n ? bf(i) : 0.0f;
+  }
+  for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
+  if ((double)amax > 30 * a.hz_ire) return;
+  // masks over the area: |x| > threshold, x < 0, rising and falling crossings of 0
+  unsigned long long above[kAreaWords], neg[kAreaWords], rise[kAreaWords], fall[kAreaWords];
+#pragma unroll
+  for (int h = 0; h < kAreaWords; ++h) {
+    const int i = lane + 64 * h;
+    const float xp = (i >= 1 && i < n) ? bf(i - 1) : 0.0f;
+    above[h] = __ballot(i < n && fabsf(x[h]) > thr);
+    neg[h] = __ballot(i < n && x[h] < 0.0f);
+    rise[h] = __ballot(i >= 1 && i < n && xp < 0.0f && x[h] >= 0.0f);
+    fall[h] = __ballot(i >= 1 && i < n && xp > 0.0f && x[h] <= 0.0f);
+  }
+  float zcs[kZc];
+#pragma unroll
+  for (int k = 0; k < kZc; ++k) zcs[k] = 0.0f;
+  int count = 0, rising = 0;
+  int j = 0;
+  const int end = n - 1;
+  while (j < end && count < kZc) {
+    j = next_bit(above, j);
+    if (j < 0 || j >= end) break;
+    const bool up = (neg[j >> 6] >> (j & 63)) & 1;
+    // calczc_do: first crossing i in (j, min(j + 17, n))
+    const int i = up ? next_bit(rise, j + 1) : next_bit(fall, j + 1);
+    if (i < 0 || i >= (j + 17 < n ? j + 17 : n)) break;
+    const double pa = (double)area_at(x, i - 1), pb2 = (double)area_at(x, i);
+    const double y = (pb2 - pa != 0.0) ? -pa / (-pa + pb2) : 0.0;
+    const double zc = (double)(i - 1) + y;
+    if (up) rising |= 1 << count;
+#pragma unroll
+    for (int k = 0; k < kZc; ++k)
+      if (k == count) zcs[k] = (float)zc;  // zcs is a float32 array in the reference
+    ++count;
+    j = (int)zc + 1;
+  }
+  double base[kZc];
+#pragma unroll
+  for (int k = 0; k < kZc; ++k) base[k] = (((double)ba.bstart + (double)zcs[k]) - ba.s_rem) / ba.zcburstdiv;
+  if (lane < kZc) {
+    double z = 0.0;
+#pragma unroll
+    for (int k = 0; k < kZc; ++k)
+      if (k == lane) z = base[k];
+    out.base[lane] = z;
+  }
+  burst_table(out, base, count, rising, sbp[wv], lane);
+  if (lane == 0) {
+    double mb2 = base[0];
+#pragma unroll
+    for (int k = 1; k < kZc; ++k) mb2 = fmin(mb2, base[k]);
+    out.minbase = mb2;
+    out.count = count;
+    out.rising = rising;
+    out.valid = 1;
+  }
+}
+
+// Order-preserving 64-bit key of a double with its lowest 9 mantissa bits
+// replaced by a unique index: keys of distinct entries never tie, and two
+// values order by index only when they differ in those 9 bits (about 1e-13
+// relative), which leaves the selected median within that of numpy's.
+__device__ __forceinline__ unsigned long long rank_key(double v, int idx) {
+  unsigned long long u = __double_as_longlong(v);
+  u = (u >> 63) ? ~u : (u | 0x8000000000000000ull);
+  return (u & ~0x1ffull) | (unsigned long long)idx;
+}
+
+// Rank of key among keys[0, n) (n even; unused entries hold keys above
+// every value): 128-bit LDS reads, one 64-bit compare per entry.
+__device__ __forceinline__ int rank_in(const unsigned long long* keys, int n, unsigned long long key) {
+  const ulonglong2* kv = reinterpret_cast<const ulonglong2*>(keys);
+  int r = 0;
+#pragma unroll 8
+  for (int j2 = 0; j2 < n / 2; ++j2) {
+    const ulonglong2 q = kv[j2];
+    r += q.x < key;
+    r += q.y < key;
+  }
+  return r;
+}
+
+// One workgroup walks the fields in order; line l = thread l.  Per field:
+// both burst passes from the previous field's median, then the median of
+// the line adjustments by rank (two barriers; LDS double-buffered by field
+// parity).  The next field's line data load while the median forms.
+__global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
+  static_assert(kBurstLines % 2 == 0, "rank_in reads pairs");
+  __shared__ __align__(16) unsigned long long keys[2][kRefThreads];
+  __shared__ int wcnt[2][kRefThreads / 64], wrise[2][kRefThreads / 64], wnan[2][kRefThreads / 64];
+  __shared__ double mid[2][2];
+  const int l = threadIdx.x, w = l >> 6, lane = l & 63;
+  double prev = a.prev_phase;
+  double bp[kZc], tab[kZc + 1], minbase = 0.0;
+  unsigned rcw[5];
+  int count = 0, rmask = 0, valid = 0;
+  auto load = [&](int f) {
+    valid = 0;
+    if (l < kBurstLines) {
+      const BurstLine& bl = a.lines[(size_t)f * kBurstLines + l];
+      valid = bl.valid;
+      count = bl.count;
+      rmask = bl.rising;
+      minbase = bl.minbase;
+#pragma unroll
+      for (int k = 0; k < kZc; ++k) bp[k] = bl.bp[k];
+#pragma unroll
+      for (int k = 0; k <= kZc; ++k) tab[k] = bl.tab[k];
+      const unsigned* rw = reinterpret_cast<const unsigned*>(bl.rc);
+#pragma unroll
+      for (int k = 0; k < 5; ++k) rcw[k] = rw[k];
+    }
+  };
+  load(0);
+  for (int f = 0; f < a.nfields; ++f) {
+    const int p = f & 1;
+    double adj = INFINITY;
+    int rise = 0;
+    const int v = valid;
+    if (v) {
+      const double p0 = -prev;
+      double pa;
+      int rc = 0;
+      if (count == 0) {
+        pa = p0;  // clb_findbursts leaves phase_adjust unchanged
+      } else if (minbase > 0.0 && minbase + p0 + 0.5 > 0.0) {
+        // step-table lookup: p0 = q + t, t in [0, 1)
+        const double q = floor(p0);
+        const double t = p0 - q;
+        int idx = 0;
+#pragma unroll
+        for (int k = 0; k < kZc; ++k) idx += bp[k] <= t;
+        double tv = 0.0;
+        int rr = 0;
+#pragma unroll
+        for (int i = 0; i <= kZc; ++i)
+          if (i == idx) {
+            tv = tab[i];
+            rr = (rcw[i >> 2] >> (8 * (i & 3))) & 0xff;
+          }
+        pa = tv + q;
+        rc = (((long long)q) & 1) ? kZc - rr : rr;
+      } else {
+        const BurstLine& bl = a.lines[(size_t)f * kBurstLines + l];
+        double base[kZc];
+#pragma unroll
+        for (int k = 0; k < kZc; ++k) base[k] = bl.base[k];
+        pa = burst_passes(base, count, rmask, p0, rc);
+      }
+      rise = rc > (count / 2.0);
+      adj = (-pa) / 2;
+      BurstLine& bl = a.lines[(size_t)f * kBurstLines + l];
+      bl.adj = adj;
+      bl.rise = rise;
+    }
+    const unsigned long long key = v ? rank_key(adj, l) : ~0ull;
+    keys[p][l] = key;
+    const unsigned long long vb = __ballot(v);
+    const unsigned long long rb = __ballot(v && rise && (l % 2) == 0);
+    const unsigned long long nb = __ballot(v && !(adj == adj));
+    if (lane == 0) {
+      wcnt[p][w] = __popcll(vb);
+      wrise[p][w] = __popcll(rb);
+      wnan[p][w] = __popcll(nb);
+    }
+    if (f + 1 < a.nfields) load(f + 1);
     __syncthreads();
-    if (v && rise && (l % 2) == 0) atomicAdd(&rsum, 1);
-    const double med = block_median(adj, valid, kBurstLines, &cnt, mid) * 2;
+    int m = 0, rs = 0, nn = 0;
+#pragma unroll
+    for (int i = 0; i < kRefThreads / 64; ++i) {
+      m += wcnt[p][i];
+      rs += wrise[p][i];
+      nn += wnan[p][i];
+    }
+    if (v) {
+      const int r = rank_in(keys[p], kBurstLines, key);
+      if (r == (m - 1) / 2) mid[p][0] = adj;
+      if (r == m / 2) mid[p][1] = adj;
+    }
+    __syncthreads();
+    // numpy's median (NaN if any value is NaN or none exists), times 2
+    const double med = (m == 0 || nn > 0) ? (double)NAN : ((m % 2) ? mid[p][1] : (mid[p][0] + mid[p][1]) / 2.0);
+    const double pm = med * 2;
     if (l == 0) {
-      a.phase_out[f] = med;
-      a.state[f].field14 = rsum > (cnt / 4);
-      a.state[f].nvalid = cnt;
+      a.phase_out[f] = pm;
+      a.state[f].field14 = rs > (m / 4);
+      a.state[f].nvalid = m;
     }
     // the next field starts from this field's median (its own median is
     // still 0 when its lines run, core.py:3366-3369)
-    prev = med;
-    __syncthreads();
+    prev = pm;
   }
 }
 
 __global__ __launch_bounds__(kRefThreads) void k_burst_finish(RefineArgs a) {
-  __shared__ double ll[kRefThreads], ll2[kRefThreads], sadj[kRefThreads], tmp[kRefThreads];
-  __shared__ int bad[kRefThreads], use[kRefThreads];
-  __shared__ int cnt;
+  __shared__ double ll[kRefThreads], ll2[kRefThreads], tmp[kRefThreads];
+  __shared__ double sadj[kRefThreads];
+  __shared__ __align__(16) unsigned long long skey[kRefThreads];
+  __shared__ int bad[kRefThreads], qual[kRefThreads];
+  __shared__ int wcnt[kRefThreads / 64];
   __shared__ double mid[2];
   const int f = blockIdx.x;
   const ld_line_field fd = a.fields[f];
   const int n = fd.nlines;
-  const int l = threadIdx.x;
+  const int nb = kBurstLines < n ? kBurstLines : n;
+  const int l = threadIdx.x, w = l >> 6, lane = l & 63;
   const double* g2 = a.ll2 + fd.lineloc_offset;
   if (l < n) {
     ll2[l] = g2[l];
     bad[l] = a.bad_in[fd.lineloc_offset + l] != 0;
   }
   __syncthreads();
-  // Field.process: linebad |= deriv error of linelocs2
-  int b = 0;
-  if (l < n) b = bad[l] || deriv_error(ll2, l, n);
-  __syncthreads();
+  // Field.process: linebad |= deriv error of linelocs2; lines without a
+  // burst result are bad (core.py:3398-3400)
   const BurstLine* bl = a.lines + (size_t)f * kBurstLines;
+  int b = 0;
   if (l < n) {
+    b = bad[l] || deriv_error(ll2, l, n);
     if (l >= 1 && l < kBurstLines && !bl[l].valid) b = 1;
-    bad[l] = b;
   }
-  // sample adjustments of the good lines (core.py:3405-3415)
+  // sample adjustments of the good lines (core.py:3402-3411)
   int u = 0;
-  double s = 0.0;
-  if (l < kBurstLines && l < n && !(isnan(ll2[l]) || b) && bl[l].valid) {
+  double sv = INFINITY;
+  if (l < nb && !(isnan(ll2[l]) || b) && bl[l].valid) {
     const double lfreq = a.spl * get_linelen(ll2, l, fd.linecount, a.linelen);
-    s = bl[l].adj * lfreq * (1 / a.fsc_mhz);
+    sv = bl[l].adj * lfreq * (1 / a.fsc_mhz);
     u = 1;
   }
-  sadj[l] = s;
-  use[l] = u;
   __syncthreads();
-  const double amed = block_median(sadj, use, kBurstLines < n ? kBurstLines : n, &cnt, mid);
-  const int nadj = cnt;
-  if (l < n) ll[l] = ll2[l];
+  if (l < n) bad[l] = b;
+  sadj[l] = sv;
+  const unsigned long long key = u ? rank_key(sv, l) : ~0ull;
+  skey[l] = key;
+  const unsigned long long ub = __ballot(u);
+  if (lane == 0) wcnt[w] = __popcll(ub);
   __syncthreads();
-  if (l == 0) {
-    if (nadj > 0) {
-      double lastvalid = amed;
-      const int m = kBurstLines < n ? kBurstLines : n;
-      for (int k = 0; k < m; ++k) {
-        if (use[k] && (sadj[k] - amed >= -2) && (sadj[k] - amed <= 2)) {
-          ll[k] += sadj[k];
-          lastvalid = sadj[k];
-        } else {
-          ll[k] += lastvalid;
-        }
+  int m = 0;
+#pragma unroll
+  for (int i = 0; i < kRefThreads / 64; ++i) m += wcnt[i];
+  if (u) {
+    const int r = rank_in(skey, kBurstLines, key);
+    if (r == (m - 1) / 2) mid[0] = sv;
+    if (r == m / 2) mid[1] = sv;
+  }
+  __syncthreads();
+  const double amed = m == 0 ? (double)NAN : ((m % 2) ? mid[1] : (mid[0] + mid[1]) / 2.0);
+  // each line takes its own adjustment when within 2 samples of the median,
+  // else the last such adjustment before it, else the median (core.py:3413-3421)
+  qual[l] = u && (sv - amed >= -2) && (sv - amed <= 2);
+  __syncthreads();
+  double v = 0.0;
+  if (l < n) {
+    v = ll2[l];
+    if (m > 0 && l < nb) {
+      double add;
+      if (qual[l]) {
+        add = sadj[l];
+      } else {
+        int j = l - 1;
+        while (j >= 0 && !qual[j]) --j;
+        add = j >= 0 ? sadj[j] : amed;
       }
+      v += add;
+    }
+  }
+  if (l == 0) {
+    if (m > 0) {
       const int f14 = a.state[f].field14;
       const int first = fd.first_field != 0;
       a.fpid_out[f] = first ? (f14 ? 1 : 3) : (f14 ? 4 : 2);
@@ -473,6 +694,7 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_finish(RefineArgs a) {
       a.fpid_out[f] = 1;
     }
   }
+  if (l < n) ll[l] = v;
   __syncthreads();
   // fix_badlines(linelocs3, linelocs2) (core.py:2566-2589, :3491)
   int b2 = 0;
@@ -704,7 +926,8 @@ int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_bu
   // a pageable source is staged before hipMemcpyAsync returns
   LD_HIP(hipMemcpyAsync((void*)a.fields, fields, sizeof(ld_line_field) * (size_t)nfields, hipMemcpyHostToDevice,
                         st));
-  hipLaunchKernelGGL(k_burst_scan, dim3((kBurstLines + 63) / 64, nfields), dim3(64), 0, st, a, demod, demod_burst);
+  hipLaunchKernelGGL(k_burst_scan, dim3((kBurstLines + kScanWaves - 1) / kScanWaves, nfields), dim3(64 * kScanWaves),
+                     0, st, a, demod, demod_burst);
   LD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_burst_chain, dim3(1), dim3(kRefThreads), 0, st, a);
   LD_HIP(hipGetLastError());

@@ -166,7 +166,8 @@ def _line_fields(fields):
     return arr
 
 
-def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust=0.0, stream=None):
+def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust=0.0, stream=None, out=None,
+                        workspace=None):
     """Burst (NTSC) or pilot (PAL) line alignment + fix_badlines (+ NTSC
     subcarrier shift) for a batch of fields (FieldNTSC.process
     core.py:3472-3495, FieldPAL.process core.py:3212-3237).
@@ -177,23 +178,30 @@ def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust
     field, as compute_linelocs sets it; the pilot pass does not read it) and
     first_field;
     linelocs2 / linebad: the hsync refine's outputs for the whole batch.
+    out: optional preallocated (linelocs, linebad, phase, fpid) device
+    tensors; workspace: optional device buffer of ld_refine_workspace_bytes.
     Returns (linelocs float64, linebad uint8, phase_adjust_median per field or
     None, fieldPhaseID per field or None) as device tensors."""
     torch = rf.torch
     n = linelocs2.numel()
     ll2 = linelocs2.to(rf.device, torch.float64).contiguous()
     bad = linebad.to(rf.device, torch.uint8).contiguous()
-    ll = torch.empty(n, dtype=torch.float64, device=rf.device)
-    bad_out = torch.empty(n, dtype=torch.uint8, device=rf.device)
+    if out is not None:
+        ll, bad_out, phase, fpid = out
+    else:
+        ll = torch.empty(n, dtype=torch.float64, device=rf.device)
+        bad_out = torch.empty(n, dtype=torch.uint8, device=rf.device)
+        phase = torch.empty(len(fields), dtype=torch.float64, device=rf.device)
+        fpid = torch.empty(len(fields), dtype=torch.int32, device=rf.device)
     wsb = rf.lib.ld_refine_workspace_bytes(rf.handle, len(fields), n)
-    ws = torch.empty(max(wsb, 1), dtype=torch.uint8, device=rf.device)
+    ws = workspace if workspace is not None else torch.empty(max(wsb, 1), dtype=torch.uint8, device=rf.device)
+    if ws.numel() < wsb:
+        raise ValueError(f"refine workspace of {ws.numel()} bytes < {wsb}")
     lf = _line_fields(fields)
     lp = line_params(rf)
     st = rf._stream() if stream is None else stream
     if rf.system == "NTSC":
         demod, burst = video["demod"], video["demod_burst"]
-        phase = torch.empty(len(fields), dtype=torch.float64, device=rf.device)
-        fpid = torch.empty(len(fields), dtype=torch.int32, device=rf.device)
         _lib.check(rf.lib.ld_refine_burst_ntsc(
             rf.handle, demod.data_ptr(), burst.data_ptr(), demod.numel(), lf, len(fields), ctypes.byref(lp),
             ll2.data_ptr(), bad.data_ptr(), n, float(prev_phase_adjust), ll.data_ptr(), bad_out.data_ptr(),
