@@ -139,8 +139,8 @@ constexpr uint32_t kStrideB = kT * 8;           // bytes between a thread's elem
 // After a forward M-point FFT of y[m] = x[2m] + i x[2m+1] (v[m] = Y[k],
 // k = tid+T*m) form the half spectrum X[k] = E + W_N^k O of the real
 // sequence x in x[m], and X[M] (real) in xM on the thread holding k = 0.
-__device__ __forceinline__ void r2c_split(const float2 (&v)[kP], float2 (&x)[kP], float& xM, Lds& s, float2 wt) {
-  const int tid = threadIdx.x;
+__device__ __forceinline__ void r2c_split(const float2 (&v)[kP], float2 (&x)[kP], float& xM, Lds& s, float2 wt,
+                                          int tid) {
   const LdsPair own(s.re, s.im, pidx(tid)), mir(s.re, s.im, pidx(-tid));
   static_for<0, kP>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
@@ -159,26 +159,27 @@ __device__ __forceinline__ void r2c_split(const float2 (&v)[kP], float2 (&x)[kP]
     float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
     x[m] = cadd(E, cmul(split_tw<m>(wt), O));
     if (m == 0 && tid == 0) xM = p.x - p.y;
+    // bound how far the scheduler hoists the mirror loads (register pressure)
+    if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   });
   __syncthreads();
 }
 
 // Input of a real inverse transform of Z = X * H (Hermitian half spectrum, H
-// scaled by 1/M), conjugated so that a forward FFT produces the conjugate
-// of the inverse: v[m] = conj(E + i O).
-__device__ __forceinline__ void c2r_pre(const float2 (&x)[kP], float xM, rsrc_t H, float2 (&v)[kP], Lds& s,
-                                        float2 wt) {
-  const int tid = threadIdx.x;
+// scaled by 1/M; its bins tid + T m arrive in v, bin M in hM), conjugated
+// so that a forward FFT produces the conjugate of the inverse:
+// v[m] = conj(E + i O).
+__device__ __forceinline__ void c2r_pre(const float2 (&x)[kP], float xM, float2 hM, float2 (&v)[kP], Lds& s,
+                                        float2 wt, int tid) {
   const LdsPair own(s.re, s.im, pidx(tid)), mir(s.re, s.im, pidx(-tid));
-  const uint32_t vo = tid * 8u;
   static_for<0, kP>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    float2 z = cmul(x[m], bload2(H, vo, m * kStrideB));
+    float2 z = cmul(x[m], v[m]);
     v[m] = z;
     own.template st<POff<kT * m>::value>(z);
   });
   if (tid == 0) {  // Z[M], read by k = 0
-    float2 zM = cscale(bload2(H, 0u, kM * 8u), xM);
+    float2 zM = cscale(hM, xM);
     s.re[pidx(kM)] = zM.x;
     s.im[pidx(kM)] = zM.y;
   }
@@ -190,25 +191,54 @@ __device__ __forceinline__ void c2r_pre(const float2 (&x)[kP], float xM, rsrc_t 
     float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
     float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), split_tw<m>(wt));
     v[m] = make_float2(E.x - O.y, -(E.y + O.x));
+    if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   });
   // the FFT's first pass synchronises before it writes LDS
 }
 
+// A transform's output v goes to LDS at the thread's own positions (the
+// transform's last pass synchronises after its final LDS reads), which
+// frees v to receive the next phase's filter before this phase's global
+// stores are issued: the in-order memory counter then lets the next loads
+// complete without waiting for the stores, and the next transform runs
+// while the stores drain.
+__device__ __forceinline__ void stage_own(const float2 (&v)[kP], Lds& s, int tid) {
+  const LdsPair own(s.re, s.im, pidx(tid));
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    own.template st<POff<kT * m>::value>(v[m]);
+  });
+}
+
 // Write y (from a forward FFT of conjugated input: y[2m] = v.x, y[2m+1] =
-// -v.y) as out[i] = y[(i + cut + shift) mod N] + dc for i < len; stores
-// with i >= len fall outside the descriptor and are dropped.
-__device__ __forceinline__ void c2r_store(const float2 (&v)[kP], const C2rOut& o, int64_t base, int tid) {
+// -v.y, staged in LDS) as out[i] = y[(i + cut + shift) mod N] + dc for
+// i < len; stores with i >= len fall outside the descriptor and are dropped.
+__device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid, Lds& s) {
   const rsrc_t r = make_rsrc(o.out + base, (uint32_t)o.len * 4u);
+  const LdsPair own(s.re, s.im, pidx(tid));
   const int off = o.cut + o.shift;
   const float dc = o.dc;
-#pragma unroll
-  for (int m = 0; m < kP; ++m) {
+  if (((off | o.len) & 1) == 0) {
+    // even offset and length: y[2m], y[2m+1] land on an aligned pair that
+    // is wholly inside or wholly outside [0, len): one 8-byte store
+    static_for<0, kP>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const float2 v = own.template ld<POff<kT * m>::value>();
+      int n = 2 * (tid + kT * m);
+      uint32_t i0 = (uint32_t)((n - off) & (kN - 1));
+      bstore2(r, make_float2(v.x + dc, dc - v.y), i0 * 4u, 0u);
+    });
+    return;
+  }
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const float2 v = own.template ld<POff<kT * m>::value>();
     int n = 2 * (tid + kT * m);
     uint32_t i0 = (uint32_t)((n - off) & (kN - 1));
     uint32_t i1 = (uint32_t)((n + 1 - off) & (kN - 1));
-    bstore1(r, v[m].x + dc, i0 * 4u, 0u);
-    bstore1(r, dc - v[m].y, i1 * 4u, 0u);
-  }
+    bstore1(r, v.x + dc, i0 * 4u, 0u);
+    bstore1(r, dc - v.y, i1 * 4u, 0u);
+  });
 }
 
 // Hz of the phase increment d (radians, in (-pi, pi]) wrapped to [0, 2pi):
@@ -308,6 +338,15 @@ __device__ __forceinline__ bool notch_zero(const int* flag, int lo, int hi, int 
   return (flag[i - 1] | flag[i] | flag[i + 1]) != 0;
 }
 
+// The thread's bins tid + T m of a half-spectrum table (bin M of every
+// table sits in LDS, see bin_m).
+__device__ __forceinline__ void load_half(const float2* table, float2 (&h)[kP], int tid) {
+  const rsrc_t H = make_rsrc(table, kHalfBytes);
+  const uint32_t vo = tid * 8u;
+#pragma unroll
+  for (int m = 0; m < kP; ++m) h[m] = bload2(H, vo, m * kStrideB);
+}
+
 // ---------------------------------------------------------------- kernel
 // One workgroup demodulates one block end to end (RFDecode.demodblock_cpu,
 // core.py:755-879); only the input and the requested outputs touch HBM
@@ -330,23 +369,32 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
   __shared__ int notch_flag[kNotchMax + 4];
   __shared__ double notch_red[128];
   __shared__ uint32_t notch_mask[kT];  // bit m: bin tid + T m zeroed by the spur notch
+  // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters
+  __shared__ float2 bin_m[2 + 4];
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int64_t s0 = a.first + a.step * b;
-  const rsrc_t hrv = make_rsrc(a.h_rfv, kHalfBytes);
   const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)b * kM, kM * 8u);
   const uint32_t vo = tid * 8u;
   {  // twiddle tables into LDS
     const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
     float2* dst = &twl.p2[0];
     for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
+    if (tid < 2 + a.npost) {
+      const float2* t = tid == 0 ? a.hpf.h : (tid == 1 ? a.h_rfv : a.post[tid - 2].h);
+      bin_m[tid] = t ? bload2(make_rsrc(t, kHalfBytes), 0u, kM * 8u) : make_float2(0.0f, 0.0f);
+    }
   }
   __syncthreads();
   const TwSrcLds twf{&twl};
   const float2 wt0 = twl.split[tid];
+  // v: the transform's data; between phases it carries the next phase's
+  // filter (bins tid + T m), loaded after the transform and before that
+  // phase's output stores (see stage_own)
   float2 v[kP], x[kP];
   float xM = 0.0f;
   const int last = kPhPost + a.npost;
+  const bool has_hpf = a.hpf.out != nullptr;
 
   for (int ph = 0; ph < last; ++ph) {
     // the split twiddles derived from wt are loop-invariant; keep the
@@ -378,19 +426,19 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
       }
     } else if (ph == kPhHpf) {
       // RF high-pass branch (core.py:769-772)
-      if (a.hpf.out == nullptr) continue;
-      c2r_pre(x, xM, make_rsrc(a.hpf.h, kHalfBytes), v, s, wt);
+      if (!has_hpf) continue;
+      c2r_pre(x, xM, bin_m[0], v, s, wt, tidv);
     } else if (ph == kPhEven || ph == kPhOdd) {
       // analytic signal (core.py:800-805), conjugated: even samples from
       // Z[k] (+ Z[M] folded into bin 0), odd samples from Z[k] W_N^-k
       // (- Z[M] folded into bin 0)
-      const float2 hM = cscale(bload2(hrv, 0u, kM * 8u), xM);
+      const float2 hM = cscale(bin_m[1], xM);  // RFVideo bin M
       const bool odd = (ph == kPhOdd);
       const uint32_t zmask = a.notch_hi > a.notch_lo ? notch_mask[tidv] : 0u;
       static_for<0, kP>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const int k = tid + kT * m;
-        float2 z = cmul(x[m], bload2(hrv, vo, m * kStrideB));
+        float2 z = cmul(x[m], v[m]);
         if (zmask & (1u << m)) z = make_float2(0.0f, 0.0f);
         if (k == 0)
           z = odd ? csub(z, hM) : cadd(z, hM);
@@ -410,8 +458,8 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
       });
       __syncthreads();
       const float hz_per_rad = a.fs / kTwoPi;
-      const bool want_raw = a.demod_raw != nullptr;
-      const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)b * a.vstride, (uint32_t)a.vlen * 4u);
+      // every even-sample load comes before any demod_raw store, so the
+      // in-order memory counter never makes a load wait for a store
       static_for<0, kP>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const int j = tidv + kT * m;
@@ -421,25 +469,37 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
         float h0 = (j > 0) ? inc_to_hz(-atan2f(p.y, p.x), hz_per_rad, a.fs) : 0.0f;
         float2 q = cmulc(od, e);
         float h1 = inc_to_hz(-atan2f(q.y, q.x), hz_per_rad, a.fs);
-        if (want_raw) {  // i < 0 wraps to a huge offset: dropped
-          uint32_t i0 = (uint32_t)(2 * j - a.vcut);
-          bstore1(raw, h0, i0 * 4u, 0u);
-          bstore1(raw, h1, i0 * 4u + 4u, 0u);
-        }
-        v[m] = make_float2(fminf(fmaxf(h0, a.clip_lo), a.clip_hi) - a.dc,
-                           fminf(fmaxf(h1, a.clip_lo), a.clip_hi) - a.dc);
+        v[m] = make_float2(h0, h1);
       });
+      if (a.demod_raw != nullptr) {
+        const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)b * a.vstride, (uint32_t)a.vlen * 4u);
+        const bool pairs = ((a.vcut | a.vlen) & 1) == 0;
+#pragma unroll
+        for (int m = 0; m < kP; ++m) {  // i < 0 wraps to a huge offset: dropped
+          const uint32_t i0 = (uint32_t)(2 * (tidv + kT * m) - a.vcut);
+          if (pairs) {
+            bstore2(raw, v[m], i0 * 4u, 0u);  // aligned pair, wholly in or out
+          } else {
+            bstore1(raw, v[m].x, i0 * 4u, 0u);
+            bstore1(raw, v[m].y, i0 * 4u + 4u, 0u);
+          }
+        }
+      }
+#pragma unroll
+      for (int m = 0; m < kP; ++m)
+        v[m] = make_float2(fminf(fmaxf(v[m].x, a.clip_lo), a.clip_hi) - a.dc,
+                           fminf(fmaxf(v[m].y, a.clip_lo), a.clip_hi) - a.dc);
     } else {
       // video post filters (core.py:811-820)
-      c2r_pre(x, xM, make_rsrc(a.post[ph - kPhPost].h, kHalfBytes), v, s, wt);
+      c2r_pre(x, xM, bin_m[2 + ph - kPhPost], v, s, wt, tidv);
     }
 
     // ---------------------------------------------------------- transform
-    FwdM::run(v, s.re, s.im, twf);
+    FwdM::run(v, s.re, s.im, twf, tidv);
 
     // ---------------------------------------------------------- post
     if (ph == kPhIn || ph == kPhDemod) {
-      r2c_split(v, x, xM, s, wt);
+      r2c_split(v, x, xM, s, wt, tidv);
       if (ph == kPhIn) {
         if (a.notch_hi > a.notch_lo) {  // PAL spur notch decision (core.py:774-798)
           static_for<0, kP>([&](auto mc) {
@@ -470,13 +530,28 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
           }
         });
       }
+      // the next phase's filter: rfhpf's or RFVideo's after the input
+      // transform, the first post filter's after the demod transform
+      if (ph == kPhIn)
+        load_half(has_hpf ? a.hpf.h : a.h_rfv, v, tidv);
+      else if (a.npost > 0)
+        load_half(a.post[0].h, v, tidv);
     } else if (ph == kPhEven) {
-#pragma unroll
-      for (int m = 0; m < kP; ++m) bstore2(zev, v[m], vo, m * kStrideB);
+      stage_own(v, s, tidv);
+      load_half(a.h_rfv, v, tidv);  // RFVideo again for the odd half
+      const LdsPair own(s.re, s.im, pidx(tidv));
+      static_for<0, kP>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        bstore2(zev, own.template ld<POff<kT * m>::value>(), vo, m * kStrideB);
+      });
     } else if (ph == kPhHpf) {
-      c2r_store(v, a.hpf, (int64_t)b * a.vstride, tidv);
+      stage_own(v, s, tidv);
+      load_half(a.h_rfv, v, tidv);
+      c2r_store(a.hpf, (int64_t)b * a.vstride, tidv, s);
     } else if (ph >= kPhPost) {
-      c2r_store(v, a.post[ph - kPhPost], (int64_t)b * a.vstride, tidv);
+      stage_own(v, s, tidv);
+      if (ph + 1 < last) load_half(a.post[ph + 1 - kPhPost].h, v, tidv);
+      c2r_store(a.post[ph - kPhPost], (int64_t)b * a.vstride, tidv, s);
     }
   }
 

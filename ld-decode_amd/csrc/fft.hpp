@@ -230,12 +230,12 @@ __device__ __forceinline__ void lds_barrier() {
 // registers in natural layout).
 template <int N, int T, int R, int NS, int DIR, bool FROM_LDS, bool TO_LDS, int SYNC, class TWF>
 __device__ __forceinline__ void stockham_pass(float2 (&v)[N / T], float* __restrict__ lre,
-                                              float* __restrict__ lim, const TWF& twf) {
+                                              float* __restrict__ lim, const TWF& twf, int tid_in) {
   constexpr int P = N / T;
   constexpr int Q = P / R;  // butterflies per thread
   static_assert(P % R == 0, "radix must divide points per thread");
   static_assert(TO_LDS || NS * R == N, "only the last pass may end in registers");
-  const int tid = threadIdx.x & (T - 1);
+  const int tid = tid_in & (T - 1);
   static_assert(T % 32 == 0 && (N / R) % 32 == 0, "LDS offsets assume 32-aligned strides");
   if constexpr (FROM_LDS) {
     const LdsPair b(lre, lim, pidx(tid));
@@ -331,10 +331,13 @@ template <int DIR, int SYNC>
 struct WgFft<16384, DIR, SYNC> {
   static constexpr int T = 512;
   template <class TWF>
-  static __device__ __forceinline__ void run(float2 (&v)[32], float* lre, float* lim, const TWF& twf) {
-    stockham_pass<16384, 512, 32, 1, DIR, false, true, SYNC>(v, lre, lim, twf);
-    stockham_pass<16384, 512, 32, 32, DIR, true, true, SYNC>(v, lre, lim, twf);
-    stockham_pass<16384, 512, 16, 1024, DIR, true, false, SYNC>(v, lre, lim, twf);
+  // tid: the thread index (callers may pass an opaque copy to keep address
+  // arithmetic from being hoisted out of their loops)
+  static __device__ __forceinline__ void run(float2 (&v)[32], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<16384, 512, 32, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<16384, 512, 32, 32, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<16384, 512, 16, 1024, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
   }
 };
 
@@ -343,10 +346,11 @@ template <int DIR, int SYNC>
 struct WgFft<1024, DIR, SYNC> {
   static constexpr int T = 64;
   template <class TWF>
-  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf) {
-    stockham_pass<1024, 64, 16, 1, DIR, false, true, SYNC>(v, lre, lim, twf);
-    stockham_pass<1024, 64, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf);
-    stockham_pass<1024, 64, 4, 256, DIR, true, false, SYNC>(v, lre, lim, twf);
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<1024, 64, 16, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<1024, 64, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<1024, 64, 4, 256, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
   }
 };
 
