@@ -134,20 +134,19 @@ class CaptureDecoder:
 
 
 def truth_fields(rf, cap, nfields, first_field=0, margin_lines=2, delay=13.0):
-    """Field plan from a synthetic capture's ground truth: lines sit on the
-    field's video-line grid (half-line f*hpf + first + 2*(l - L0), with line
-    1 the first line starting in the field's first two half-lines); the field
+    """Field plan from a synthetic capture's ground truth: line l of field f
+    starts at half-line synth.field_line0(system, f) + 2*l (line 0 is the
+    last line start before the field's equalising pulses; even-numbered
+    fields are first fields); the field
     buffer begins at the block holding the start of line -margin_lines;
     linelocs1 = rounded true starts + the typical filter delay of demod_05's
     sync edge, relative to the buffer origin."""
-    from .synth import _field_layout
-    hpf, _, _, first = _field_layout(rf.system)
-    l0 = first // 2 + 1
+    from .synth import field_line0
     proclines = rf.SysParams["frame_lines"] // 2 + 1 + 10 + (3 if rf.system == "PAL" else 0)
     hs = cap.halfline_starts
     fields, lls = [], []
     for f in range(first_field, first_field + nfields):
-        idx = f * hpf + first + 2 * (np.arange(proclines) - l0)
+        idx = field_line0(rf.system, f) + 2 * np.arange(proclines)
         if idx[0] < 2 * margin_lines or idx[-1] >= len(hs):
             raise ValueError(f"field {f} does not fit the capture")
         raw = hs[idx]

@@ -44,10 +44,24 @@ class Capture:
 
 
 def _field_layout(system):
-    """(halflines per field, eq pulses, vsync pulses, first video halfline)."""
+    """(halflines per field, eq pulses, vsync pulses, first video halfline of
+    an even-numbered field; odd-numbered fields start their video one
+    half-line later)."""
     if system == "NTSC":
         return 525, 6, 6, 18
     return 625, 5, 5, 15
+
+
+def field_line0(system, field):
+    """Global half-line index at which line 0 of field `field` begins: the
+    last line start before the field's first equalising pulse (lines run on
+    whole-line steps from there).  Even-numbered fields are the reference's
+    first fields (their line 0 sits 2 half-lines before the pulses for NTSC,
+    1 for PAL; core.py:2039-2054 with firstFieldH), odd-numbered fields the
+    second ones."""
+    hpf, _, _, first = _field_layout(system)
+    first_q = first + (field % 2)
+    return field * hpf - (2 if first_q % 2 == 0 else 1)
 
 
 def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0, pilot_ire=0.0):
@@ -56,8 +70,10 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
     Half-line h (global index) starts at h * H/2 * fs plus the jitter of its
     full line.  Within a field (hpf half-lines), half-lines [0, neq) carry
     equalising pulses, [neq, neq+nvs) vertical sync pulses, [neq+nvs, 3*neq)
-    equalising pulses, then every other half-line starts a video line with an
-    hsync pulse, burst and colour bars.  pilot_ire > 0 adds the PAL disc
+    equalising pulses, then every other half-line from the field's first
+    video half-line (first, or first + 1 in odd-numbered fields, so that the
+    line grid runs on across the interlaced field boundary) starts a video
+    line with an hsync pulse, burst and colour bars.  pilot_ire > 0 adds the PAL disc
     pilot (pilot_mhz, continuous phase) of that peak amplitude on every
     hsync pulse."""
     hpf, neq, nvs, first = _field_layout(system)
@@ -71,7 +87,9 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
     h_of_n = np.searchsorted(hstart, n, side="right") - 1
     h_of_n = np.clip(h_of_n, 0, nhalf - 1)
     t_us = (n - hstart[h_of_n]) / fs * 1e6      # time since the half-line start
-    hf = (h_of_n + start_halfline) % hpf          # half-line within field
+    hg = h_of_n + start_halfline
+    hf = hg % hpf                                 # half-line within field
+    first_n = first + (hg // hpf) % 2             # first video half-line
     sync = sysp["vsync_ire"]
     setup = 7.5 if system == "NTSC" else 0.0
     ire = np.zeros(nsamples)
@@ -79,8 +97,8 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
     vs = (hf >= neq) & (hf < neq + nvs)
     ire[eq & (t_us < sysp["eqPulseUS"])] = sync
     ire[vs & (t_us < sysp["vsyncPulseUS"])] = sync
-    video = hf >= first
-    second_half = ((hf - first) % 2) == 1
+    video = hf >= first_n
+    second_half = ((hf - first_n) % 2) == 1
     # time since the line's hsync for video half-lines
     tl = t_us + np.where(second_half, sysp["line_period"] / 2, 0.0)
     vline = video & ~second_half
@@ -102,9 +120,10 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
     cph = np.deg2rad(np.array([b[2] for b in _BARS], float))
     ire[act] = (setup + lum[bar] * (1 - setup / 100)
                 + cam[bar] * np.sin(2 * np.pi * fsc * t_abs_us[act] + cph[bar]))
-    # full-line starts: video half-lines at even offset from `first`
-    hf_all = (hidx) % hpf
-    is_line = (hf_all >= first) & (((hf_all - first) % 2) == 0)
+    # full-line starts: video half-lines at even offset from the field's first
+    hf_all = hidx % hpf
+    first_all = first + (hidx // hpf) % 2
+    is_line = (hf_all >= first_all) & (((hf_all - first_all) % 2) == 0)
     is_line |= hf_all < 3 * neq  # vblank half-lines also begin with a pulse
     line_starts = hstart[is_line & (hstart < nsamples)]
     return ire, line_starts, hstart[hstart < nsamples]

@@ -11,7 +11,7 @@ import numpy as np
 import pytest
 
 from lddecode_amd import capture, synth
-from lddecode_amd.synth import _field_layout
+from lddecode_amd.synth import field_line0
 from oracle import dsp as O
 from oracle import lines as L
 from oracle import tbc as T
@@ -39,8 +39,7 @@ def _field_case(system, seed, jitter_ns):
     ll2, bad = T.refine_linelocs_hsync(video["demod_05"], ll1, np.zeros(len(ll1), bool), system, 40.0,
                                        orf.iretohz, orf.DecoderParams["vsync_ire"])
     bad = L.compute_deriv_error(ll2, bad)
-    hpf, _, _, first = _field_layout(system)
-    idx = hpf + first + 2 * (np.arange(len(ll1)) - (first // 2 + 1))
+    idx = field_line0(system, 1) + 2 * np.arange(len(ll1))
     nominal = idx * (orf.SysParams["line_period"] * 40 / 2) - (org + 1024)
     return orf, video, ll2, bad, nominal
 
@@ -53,7 +52,12 @@ def test_refinement_recovers_unjittered_grid(system):
     orf, video, ll2, bad, nominal = _field_case(system, seed=2, jitter_ns=30)
     p = _params(orf)
     if system == "NTSC":
-        ll, badf, fpid, med = L.ntsc_field_lines(p, video["demod"], video["demod_burst"], ll2, bad, 263, True, 0.0)
+        # field 1 is a second field; its burst search starts from the phase
+        # median a previous field leaves behind (compute_burst_offsets uses
+        # prevfield.phase_adjust_median), here from a first pass over itself
+        _, _, _, med0 = L.ntsc_field_lines(p, video["demod"], video["demod_burst"], ll2, bad, 262, False, 0.0)
+        ll, badf, fpid, med = L.ntsc_field_lines(p, video["demod"], video["demod_burst"], ll2, bad, 262, False,
+                                                 med0)
         n = L.NTSC_BURST_LINES
         assert fpid in (1, 2, 3, 4) and np.isfinite(med)
     else:
