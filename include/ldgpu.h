@@ -261,6 +261,167 @@ int ld_refine_pilot_pal(ld_handle* h, const float* demod_pilot, int64_t demod_to
                         double* linelocs_out, uint8_t* linebad_out, void* workspace,
                         size_t workspace_bytes, void* stream);
 
+/* ---------------------------------------------------------------------
+ * Line location (Field.compute_linelocs, core.py:2325-2481, and the field
+ * walk of LDdecode.decodefield, core.py:4018-4098).  Sync pulses, their
+ * levels and the skip-check medians are found on the GPU; the pulse state
+ * machine, vertical-blanking analysis and line-0 logic run on the host over
+ * the pulse lists.
+ * ------------------------------------------------------------------- */
+
+/* Device workspace of ld_find_pulses for an n-sample buffer. */
+size_t ld_pulse_workspace_bytes(const ld_handle* h, int64_t n);
+
+/* Sync-pulse edges of the DEVICE float32 buffer x[0, n) (findpulses_numba_raw,
+ * utils.py:1101-1141, run over the whole buffer): a sample is in a pulse when
+ * (double)x <= high.  edges (DEVICE) receives, in order, the first sample of
+ * every run of in-pulse samples and the first sample after the run,
+ * alternating start / end (a run reaching the end of the buffer has no end
+ * edge); at most `cap` are written and *count (DEVICE) receives the total
+ * number of edges, so a count above cap means the list was cut short.
+ * Pulse k is (edges[2k], edges[2k+1] - edges[2k]).  The reference's
+ * per-field pulse list is the subset with start > the field buffer's first
+ * sample, end < its last and length <= 5000. */
+int ld_find_pulses(ld_handle* h, const float* x, int64_t n, double high, int64_t* edges, int64_t cap,
+                   int64_t* count, void* workspace, size_t workspace_bytes, void* stream);
+
+/* getpulses' vertical-sync levels (core.py:2274-2286): for every pulse k
+ * of ld_find_pulses' list whose length exceeds min_len, the mean of
+ * x[int(start + trim), int(start + len - trim)) as np.mean computes it on a
+ * float32 array (pairwise float32 sum, float32 divide); NaN for the other
+ * pulses.  The edge count is read from the DEVICE `count`; means (DEVICE)
+ * gets one value per pulse k < min(count / 2, cap_pulses). */
+int ld_pulse_means(ld_handle* h, const float* x, int64_t n, const int64_t* edges, const int64_t* count,
+                   int64_t cap_pulses, double min_len, double trim, float* means, void* stream);
+
+/* np.mean (float32) of x[begin[i], end[i]) for `count` ranges (DEVICE x,
+ * begin, end, out); an empty range gives NaN.  getpulses' black levels
+ * (core.py:2296-2312). */
+int ld_range_means(ld_handle* h, const float* x, int64_t n, const int64_t* begin, const int64_t* end,
+                   int count, float* out, void* stream);
+
+/* What a field leaves for the field after it (getLine0, core.py:2157-2239):
+ * ll_linecount = its final linelocs[linecount] + its buffer origin (a
+ * stitched index; linecount 263 / 262 NTSC, 312 / 313 PAL) and the eight
+ * line medians skip_check (core.py:2122-2155) takes over demod[int(b),
+ * int(b + line_samples + 1)) with b = linelocs[outlinecount + lineoffset + j].
+ * known = 0 marks facts not measured yet. */
+typedef struct ld_field_facts {
+  double ll_linecount;
+  double medians[8];
+  int32_t known;
+  int32_t reserved;
+} ld_field_facts;
+
+/* Measure ld_field_facts of every field from its final line locations (the
+ * output of ld_refine_burst_ntsc / ld_refine_pilot_pal).  fields is a HOST
+ * array (demod_offset, demod_len, lineloc_offset, nlines, first_field are
+ * used); demod, linelocs and facts are DEVICE arrays.  pal selects PAL's
+ * line counts and offsets (312 / 313, lineoffset 2 / 3).  The medians are
+ * numba's np.median of float32 data (middle value; for an even count the
+ * float32 sum of the two middle values halved in float64).  A line whose
+ * span leaves the field buffer gives a NaN median.  `workspace`: device memory
+ * of ld_field_facts_workspace_bytes(h, nfields). */
+size_t ld_field_facts_workspace_bytes(const ld_handle* h, int nfields);
+int ld_field_facts_measure(ld_handle* h, const float* demod, int64_t demod_total, const ld_line_field* fields,
+                           int nfields, const double* linelocs, int64_t nlinelocs, int pal, int outlinecount,
+                           double line_samples, ld_field_facts* facts, void* workspace, size_t workspace_bytes,
+                           void* stream);
+
+/* RFDecode / SysParams / DecoderParams / LDdecode values line location reads
+ * (core.py:62-170, :384-393, :3625-3634). */
+typedef struct ld_locate_params {
+  double freq;                /* rf.freq, MHz */
+  double linelen;             /* rf.linelen (rounded to an integer) */
+  double ire0, hz_ire, vsync_ire;
+  double hsync_us, eq_us, vsync_us, line_period_us;
+  double first_field_h[2];    /* SysParams firstFieldH */
+  double hsync_tolerance;     /* rf.hsync_tolerance */
+  double prov_delta;          /* provisional final - linelocs1 offset at linecount (facts unknown) */
+  int32_t pal;
+  int32_t num_pulses;         /* SysParams numPulses */
+  int32_t field_lines[2];     /* SysParams field_lines */
+  int32_t frame_lines;
+  int32_t reserved;
+  int64_t blocklen;           /* rf.blocklen: decodefield's read granularity */
+  int64_t blockcut;           /* rf.blockcut */
+  int64_t block_stride;       /* DemodCache.blocksize: usable samples per block */
+  int64_t readlen;            /* LDdecode.readlen */
+} ld_locate_params;
+
+/* The previous field as getLine0 / getpulses see it. */
+typedef struct ld_prev_field {
+  int32_t valid;
+  int32_t is_first_field;
+  int32_t linecount;          /* 263 / 262 NTSC, 312 / 313 PAL */
+  int32_t sync_confidence;
+  int64_t startloc;
+  double ll_linecount;        /* stitched index of its final linelocs[linecount] */
+  int32_t skip_score;         /* its skip_check() */
+  int32_t fields_written;     /* LDdecode.fields_written after it */
+} ld_prev_field;
+
+enum {
+  LD_LOC_OK = 0,              /* valid field: linelocs1 / linebad written */
+  LD_LOC_FAILED = 1,          /* compute_linelocs found no usable field (Field.valid False) */
+  LD_LOC_NEED_LEVELS = 2,     /* getpulses would re-threshold at measured sync / black levels */
+  LD_LOC_NEED_IRE0 = 3        /* getpulses would recalibrate ire0 (first field without pulses) */
+};
+
+/* One field of the walk: decodefield's read and compute_linelocs' results.
+ * Positions are relative to the field buffer (stitched index startloc),
+ * as the reference's Field sees them. */
+typedef struct ld_field_loc {
+  double start;               /* raw position decodefield was called with */
+  int64_t readloc;
+  int64_t startloc;           /* field buffer origin: DemodCache.read's startloc */
+  int64_t buflen;             /* field buffer length (whole blocks) */
+  double line0loc;
+  double meanlinelen;
+  double nextfieldoffset;     /* Field.nextfieldoffset (after process()) */
+  double offset;              /* decodefield's returned offset: next start = start + offset */
+  int64_t vblank_next;        /* -1 when none */
+  int32_t status;             /* LD_LOC_* */
+  int32_t is_first_field;
+  int32_t linecount;          /* 263 / 262 */
+  int32_t sync_confidence;
+  int32_t skipdetected;
+  int32_t npulses;            /* pulses in the field buffer */
+  int32_t nlines;             /* proclines written */
+  int32_t reserved;
+} ld_field_loc;
+
+/* compute_linelocs for one field from its pulse list (HOST arrays; starts
+ * relative to the field buffer, as getpulses returns them) and the previous
+ * field (NULL: none).  Writes *out and, for a valid field, linelocs1 /
+ * linebad[0, proclines) (max_lines must be >= proclines: 273 NTSC, 326 PAL).
+ * Host only. */
+int ld_locate_field(const ld_locate_params* p, const int64_t* starts, const int64_t* lens, int npulses,
+                    const ld_prev_field* prev, int64_t startloc, int fields_written, ld_field_loc* out,
+                    double* linelocs1, uint8_t* linebad, int max_lines);
+
+/* The field walk over a stitched demod_05 buffer of `total` samples from raw
+ * sample `start`, given ld_find_pulses' edges and ld_pulse_means' levels
+ * (HOST copies).  Field k's buffer, pulses and compute_linelocs results go to
+ * out[k] and linelocs1 / linebad[k * max_lines, + proclines); the next field
+ * starts at start + out[k].offset.  What field k leaves for field k + 1 comes
+ * from facts[k] when facts is non-NULL and facts[k].known, otherwise from a
+ * provisional estimate (its linelocs1[linecount] + prov_delta, skip score
+ * 100): a caller verifies a provisional walk by walking again with the
+ * measured facts.  The walk stops after max_fields fields, before a field
+ * whose buffer would pass `total`, or after a field with status
+ * LD_LOC_NEED_LEVELS / LD_LOC_NEED_IRE0 (the caller resolves it with
+ * ld_range_means / ld_find_pulses and ld_locate_field).  *nfields_out gets
+ * the number of fields written, *prev_out the state after the last one.
+ * Host only. */
+int ld_locate_fields(const ld_locate_params* p, const int64_t* edges, int64_t nedges, const float* means,
+                     int64_t total, double start, const ld_prev_field* prev, const ld_field_facts* facts,
+                     int max_fields, ld_field_loc* out, double* linelocs1, uint8_t* linebad, int max_lines,
+                     ld_prev_field* prev_out, int* nfields_out);
+
+/* skip_check's score (core.py:2139-2155) from a field's eight line medians. */
+int ld_skip_score(const ld_locate_params* p, const double* medians8);
+
 #ifdef __cplusplus
 }
 #endif

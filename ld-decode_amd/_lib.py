@@ -22,7 +22,11 @@ EXPORTS = (
     "ld_set_demod_params", "ld_demod_workspace_bytes", "ld_demod_blocks", "ld_set_sinc_lut",
     "ld_tbc_workspace_bytes", "ld_tbc_fields", "ld_hsync_workspace_bytes", "ld_refine_hsync", "ld_set_notch",
     "ld_refine_workspace_bytes", "ld_refine_burst_ntsc", "ld_refine_pilot_pal",
+    "ld_pulse_workspace_bytes", "ld_find_pulses", "ld_pulse_means", "ld_range_means",
+    "ld_field_facts_workspace_bytes", "ld_field_facts_measure", "ld_locate_field", "ld_locate_fields",
+    "ld_skip_score",
 )
+LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
 
 class LdGpuUnavailable(RuntimeError):
@@ -77,6 +81,35 @@ class LineParams(ctypes.Structure):
     _fields_ = [(n, ctypes.c_double) for n in ("freq_mhz", "linelen", "fsc_mhz", "pilot_mhz", "hz_ire")]
 
 
+class FieldFacts(ctypes.Structure):
+    _fields_ = [("ll_linecount", ctypes.c_double), ("medians", ctypes.c_double * 8), ("known", ctypes.c_int32),
+                ("reserved", ctypes.c_int32)]
+
+
+class LocateParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in (
+        "freq", "linelen", "ire0", "hz_ire", "vsync_ire", "hsync_us", "eq_us", "vsync_us", "line_period_us")] + [
+        ("first_field_h", ctypes.c_double * 2), ("hsync_tolerance", ctypes.c_double),
+        ("prov_delta", ctypes.c_double), ("pal", ctypes.c_int32), ("num_pulses", ctypes.c_int32),
+        ("field_lines", ctypes.c_int32 * 2), ("frame_lines", ctypes.c_int32), ("reserved", ctypes.c_int32),
+        ("blocklen", ctypes.c_int64), ("blockcut", ctypes.c_int64), ("block_stride", ctypes.c_int64),
+        ("readlen", ctypes.c_int64)]
+
+
+class PrevField(ctypes.Structure):
+    _fields_ = [("valid", ctypes.c_int32), ("is_first_field", ctypes.c_int32), ("linecount", ctypes.c_int32),
+                ("sync_confidence", ctypes.c_int32), ("startloc", ctypes.c_int64), ("ll_linecount", ctypes.c_double),
+                ("skip_score", ctypes.c_int32), ("fields_written", ctypes.c_int32)]
+
+
+class FieldLoc(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_double), ("readloc", ctypes.c_int64), ("startloc", ctypes.c_int64),
+                ("buflen", ctypes.c_int64), ("line0loc", ctypes.c_double), ("meanlinelen", ctypes.c_double),
+                ("nextfieldoffset", ctypes.c_double), ("offset", ctypes.c_double), ("vblank_next", ctypes.c_int64)] + [
+        (n, ctypes.c_int32) for n in ("status", "is_first_field", "linecount", "sync_confidence", "skipdetected",
+                                      "npulses", "nlines", "reserved")]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -108,6 +141,19 @@ def _declare(lib):
                                          c_i64, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
         "ld_refine_pilot_pal": (c_int, [c_vp, c_vp, c_i64, P(LineField), c_int, P(LineParams), c_vp, c_vp, c_i64,
                                         c_vp, c_vp, c_vp, c_sz, c_vp]),
+        "ld_pulse_workspace_bytes": (c_sz, [c_vp, c_i64]),
+        "ld_find_pulses": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
+        "ld_pulse_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
+        "ld_range_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp, c_vp]),
+        "ld_field_facts_workspace_bytes": (c_sz, [c_vp, c_int]),
+        "ld_field_facts_measure": (c_int, [c_vp, c_vp, c_i64, P(LineField), c_int, c_vp, c_i64, c_int, c_int, c_dbl,
+                                           c_vp, c_vp, c_sz, c_vp]),
+        "ld_locate_field": (c_int, [P(LocateParams), c_vp, c_vp, c_int, P(PrevField), c_i64, c_int, P(FieldLoc),
+                                    c_vp, c_vp, c_int]),
+        "ld_locate_fields": (c_int, [P(LocateParams), c_vp, c_i64, c_vp, c_i64, c_dbl, P(PrevField),
+                                     P(FieldFacts), c_int, P(FieldLoc), c_vp, c_vp, c_int, P(PrevField),
+                                     P(c_int)]),
+        "ld_skip_score": (c_int, [P(LocateParams), P(c_dbl)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -25,6 +25,7 @@
 #include <vector>
 
 #include "internal.hpp"
+#include "npstat.hpp"
 
 #pragma clang fp contract(off)
 
@@ -63,16 +64,6 @@ struct FieldState {
 };
 
 // ---------------------------------------------------------------- helpers
-// Python/numpy slice bounds of [a, b) over n elements.
-__device__ __forceinline__ void np_slice(long long a, long long b, long long n, long long& s, long long& e) {
-  if (a < 0) a = (a + n < 0) ? 0 : a + n;
-  if (a > n) a = n;
-  if (b < 0) b = (b + n < 0) ? 0 : b + n;
-  if (b > n) b = n;
-  s = a;
-  e = b > a ? b : a;
-}
-
 // numba's np.mean of a float32 array: float32 accumulation in order, the
 // division in float64.
 template <class F>
@@ -97,28 +88,6 @@ __device__ __forceinline__ double nb_mean_f32_lds(const float* x, int n) {
   }
   for (; i < n; ++i) c = c + x[i];
   return (double)c / (double)n;
-}
-
-// numpy's pairwise float32 summation (8 accumulators per block of at most
-// 128 elements, halves rounded down to a multiple of 8 above that).
-template <class F>
-__device__ float np_block_sum(F x, int off, int n) {
-  if (n < 8) {
-    float r = -0.0f;
-    for (int i = 0; i < n; ++i) r = r + x(off + i);
-    return r;
-  }
-  float r[8];
-#pragma unroll
-  for (int j = 0; j < 8; ++j) r[j] = x(off + j);
-  int i = 8;
-  for (; i < n - (n % 8); i += 8) {
-#pragma unroll
-    for (int j = 0; j < 8; ++j) r[j] = r[j] + x(off + i + j);
-  }
-  float res = ((r[0] + r[1]) + (r[2] + r[3])) + ((r[4] + r[5]) + (r[6] + r[7]));
-  for (; i < n; ++i) res = res + x(off + i);
-  return res;
 }
 
 // numpy's recursion for n <= 256 (two levels at most).

@@ -1,0 +1,367 @@
+// GPU side of line location (Field.getpulses / skip_check, core.py:2122-2322;
+// findpulses_numba_raw, utils.py:1101-1141):
+//   k_edge_count / k_edge_scan / k_edge_write   findpulses over a whole
+//       stitched buffer as an order-preserving stream compaction of the
+//       threshold crossings: per-tile counts, one exclusive scan, then each
+//       tile writes its edges at its offset
+//   k_pulse_means   getpulses' vertical-sync levels: numpy's float32 mean of
+//       the body of every long pulse, one wavefront per pulse, the samples
+//       staged in LDS
+//   k_range_means   the same mean over caller-listed ranges (black levels)
+//   k_field_facts   skip_check's line medians (bitonic sort in LDS, one
+//       workgroup per field line) and the final line location a field's
+//       successor reads
+// Edges and positions are int64; sample comparisons are made in double, as
+// numba compares the float32 buffer with a float64 threshold.
+#include <math.h>
+
+#include "internal.hpp"
+#include "npstat.hpp"
+
+#pragma clang fp contract(off)
+
+namespace ldgpu {
+
+constexpr int kEdgeThreads = 256;
+constexpr int kEdgePer = 16;                          // samples per thread
+constexpr int kEdgeTile = kEdgeThreads * kEdgePer;    // samples per workgroup
+constexpr int kScanThreads = 1024;
+constexpr int kMeanMax = 5000;                        // findpulses' longest pulse
+constexpr int kMeanWaves = 4;                         // pulses per workgroup
+constexpr int kFactThreads = 256;
+constexpr int kFactMax = 8192;                        // longest skip_check line slice
+
+// Bit i of the result: sample base + i is "in pulse"; bit 16 (prev) for the
+// sample before base.
+__device__ __forceinline__ unsigned below_mask(const float* x, long long n, long long base, double high,
+                                               unsigned& prev) {
+  unsigned m = 0;
+  if (base + kEdgePer <= n) {
+    const float4* p = reinterpret_cast<const float4*>(x + base);
+#pragma unroll
+    for (int q = 0; q < kEdgePer / 4; ++q) {
+      const float4 v = p[q];
+      m |= ((double)v.x <= high ? 1u : 0u) << (4 * q);
+      m |= ((double)v.y <= high ? 1u : 0u) << (4 * q + 1);
+      m |= ((double)v.z <= high ? 1u : 0u) << (4 * q + 2);
+      m |= ((double)v.w <= high ? 1u : 0u) << (4 * q + 3);
+    }
+  } else {
+    for (int i = 0; i < kEdgePer; ++i)
+      if (base + i < n && (double)x[base + i] <= high) m |= 1u << i;
+  }
+  prev = (base > 0 && base - 1 < n && (double)x[base - 1] <= high) ? 1u : 0u;
+  return m;
+}
+
+// Edge bits among the thread's samples (valid samples only): a change of
+// state against the previous sample; sample 0 counts as a start when it is
+// in a pulse.
+__device__ __forceinline__ unsigned edge_bits(unsigned m, unsigned prev, long long n, long long base) {
+  unsigned e = m ^ ((m << 1) | prev);
+  const long long left = n - base;
+  if (left < kEdgePer) e &= left > 0 ? ((1u << left) - 1u) : 0u;
+  return e & 0xffffu;
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_edge_count(const float* __restrict__ x, long long n, double high,
+                                                             long long* __restrict__ counts) {
+  __shared__ int part[kEdgeThreads / 64];
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * kEdgeTile + (long long)t * kEdgePer;
+  unsigned prev;
+  const unsigned m = below_mask(x, n, base, high, prev);
+  int c = __popc(edge_bits(m, prev, n, base));
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((t & 63) == 0) part[t >> 6] = c;
+  __syncthreads();
+  if (t == 0) {
+    long long s = 0;
+    for (int w = 0; w < kEdgeThreads / 64; ++w) s += part[w];
+    counts[blockIdx.x] = s;
+  }
+}
+
+// Exclusive scan of the per-tile counts in place; the total goes to *total.
+__global__ __launch_bounds__(kScanThreads) void k_edge_scan(long long* __restrict__ counts, long long ntiles,
+                                                            long long* __restrict__ total) {
+  __shared__ long long part[kScanThreads];
+  const int t = threadIdx.x;
+  const long long per = (ntiles + kScanThreads - 1) / kScanThreads;
+  const long long b = (long long)t * per;
+  const long long e = b + per < ntiles ? b + per : ntiles;
+  long long s = 0;
+  for (long long i = b; i < e; ++i) s += counts[i];
+  part[t] = s;
+  __syncthreads();
+  for (int o = 1; o < kScanThreads; o <<= 1) {
+    const long long v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long run = part[t] - s;
+  for (long long i = b; i < e; ++i) {
+    const long long c = counts[i];
+    counts[i] = run;
+    run += c;
+  }
+  if (t == kScanThreads - 1) *total = part[t];
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const float* __restrict__ x, long long n, double high,
+                                                             const long long* __restrict__ offsets,
+                                                             long long* __restrict__ edges, long long cap) {
+  __shared__ int part[kEdgeThreads];
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * kEdgeTile + (long long)t * kEdgePer;
+  unsigned prev;
+  const unsigned m = below_mask(x, n, base, high, prev);
+  unsigned e = edge_bits(m, prev, n, base);
+  const int c = __popc(e);
+  part[t] = c;
+  __syncthreads();
+  for (int o = 1; o < kEdgeThreads; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long k = offsets[blockIdx.x] + part[t] - c;
+  while (e) {
+    const int i = __ffs(e) - 1;
+    e &= e - 1;
+    if (k < cap) edges[k] = base + i;
+    ++k;
+  }
+}
+
+// numpy's float32 mean of an LDS-staged range of at most kMeanMax samples.
+__device__ float lds_np_mean(const float* s, int m) {
+  auto at = [s](int i) { return s[i]; };
+  return np_pairwise_sum_any<6>(at, 0, m) / (float)m;
+}
+
+__global__ __launch_bounds__(64 * kMeanWaves) void k_pulse_means(const float* __restrict__ x, long long n,
+                                                                 const long long* __restrict__ edges,
+                                                                 const long long* __restrict__ count,
+                                                                 long long cap_pulses, double min_len, double trim,
+                                                                 float* __restrict__ means) {
+  __shared__ float stage[kMeanWaves][kMeanMax];
+  const int lane = threadIdx.x & 63;
+  const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
+  long long np = *count / 2;
+  if (np > cap_pulses) np = cap_pulses;
+  float* s = stage[w];
+  for (long long k = (long long)blockIdx.x * kMeanWaves + w; k < np; k += (long long)gridDim.x * kMeanWaves) {
+    const long long st = edges[2 * k], len = edges[2 * k + 1] - st;
+    float r = __builtin_nanf("");
+    if ((double)len > min_len && len <= kMeanMax) {
+      // int(start + trim) : int(start + len - trim), Python slice bounds
+      long long a, b;
+      np_slice((long long)((double)st + trim), (long long)((double)st + (double)len - trim), n, a, b);
+      const int m = (int)(b - a);
+      if (m > 0 && m <= kMeanMax) {
+        for (int i = lane; i < m; i += 64) s[i] = x[a + i];
+        __builtin_amdgcn_wave_barrier();
+        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
+        if (lane == 0) r = lds_np_mean(s, m);
+        __builtin_amdgcn_wave_barrier();
+      }
+    }
+    if (lane == 0) means[k] = r;
+  }
+}
+
+__global__ __launch_bounds__(64) void k_range_means(const float* __restrict__ x, long long n,
+                                                    const long long* __restrict__ begin,
+                                                    const long long* __restrict__ end, int count,
+                                                    float* __restrict__ out) {
+  const int k = blockIdx.x;
+  if (k >= count || threadIdx.x != 0) return;
+  long long a, b;
+  np_slice(begin[k], end[k], n, a, b);
+  const long long m = b - a;
+  float r = __builtin_nanf("");
+  if (m > 0 && m <= (128LL << 12)) {
+    const float* p = x + a;
+    auto at = [p](int i) { return p[i]; };
+    r = np_pairwise_sum_any<12>(at, 0, (int)m) / (float)m;
+  }
+  out[k] = r;
+}
+
+struct FactArgs {
+  const float* demod;
+  const double* ll;
+  ld_field_facts* facts;
+  double line_samples;
+  int outlinecount;
+  int pal;
+};
+
+// One workgroup per (field, j): the median of demod over skip_check's
+// line slice j, as numba's np.median; j == 0 also stores ll_linecount.
+__global__ __launch_bounds__(kFactThreads) void k_field_facts(FactArgs a, const ld_line_field* __restrict__ fields) {
+  __shared__ float s[kFactMax];
+  const int f = blockIdx.x >> 3, j = blockIdx.x & 7;
+  const ld_line_field fd = fields[f];
+  const int t = threadIdx.x;
+  const int first = fd.first_field != 0;
+  const int lineoffset = a.pal ? (first ? 2 : 3) : 0;
+  const int line = a.outlinecount + j + lineoffset;
+  if (j == 0 && t == 0) {
+    const int lc = a.pal ? (first ? 312 : 313) : (first ? 263 : 262);
+    a.facts[f].ll_linecount = lc < fd.nlines ? a.ll[fd.lineloc_offset + lc] + (double)fd.demod_offset : __builtin_nan("");
+    a.facts[f].known = 1;
+    a.facts[f].reserved = 0;
+  }
+  double med = __builtin_nan("");
+  long long lo = 0, hi = 0;
+  if (line < fd.nlines) {
+    const double b = a.ll[fd.lineloc_offset + line];
+    // slice(int(b), int(b + length + 1)); a negative start would index from
+    // the end in Python: counted as no data here
+    if (b >= 0.0 && isfinite(b)) np_slice((long long)b, (long long)(b + a.line_samples + 1.0), fd.demod_len, lo, hi);
+  }
+  const long long m = hi - lo;
+  if (m > 0 && m <= kFactMax) {
+    int p2 = 1;
+    while (p2 < m) p2 <<= 1;
+    const float* src = a.demod + fd.demod_offset + lo;
+    for (int i = t; i < p2; i += kFactThreads) s[i] = i < m ? src[i] : __builtin_inff();
+    __syncthreads();
+    for (int k = 2; k <= p2; k <<= 1) {
+      for (int jj = k >> 1; jj > 0; jj >>= 1) {
+        for (int i = t; i < p2; i += kFactThreads) {
+          const int l = i ^ jj;
+          if (l > i) {
+            const float u = s[i], v = s[l];
+            const bool up = (i & k) == 0;
+            if (up ? (u > v) : (u < v)) {
+              s[i] = v;
+              s[l] = u;
+            }
+          }
+        }
+        __syncthreads();
+      }
+    }
+    const int h = (int)(m / 2);
+    med = (m & 1) ? (double)s[h] : (double)(s[h - 1] + s[h]) / 2.0;
+  }
+  if (t == 0) a.facts[f].medians[j] = med;
+}
+
+}  // namespace ldgpu
+
+using namespace ldgpu;
+
+extern "C" size_t ld_pulse_workspace_bytes(const ld_handle*, int64_t n) {
+  const int64_t tiles = n > 0 ? (n + kEdgeTile - 1) / kEdgeTile : 1;
+  return (size_t)(tiles + 1) * sizeof(long long);
+}
+
+extern "C" int ld_find_pulses(ld_handle* h, const float* x, int64_t n, double high, int64_t* edges, int64_t cap,
+                              int64_t* count, void* workspace, size_t ws_bytes, void* stream) {
+  if (!h || !x || n <= 0 || !count || cap < 0 || (cap > 0 && !edges) || !workspace) {
+    set_error("ld_find_pulses: bad arguments");
+    return LD_EINVAL;
+  }
+  if (((uintptr_t)x & 15) != 0) {
+    set_error("ld_find_pulses: x must be 16-byte aligned");
+    return LD_EINVAL;
+  }
+  if (ws_bytes < ld_pulse_workspace_bytes(h, n)) {
+    set_error("ld_find_pulses: workspace %zu bytes < %zu needed", ws_bytes, ld_pulse_workspace_bytes(h, n));
+    return LD_EINVAL;
+  }
+  const long long tiles = (n + kEdgeTile - 1) / kEdgeTile;
+  if (tiles > 0x7fffffffLL) {
+    set_error("ld_find_pulses: buffer too long");
+    return LD_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  long long* counts = (long long*)workspace;
+  hipLaunchKernelGGL(k_edge_count, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, x, (long long)n, high, counts);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_edge_scan, dim3(1), dim3(kScanThreads), 0, st, counts, tiles, (long long*)count);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_edge_write, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, x, (long long)n, high,
+                     (const long long*)counts, (long long*)edges, (long long)cap);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+extern "C" int ld_pulse_means(ld_handle* h, const float* x, int64_t n, const int64_t* edges, const int64_t* count,
+                              int64_t cap_pulses, double min_len, double trim, float* means, void* stream) {
+  if (!h || !x || n <= 0 || !edges || !count || cap_pulses < 0 || (cap_pulses > 0 && !means)) {
+    set_error("ld_pulse_means: bad arguments");
+    return LD_EINVAL;
+  }
+  if (cap_pulses == 0) return LD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  long long grid = (cap_pulses + kMeanWaves - 1) / kMeanWaves;
+  if (grid > 2048) grid = 2048;
+  hipLaunchKernelGGL(k_pulse_means, dim3((unsigned)grid), dim3(64 * kMeanWaves), 0, st, x, (long long)n,
+                     (const long long*)edges, (const long long*)count, (long long)cap_pulses, min_len, trim, means);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+extern "C" int ld_range_means(ld_handle* h, const float* x, int64_t n, const int64_t* begin, const int64_t* end,
+                              int count, float* out, void* stream) {
+  if (!h || !x || n <= 0 || count < 0 || (count > 0 && (!begin || !end || !out))) {
+    set_error("ld_range_means: bad arguments");
+    return LD_EINVAL;
+  }
+  if (count == 0) return LD_OK;
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  hipLaunchKernelGGL(k_range_means, dim3(count), dim3(64), 0, st, x, (long long)n, (const long long*)begin,
+                     (const long long*)end, count, out);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+extern "C" size_t ld_field_facts_workspace_bytes(const ld_handle*, int nfields) {
+  return (size_t)(nfields > 0 ? nfields : 1) * sizeof(ld_line_field);
+}
+
+extern "C" int ld_field_facts_measure(ld_handle* h, const float* demod, int64_t demod_total,
+                                      const ld_line_field* fields, int nfields, const double* linelocs,
+                                      int64_t nlinelocs, int pal, int outlinecount, double line_samples,
+                                      ld_field_facts* facts, void* workspace, size_t ws_bytes, void* stream) {
+  if (!h || !demod || !fields || nfields <= 0 || !linelocs || !facts || outlinecount <= 0 ||
+      !(line_samples > 0) || line_samples + 2 > kFactMax || !workspace) {
+    set_error("ld_field_facts_measure: bad arguments");
+    return LD_EINVAL;
+  }
+  if (ws_bytes < ld_field_facts_workspace_bytes(h, nfields)) {
+    set_error("ld_field_facts_measure: workspace %zu bytes < %zu needed", ws_bytes,
+              ld_field_facts_workspace_bytes(h, nfields));
+    return LD_EINVAL;
+  }
+  for (int f = 0; f < nfields; ++f) {
+    const ld_line_field& fd = fields[f];
+    if (fd.nlines <= 0 || fd.lineloc_offset < 0 || fd.lineloc_offset + fd.nlines > nlinelocs) {
+      set_error("ld_field_facts_measure: field %d: line range outside the arrays", f);
+      return LD_EINVAL;
+    }
+    if (fd.demod_offset < 0 || fd.demod_len < 0 || fd.demod_offset + fd.demod_len > demod_total) {
+      set_error("ld_field_facts_measure: field %d: demod span outside the buffer", f);
+      return LD_ERANGE;
+    }
+  }
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  // a pageable source is staged before hipMemcpyAsync returns
+  ld_line_field* dfields = (ld_line_field*)workspace;
+  LD_HIP(hipMemcpyAsync(dfields, fields, sizeof(ld_line_field) * nfields, hipMemcpyHostToDevice, st));
+  FactArgs a{demod, linelocs, facts, line_samples, outlinecount, pal ? 1 : 0};
+  hipLaunchKernelGGL(k_field_facts, dim3(nfields * 8), dim3(kFactThreads), 0, st, a, (const ld_line_field*)dfields);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}

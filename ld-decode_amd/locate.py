@@ -1,0 +1,331 @@
+"""Line location: the GPU finds sync pulses, their levels and the skip-check
+medians; libldgpu's host code runs the pulse state machine, the
+vertical-blanking analysis, line 0 and the field walk.
+
+Mirrors Field.getpulses / compute_linelocs (lddecode/core.py:2242-2481) and
+the offsets of LDdecode.decodefield / readfield (core.py:4018-4191).
+
+A field's line 0 can depend on the previous field's *final* line locations
+and on its skip_check (core.py:2163-2207), which only exist after that
+field's hsync and burst / pilot alignment.  The batched walk therefore runs
+ahead on provisional values (the previous field's linelocs1 at its linecount
+plus an offset, skip score 100), the GPU aligns every field, and the walk is
+repeated with the measured values (:class:`FieldFacts`); fields whose result
+changes are located and aligned again.  The result is the reference's
+sequential answer.
+"""
+
+import ctypes
+
+import numpy as np
+
+from . import _lib
+
+NTSC_LINES = 273   # proclines: outlinecount + 10
+PAL_LINES = 326    # outlinecount + 10 + 3
+
+
+def proclines(rf):
+    return rf.SysParams["frame_lines"] // 2 + 1 + 10 + (3 if rf.system == "PAL" else 0)
+
+
+def readlen(rf):
+    """LDdecode.readlen (core.py:3625-3631)."""
+    if rf.system == "PAL":
+        return rf.linelen * 400
+    return ((rf.linelen * 350) // 16384) * 16384
+
+
+def locate_params(rf, prov_delta=0.0):
+    sp, dp = rf.SysParams, rf.DecoderParams
+    p = _lib.LocateParams()
+    p.freq = float(rf.freq)
+    p.linelen = float(rf.linelen)
+    p.ire0, p.hz_ire, p.vsync_ire = float(dp["ire0"]), float(dp["hz_ire"]), float(dp["vsync_ire"])
+    p.hsync_us, p.eq_us, p.vsync_us = float(sp["hsyncPulseUS"]), float(sp["eqPulseUS"]), float(sp["vsyncPulseUS"])
+    p.line_period_us = float(sp["line_period"])
+    p.first_field_h[0], p.first_field_h[1] = (float(v) for v in sp["firstFieldH"])
+    p.hsync_tolerance = float(getattr(rf, "hsync_tolerance", 0.4))
+    p.prov_delta = float(prov_delta)
+    p.pal = 1 if rf.system == "PAL" else 0
+    p.num_pulses = int(sp["numPulses"])
+    p.field_lines[0], p.field_lines[1] = (int(v) for v in sp["field_lines"])
+    p.frame_lines = int(sp["frame_lines"])
+    p.blocklen = int(rf.blocklen)
+    p.blockcut = int(rf.blockcut)
+    p.block_stride = int(rf.block_stride)
+    p.readlen = int(readlen(rf))
+    return p
+
+
+def pulse_threshold(rf):
+    """getpulses' first-pass level: iretohz(-20) (core.py:2245-2248)."""
+    return rf.DecoderParams["ire0"] + rf.DecoderParams["hz_ire"] * -20.0
+
+
+def _ptr(a):
+    return ctypes.c_void_p(a.ctypes.data) if a is not None else None
+
+
+class PulseFinder:
+    """Device buffers and pinned host copies for the pulse pass over one
+    stitched demod_05 buffer of up to `n` samples."""
+
+    def __init__(self, rf, n, cap_edges=None):
+        torch = rf.torch
+        self.rf, self.torch, self.n = rf, torch, int(n)
+        self.cap = int(cap_edges or max(4096, n // 400))
+        self.cap -= self.cap % 2
+        dev = rf.device
+        self.edges = torch.empty(self.cap, dtype=torch.int64, device=dev)
+        self.count = torch.zeros(1, dtype=torch.int64, device=dev)
+        self.means = torch.empty(self.cap // 2, dtype=torch.float32, device=dev)
+        self.ws = torch.empty(rf.lib.ld_pulse_workspace_bytes(rf.handle, self.n), dtype=torch.uint8, device=dev)
+        self.h_edges = torch.empty(self.cap, dtype=torch.int64, pin_memory=True)
+        self.h_count = torch.empty(1, dtype=torch.int64, pin_memory=True)
+        self.h_means = torch.empty(self.cap // 2, dtype=torch.float32, pin_memory=True)
+        self.done = None
+
+    def launch(self, demod05, n=None, stream=None, high=None):
+        """Enqueue the pulse search, the sync levels and their copies to the
+        host on `stream` (default: current)."""
+        torch, rf = self.torch, self.rf
+        n = int(n if n is not None else demod05.numel())
+        assert n <= self.n and demod05.dtype == torch.float32 and demod05.is_contiguous()
+        st = stream if stream is not None else torch.cuda.current_stream(rf.device)
+        sp = ctypes.c_void_p(st.cuda_stream)
+        hi = pulse_threshold(rf) if high is None else high
+        _lib.check(rf.lib.ld_find_pulses(rf.handle, demod05.data_ptr(), n, float(hi), self.edges.data_ptr(),
+                                         self.cap, self.count.data_ptr(), self.ws.data_ptr(), self.ws.numel(), sp))
+        _lib.check(rf.lib.ld_pulse_means(rf.handle, demod05.data_ptr(), n, self.edges.data_ptr(),
+                                         self.count.data_ptr(), self.cap // 2, float(rf.freq * 10.0),
+                                         float(rf.freq), self.means.data_ptr(), sp))
+        with torch.cuda.stream(st):
+            self.h_count.copy_(self.count, non_blocking=True)
+            self.h_edges.copy_(self.edges, non_blocking=True)
+            self.h_means.copy_(self.means, non_blocking=True)
+            self.done = torch.cuda.Event()
+            self.done.record(st)
+        self._args = (demod05, n, st, hi)
+
+    def result(self):
+        """Wait for the pass; (edges int64, means float32) as numpy arrays."""
+        self.done.synchronize()
+        cnt = int(self.h_count[0])
+        if cnt > self.cap:          # more edges than room: grow and search again
+            demod05, n, st, hi = self._args
+            self.__init__(self.rf, self.n, cap_edges=cnt + 1024)
+            self.launch(demod05, n, st, hi)
+            return self.result()
+        return self.h_edges.numpy()[:cnt], self.h_means.numpy()[:cnt // 2]
+
+
+def _facts_array(facts, n):
+    """n FieldFacts: the given ones first, the rest unknown."""
+    if facts is None:
+        return None
+    arr = (_lib.FieldFacts * max(n, 1))()
+    for i, f in enumerate(list(facts)[:n]):
+        arr[i] = f
+    return arr
+
+
+def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None):
+    """ld_locate_fields: returns (locs, linelocs1, linebad, prev_out) with
+    locs a list of _lib.FieldLoc, linelocs1 / linebad arrays of shape
+    (fields, proclines)."""
+    P = params or locate_params(rf)
+    L = proclines(rf)
+    out = (_lib.FieldLoc * max(max_fields, 1))()
+    ll = np.empty((max(max_fields, 1), L), dtype=np.float64)
+    bad = np.empty((max(max_fields, 1), L), dtype=np.uint8)
+    po = _lib.PrevField()
+    nf = ctypes.c_int(0)
+    edges = np.ascontiguousarray(edges, dtype=np.int64)
+    means = np.ascontiguousarray(means, dtype=np.float32)
+    fa = _facts_array(facts, max(max_fields, 1))
+    _lib.check(rf.lib.ld_locate_fields(ctypes.byref(P), _ptr(edges), len(edges), _ptr(means), int(total),
+                                       float(start), ctypes.byref(prev) if prev is not None else None, fa,
+                                       max_fields, out, _ptr(ll), _ptr(bad), L, ctypes.byref(po), ctypes.byref(nf)))
+    n = nf.value
+    for k in range(n):
+        if out[k].status != _lib.LD_LOC_OK:
+            ll[k] = np.nan
+            bad[k] = 0
+    return list(out[:n]), ll[:n], bad[:n], po
+
+
+def window_pulses(rf, edges, startloc, buflen):
+    """The field buffer's pulse list as getpulses returns it (starts
+    relative to the buffer): runs that start after its first sample, end
+    inside it and last at most 5000 samples."""
+    s, e = edges[0:len(edges) - 1:2], edges[1::2]
+    s = s[:len(e)]
+    keep = (s > startloc) & (e < startloc + buflen) & ((e - s) <= 5000)
+    return s[keep] - startloc, (e - s)[keep]
+
+
+def locate_one(rf, starts, lens, prev=None, startloc=0, fields_written=1, params=None):
+    """ld_locate_field: compute_linelocs from one field's pulse list."""
+    P = params or locate_params(rf)
+    L = proclines(rf)
+    out = _lib.FieldLoc()
+    ll = np.empty(L, dtype=np.float64)
+    bad = np.empty(L, dtype=np.uint8)
+    starts = np.ascontiguousarray(starts, dtype=np.int64)
+    lens = np.ascontiguousarray(lens, dtype=np.int64)
+    _lib.check(rf.lib.ld_locate_field(ctypes.byref(P), _ptr(starts), _ptr(lens), len(starts),
+                                      ctypes.byref(prev) if prev is not None else None, int(startloc),
+                                      int(fields_written), ctypes.byref(out), _ptr(ll), _ptr(bad), L))
+    return out, ll, bad
+
+
+def _f32_median(v):
+    return np.float32(np.median(np.asarray(v, dtype=np.float32)))
+
+
+def relevel_pulses(rf, demod05, edges, means, loc, stream=None):
+    """getpulses' second pass for a field whose vsync level is off
+    (core.py:2292-2322): black levels from the equalising pulses around the
+    vertical sync (GPU means), the threshold halfway between black and sync,
+    and findpulses again over the field buffer (GPU).  Returns the field's
+    (starts, lens)."""
+    torch = rf.torch
+    ws, n = int(loc.startloc), int(loc.buflen)
+    s, e = edges[0:len(edges) - 1:2], edges[1::2]
+    s = s[:len(e)]
+    k = np.flatnonzero((s > ws) & (e < ws + n) & ((e - s) <= 5000))
+    st, ln = s[k] - ws, (e - s)[k]
+    minlength = rf.freq * 10.0
+    vl = np.flatnonzero(ln > minlength)
+    synclevel = _f32_median(means[k[vl]])
+    r = [i for i in list(range(vl[0] - 5, vl[0])) + list(range(vl[-1] + 1, vl[-1] + 6)) if 0 <= i < len(st)]
+    r = [i for i in r if rf.freq * 0.75 <= ln[i] <= rf.freq * 2.5]
+    begin = np.array([int(st[i] + rf.freq * 5) + ws for i in r], dtype=np.int64)
+    end = np.array([int(st[i] + rf.freq * 20) + ws for i in r], dtype=np.int64)
+    dev = rf.device
+    cur = stream if stream is not None else torch.cuda.current_stream(dev)
+    out = torch.empty(max(len(r), 1), dtype=torch.float32, device=dev)
+    if len(r):
+        b = torch.from_numpy(begin).to(dev)
+        en = torch.from_numpy(end).to(dev)
+        _lib.check(rf.lib.ld_range_means(rf.handle, demod05.data_ptr(), demod05.numel(), b.data_ptr(),
+                                         en.data_ptr(), len(r), out.data_ptr(), ctypes.c_void_p(cur.cuda_stream)))
+    cur.synchronize()
+    blacklevel = _f32_median(out[:len(r)].cpu().numpy()) if len(r) else np.float32(np.nan)
+    high = np.float32((blacklevel + synclevel) / np.float32(2))
+    pf = PulseFinder(rf, n)
+    pf.launch(demod05[ws:ws + n], n, cur, float(high))
+    e2, _ = pf.result()
+    s2, t2 = e2[0:len(e2) - 1:2], e2[1::2]
+    s2 = s2[:len(t2)]
+    keep = (s2 != 0) & ((t2 - s2) <= 5000)
+    return s2[keep], (t2 - s2)[keep]
+
+
+def recalibrated_ire0(rf, demod05, loc):
+    """getpulses' first-field retry (core.py:2250-2257): ire0 = the 15th
+    percentile of the field buffer (numpy's linear method: the two order
+    statistics around 0.15 * (n - 1), interpolated in float64)."""
+    torch = rf.torch
+    ws, n = int(loc.startloc), int(loc.buflen)
+    x = demod05[ws:ws + n]
+    pos = 0.15 * (n - 1)
+    lo = int(np.floor(pos))
+    vals = torch.sort(x).values[lo:lo + 2].double().cpu().numpy()
+    frac = pos - lo
+    return float(vals[0] + (vals[1] - vals[0]) * frac) if len(vals) > 1 else float(vals[0])
+
+
+def skip_scores(rf, facts, params=None):
+    P = params or locate_params(rf)
+    out = []
+    for f in facts:
+        m = (ctypes.c_double * 8)(*f.medians)
+        out.append(rf.lib.ld_skip_score(ctypes.byref(P), m))
+    return out
+
+
+class FieldWalker:
+    """The decodefield walk over one stitched demod_05 buffer, resolving
+    the getpulses retries on the GPU.  walk() returns every field visited
+    (valid or not) with its linelocs1 / linebad."""
+
+    def __init__(self, rf, total, max_fields=64):
+        self.rf = rf
+        self.total = int(total)
+        self.max_fields = max_fields
+        self.finder = PulseFinder(rf, total)
+        self.prov_delta = 0.0
+
+    def pulses(self, demod05, stream=None):
+        self.finder.launch(demod05, self.total, stream)
+
+    def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None):
+        rf = self.rf
+        edges, means = self.finder.result()
+        P = locate_params(rf, self.prov_delta)
+        locs, lls, bads = [], [], []
+        while len(locs) < self.max_fields:
+            fk = None if facts is None else facts[len(locs):]
+            requested = self.max_fields - len(locs)
+            got, ll, bad, po = walk(rf, edges, means, self.total, start, prev, fk, requested, P)
+            if not got:
+                break
+            tail = got[-1]
+            if tail.status in (_lib.LD_LOC_NEED_LEVELS, _lib.LD_LOC_NEED_IRE0):
+                locs += got[:-1]
+                lls.append(ll[:-1])
+                bads.append(bad[:-1])
+                written = po.fields_written
+                pv = po if len(got) > 1 else prev
+                if tail.status == _lib.LD_LOC_NEED_IRE0:
+                    rf.DecoderParams["ire0"] = recalibrated_ire0(rf, demod05, tail)
+                    P = locate_params(rf, self.prov_delta)
+                    self.finder.launch(demod05, self.total, stream)
+                    edges, means = self.finder.result()
+                    start = tail.start
+                    prev = pv
+                    continue
+                st, ln = relevel_pulses(rf, demod05, edges, means, tail, stream)
+                loc, l1, b1 = locate_one(rf, st, ln, pv if (pv is not None and pv.valid) else None,
+                                         tail.startloc, written, P)
+                loc.start, loc.readloc, loc.startloc, loc.buflen = tail.start, tail.readloc, tail.startloc, tail.buflen
+                loc.offset = (loc.nextfieldoffset - (loc.readloc - loc.startloc)
+                              if loc.status == _lib.LD_LOC_OK else loc.nextfieldoffset)
+                locs.append(loc)
+                lls.append(l1[None])
+                bads.append(b1[None])
+                prev = _prev_from(loc, l1, rf, P, written, None if facts is None else
+                                  (facts[len(locs) - 1] if len(locs) - 1 < len(facts) else None))
+                start = loc.start + loc.offset
+                continue
+            locs += got
+            lls.append(ll)
+            bads.append(bad)
+            prev = po
+            start = got[-1].start + got[-1].offset
+            if len(got) < requested:
+                break       # the next field's buffer passes the end
+        L = proclines(rf)
+        ll = np.concatenate(lls) if lls else np.empty((0, L))
+        bad = np.concatenate(bads) if bads else np.empty((0, L), np.uint8)
+        return locs, ll, bad, prev, start
+
+
+def _prev_from(loc, ll1, rf, P, written, fact):
+    """The state a resolved field leaves (what ld_locate_fields keeps)."""
+    pv = _lib.PrevField()
+    pv.fields_written = written + (1 if loc.status == _lib.LD_LOC_OK else 0)
+    if loc.status != _lib.LD_LOC_OK:
+        return pv
+    lc = (312 if loc.is_first_field else 313) if rf.system == "PAL" else loc.linecount
+    pv.valid, pv.is_first_field, pv.linecount = 1, loc.is_first_field, lc
+    pv.sync_confidence, pv.startloc = loc.sync_confidence, loc.startloc
+    if fact is not None and fact.known:
+        pv.ll_linecount = fact.ll_linecount
+        pv.skip_score = skip_scores(rf, [fact], P)[0]
+    else:
+        pv.ll_linecount = ll1[lc] + loc.startloc + P.prov_delta
+        pv.skip_score = 100
+    return pv

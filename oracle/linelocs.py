@@ -422,6 +422,43 @@ def skip_check(p, demod, linelocs, lineoffset=0):
     return 0
 
 
+def field_facts(p, demod, final, isfirst):
+    """What a field leaves for its successor: (final linelocs[linecount],
+    the eight skip_check medians) over its own buffer (core.py:2122-2155,
+    :2196-2203; PAL's linecount 312 / 313 and lineoffset 2 / 3 from
+    FieldPAL.process)."""
+    if p.system == "PAL":
+        lc, lo = (312, 2) if isfirst else (313, 3)
+    else:
+        lc, lo = (263 if isfirst else 262), 0
+    length = p.usectoinpx(p.line_period)
+    meds = []
+    for line in range(p.outlinecount, p.outlinecount + 8):
+        b = final[line + lo]
+        meds.append(numba_median(demod[int(b):int(b + length + 1)]))
+    return float(final[lc]), meds
+
+
+def skip_score_from(p, meds):
+    """skip_check's score from its eight medians (core.py:2139-2155)."""
+    score = vsync_lines = 0
+    for m in meds:
+        ire = p.hztoire(m)
+        if inrange(ire, p.vsync_ire - 10, p.vsync_ire / 2):
+            vsync_lines += 1
+        elif inrange(ire, -5, 5):
+            score += 1
+        else:
+            score -= 1
+    if vsync_lines >= 2:
+        return 100
+    if vsync_lines == 1 and score > 0:
+        return 50
+    if score > 0:
+        return 25
+    return 0
+
+
 @dataclass
 class FieldLines:
     """compute_linelocs results plus the state later fields read."""
@@ -591,11 +628,13 @@ def walk_fields(p, demod_05, nfields, start, finish, demod=None):
         rec = dict(start=start, startloc=startloc, nblocks=nb, res=res, valid=valid)
         if valid:
             final = finish(len(out), res, startloc)
-            lo, lc = 0, res.linecount
+            lc = res.linecount
             if p.system == "PAL":                   # FieldPAL.process, core.py:3230-3231
                 lc = 312 if res.isFirstField else 313
-                lo = 2 if res.isFirstField else 3
-            skip = skip_check(p, demod[startloc:startloc + nb * S], final, lo) if demod is not None else 100
+            skip = 100
+            if demod is not None:
+                _, meds = field_facts(p, demod[startloc:startloc + nb * S], final, res.isFirstField)
+                skip = skip_score_from(p, meds)
             prev = PrevField(True, startloc, final, lc, res.isFirstField, res.sync_confidence, skip)
             rec["final"] = final
             offset = res.nextfieldoffset - (readloc - startloc)
