@@ -2,17 +2,19 @@
 """Throughput of the RF demod + TBC hot path on one MI355X.
 
 Workload (one "step"): full NTSC field decode, batched -- a synthetic NTSC
-capture of `--fields` consecutive fields (667,333 int16 samples at 40 MSPS
-each, sync + burst + 75% bars, audio FM carriers, 40 dB AWGN, +-50 ns line
-jitter) resident in HBM is demodulated block by block (32768-sample blocks
-at stride 31712, FP32, all outputs of RFDecode.demodblock incl. audio
-stage 1), every field's line locations are refined in FP64 (hsync edge,
-colour-burst alignment, bad-line repair, subcarrier shift), and every field
-is resampled to a 910 x 263 uint16 4fsc frame.  Consecutive steps overlap on
-two streams with two buffer sets (the line alignment and TBC of step k-1
-run beside the demod of step k); the per-stage times come from a separate
-sequential pass.  The "plain" figure decodes a single field per step,
-sequentially.
+capture (667,333 int16 samples at 40 MSPS per field, sync + burst + 75%
+bars, audio FM carriers, 40 dB AWGN, +-50 ns line jitter) resident in HBM is
+demodulated block by block (32768-sample blocks at stride 31712, FP32, all
+outputs of RFDecode.demodblock incl. audio stage 1); `--fields` fields are
+located in it (sync pulses on the GPU, the reference's field walk and line-0
+logic on the host, verified against each field's final line positions),
+every field's line locations are refined in FP64 (hsync edge, colour-burst
+alignment, bad-line repair, subcarrier shift), and every field is resampled
+to a 910 x 263 uint16 4fsc frame.  Consecutive steps overlap on two streams
+with two buffer sets (stream A demodulates step k while the host walks step
+k-1 and stream B aligns and resamples it); the per-stage times come from a
+separate sequential pass.  The "plain" figure decodes a single field per
+step (one field read of 29 blocks), sequentially.
 
 Prints one JSON line.  value = usable RF samples demodulated per second
 (Msamples/s, whole job, inputs already in HBM).
@@ -44,56 +46,66 @@ def algorithmic_bytes(nblocks, S, nfields, pixels_per_field, field_samples, audi
     return demod, tbc, U
 
 
-def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=False):
+def capture_samples(nfields):
+    """Capture length holding `nfields` complete field reads after the
+    partial field at its start (each read spans 29 blocks, about 1.4 fields)."""
+    return (nfields + 2) * NTSC_FIELD_SAMPLES + 100_000
+
+
+def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None):
+    """Time the located decode.  single: (start, startloc, buflen) of one
+    field read -- the plain figure decodes just that field per step."""
     from lddecode_amd import capture
-    fields, ll1 = capture.truth_fields(rf, cap, nfields, first_field=1)
     data = cap.data
-    if single:
-        # one field read: only the blocks its lines need, starting at its origin
-        S, N = rf.block_stride, rf.blocklen
-        org = fields[0]["demod_offset"]
-        nb = int(np.ceil((ll1.max() + 3000) / S)) + 1
-        data = data[org: org + (nb - 1) * S + N]
-        fields = [dict(fields[0], demod_offset=0)]
+    start = 0.0
+    if single is not None:
+        st, org, n = single
+        data = data[org: org + n + rf.blocklen - rf.block_stride]
+        start = st - org
     x = torch.from_numpy(np.ascontiguousarray(data)).to(rf.device)
     audio = True
-    dec = capture.CaptureDecoder(rf, x, fields, ll1, audio=audio)
+    dec = capture.CaptureDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start)
     stream = torch.cuda.current_stream(rf.device)
 
-    def step():
-        dec.demod()
-        dec.refine()
-        dec.lines()
-        dec.tbc()
-
     for _ in range(warmup):
-        step()
+        dec.step()
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(5)] for _ in range(steps)]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(steps)]
+    walk_s = []
     t0 = time.perf_counter()
     for i in range(steps):
         e = ev[i]
         e[0].record(stream)
         dec.demod()
         e[1].record(stream)
-        dec.refine()
+        dec.pulses()
         e[2].record(stream)
-        dec.lines()
+        t = time.perf_counter()
+        dec.locate()
+        walk_s.append(time.perf_counter() - t)
+        dec.refine()
         e[3].record(stream)
-        dec.tbc()
+        dec.lines()
         e[4].record(stream)
+        dec.tbc()
+        e[5].record(stream)
+        dec.measure_facts()
+        dec.verify()
+        e[6].record(stream)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    demod_ms = np.mean([e[0].elapsed_time(e[1]) for e in ev])
-    hsync_ms = np.mean([e[1].elapsed_time(e[2]) for e in ev])
-    lines_ms = np.mean([e[2].elapsed_time(e[3]) for e in ev])
-    tbc_ms = np.mean([e[3].elapsed_time(e[4]) for e in ev])
-    step_ms = np.mean([e[0].elapsed_time(e[4]) for e in ev])
+
+    def span(a, b):
+        return float(np.mean([e[a].elapsed_time(e[b]) for e in ev]))
+    out = dict(wall_s=wall, step_ms=span(0, 6), demod_ms=span(0, 1), pulses_ms=span(1, 2),
+               locate_hsync_ms=span(2, 3), lines_ms=span(3, 4), tbc_ms=span(4, 5), facts_ms=span(5, 6),
+               walk_host_ms=1e3 * float(np.mean(walk_s)), relocated=dec.relocated)
     pipe_ms = None
-    if not single:
-        # consecutive passes overlapped on two streams (line alignment + TBC of
-        # pass k-1 beside demod + hsync of pass k), two buffer sets
-        pipe = capture.PipelinedDecoder(rf, x, fields, ll1, audio=audio)
+    if single is None:
+        # consecutive passes overlapped on two streams: stream A demodulates
+        # pass k and finds its pulses while the host walks pass k-1 and
+        # stream B aligns and resamples it, two buffer sets
+        pipe = capture.PipelinedDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start)
         for _ in range(warmup):
             pipe.step()
         pipe.join()
@@ -106,22 +118,27 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=False):
         e1.record(stream)
         torch.cuda.synchronize()
         pipe_ms = e0.elapsed_time(e1) / steps
+        out["relocated_pipe"] = sum(d.relocated for d in pipe.sets)
         del pipe
     W, lines = rf.SysParams["outlinelen"], rf.SysParams["frame_lines"] // 2 + 1
-    db, tb, U = algorithmic_bytes(dec.nblocks, rf.block_stride, nfields, W * lines, NTSC_FIELD_SAMPLES, audio)
-    return dict(wall_s=wall, step_ms=step_ms, pipe_ms=pipe_ms, demod_ms=demod_ms, hsync_ms=hsync_ms, lines_ms=lines_ms, tbc_ms=tbc_ms,
-                demod_bytes=db, tbc_bytes=tb, usable=U, nblocks=dec.nblocks, fields=fields, ll1=ll1,
-                bad_lines=float(dec.linebad.float().mean().item()))
+    nf = len(dec.fields)
+    db, tb, U = algorithmic_bytes(dec.nblocks, rf.block_stride, nf, W * lines, NTSC_FIELD_SAMPLES, audio)
+    locs = dec.walk_state[0]
+    first = next(c for c in locs if c.status == 0)
+    out.update(pipe_ms=pipe_ms, demod_bytes=db, tbc_bytes=tb, usable=U, nblocks=dec.nblocks, nfields=nf,
+               fields=dec.fields, walk=dec.walk_state, first_field=(first.start, first.startloc, first.buflen),
+               bad_lines=float(dec.linebad_final[:nf * dec.L].float().mean().item()))
+    return out
 
 
-def cpu_baseline(cap, fields, ll1, nsample, workers, min_seconds, max_seconds):
-    """The same step on the host: the FP64 numpy/scipy oracle demodulates the
-    blocks of the first `nsample` fields of the batch on a thread pool (one
-    block per task, as DemodCache's workers), then refines every field's
-    hsync line locations, aligns them to the colour burst and resamples it to
-    uint16 (one field per task; the
-    resampler loop is the oracle's compiled C restatement, as the reference
-    runs it compiled with numba).
+def cpu_baseline(cap, walk, nsample, workers, min_seconds, max_seconds):
+    """The same step on the host with the FP64 numpy/scipy oracle: the
+    blocks of the first `nsample` located fields' reads are demodulated on a
+    thread pool (one block per task, as DemodCache's workers), then each field
+    (one per task) is located (compute_linelocs on its buffer), its lines are
+    refined (hsync edge, colour-burst alignment, bad-line repair, subcarrier
+    shift) and it is resampled to uint16 (the resampler loop is the oracle's
+    compiled C restatement, as the reference runs it compiled with numba).
     Passes over this fixed sample repeat until `min_seconds` have elapsed
     (stopping early past `max_seconds`).  Returns (Msamples/s, seconds,
     passes, blocks per pass)."""
@@ -130,6 +147,7 @@ def cpu_baseline(cap, fields, ll1, nsample, workers, min_seconds, max_seconds):
     from oracle import ctbc
     from oracle import dsp as O
     from oracle import lines as L
+    from oracle import linelocs as LL
     from oracle import tbc as T
     from oracle.params import out_scale
     ctbc.lib()
@@ -140,28 +158,26 @@ def cpu_baseline(cap, fields, ll1, nsample, workers, min_seconds, max_seconds):
     vs = dp["vsync_ire"]
     osc = out_scale("NTSC", vs)
     lut = T.build_kaiser_lut()
-    nl = fields[0]["nlines"]
-    plan = []
-    for f in range(nsample):
-        ll = np.asarray(ll1[f * nl:(f + 1) * nl])
-        plan.append((fields[f]["demod_offset"], ll))
-    b0 = plan[0][0] // S
-    b1 = max((org + int(ll.max()) + 3000) // S + 1 for org, ll in plan)
-    data = cap.data[b0 * S: b1 * S + N].astype(np.float64)
-    blocks = list(range(b1 - b0 + 1))
-
     lp = L.LineRefineParams(orf)
+    pp = LL.LocParams.from_rf(orf)
+    plan = [(int(c.startloc), int(c.buflen)) for c in walk[0] if c.status == 0][:nsample]
+    b0 = plan[0][0] // S
+    b1 = max((org + n) // S for org, n in plan)
+    data = cap.data[b0 * S: b1 * S + N - S].astype(np.float64)
+    blocks = list(range(b1 - b0))
 
     def demod(b):
         r = orf.demodblock(data=data[b * S:b * S + N], cut=True)
         return r["video"]["demod"], r["video"]["demod_05"], r["video"]["demod_burst"]
 
-    def tbc(args):
-        off, ll, demod_all, d05_all, burst_all = args
-        ll2, bad = T.refine_linelocs_hsync(d05_all[off:], ll, np.zeros(len(ll), bool), "NTSC", 40.0,
-                                           orf.iretohz, vs)
+    def field(args):
+        off, n, demod_all, d05_all, burst_all = args
+        res = LL.compute_linelocs(pp, d05_all[off:off + n], None, off, 1)
+        bad = L.compute_deriv_error(res.linelocs1, res.linebad)
+        ll2, bad = T.refine_linelocs_hsync(d05_all[off:], res.linelocs1, bad, "NTSC", 40.0, orf.iretohz, vs)
         bad = L.compute_deriv_error(ll2, bad)
-        ll4, _, _, _ = L.ntsc_field_lines(lp, demod_all[off:], burst_all[off:], ll2, bad, 263, True, 0.0)
+        ll4, _, _, _ = L.ntsc_field_lines(lp, demod_all[off:], burst_all[off:], ll2, bad, res.linecount,
+                                          res.isFirstField, 0.0)
         locs, wow = T.computewow_scaled(ll4, orf.linelen, W, lines, 0)
         ds = ctbc.scale_field(demod_all[off:], locs, wow, lut, 0, W, lines * W)
         return T.hz_to_output_array(ds.astype(np.float32), dp["ire0"], dp["hz_ire"], sp["outputZero"], vs, osc)
@@ -174,7 +190,7 @@ def cpu_baseline(cap, fields, ll1, nsample, workers, min_seconds, max_seconds):
             demod_all = np.concatenate([o[0] for o in outs]).astype(np.float32)
             d05_all = np.concatenate([o[1] for o in outs]).astype(np.float32)
             burst_all = np.concatenate([o[2] for o in outs]).astype(np.float32)
-            list(ex.map(tbc, [(org - b0 * S, ll, demod_all, d05_all, burst_all) for org, ll in plan]))
+            list(ex.map(field, [(org - b0 * S, n, demod_all, d05_all, burst_all) for org, n in plan]))
             passes += 1
             dt = time.perf_counter() - t0
             if dt >= min_seconds or dt >= max_seconds:
@@ -196,9 +212,9 @@ def profile_step(args):
     from lddecode_amd.rfdecode import RFDecode
     rf = RFDecode(system="NTSC", decode_analog_audio=44100, device=0)
     nf = args.fields
-    cap = synth.make_capture("NTSC", (nf + 2) * NTSC_FIELD_SAMPLES + 100_000, seed=args.seed, snr_db=40.0)
-    fields, ll1 = capture.truth_fields(rf, cap, nf, first_field=1)
-    dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).to(rf.device), fields, ll1, audio=True)
+    cap = synth.make_capture("NTSC", capture_samples(nf), seed=args.seed, snr_db=40.0)
+    dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).to(rf.device), audio=True, locate=True,
+                                 max_fields=nf)
     for _ in range(args.warmup + args.steps):
         dec.step()
     torch.cuda.synchronize()
@@ -306,14 +322,17 @@ def main():
     rf = RFDecode(system="NTSC", decode_analog_audio=44100, device=0)
     t = time.perf_counter()
     nf = args.fields
-    cap = synth.make_capture("NTSC", (nf + 2) * NTSC_FIELD_SAMPLES + 100_000, seed=args.seed, snr_db=40.0)
+    cap = synth.make_capture("NTSC", capture_samples(nf), seed=args.seed, snr_db=40.0)
     gen_s = time.perf_counter() - t
     print(f"# generated {len(cap.data)} samples in {gen_s:.1f} s", file=sys.stderr, flush=True)
 
     batched = run_gpu(args, torch, rf, cap, nf, args.steps, args.warmup)
-    print(f"# batched: {batched['pipe_ms']:.3f} ms/step overlapped, {batched['step_ms']:.3f} ms/step sequential",
+    print(f"# batched: {batched['pipe_ms']:.3f} ms/step overlapped, {batched['step_ms']:.3f} ms/step sequential, "
+          f"{batched['nfields']} fields located, host walk {batched['walk_host_ms']:.3f} ms",
           file=sys.stderr, flush=True)
-    plain = run_gpu(args, torch, rf, cap, 1, args.steps, args.warmup, single=True)
+    if batched["nfields"] != nf:
+        raise RuntimeError(f"located {batched['nfields']} fields, expected {nf}")
+    plain = run_gpu(args, torch, rf, cap, 1, args.steps, args.warmup, single=batched["first_field"])
     print(f"# plain: {plain['step_ms']:.3f} ms/step", file=sys.stderr, flush=True)
 
     value = batched["usable"] / (batched["pipe_ms"] / 1e3) / 1e6
@@ -327,12 +346,13 @@ def main():
     cpu = None
     if args.cpu_fields > 0:
         workers = min(os.cpu_count() or 1, 16)
-        rate, secs, passes, nblk = cpu_baseline(cap, batched["fields"], batched["ll1"], args.cpu_fields, workers,
+        rate, secs, passes, nblk = cpu_baseline(cap, batched["walk"], args.cpu_fields, workers,
                                                 args.cpu_seconds, 3 * args.cpu_seconds)
         cpu = {"value": round(rate, 3), "unit": "Msamples/s", "cores": workers, "kind": "port",
                "sample": f"FP64 numpy/scipy oracle on a {workers}-thread pool, the same step on the first "
-                         f"{args.cpu_fields} fields of the batch ({nblk} blocks demodulated, hsync refine, "
-                         f"TBC to uint16 per field), {passes} passes in {secs:.1f} s wall"}
+                         f"{args.cpu_fields} located fields of the batch ({nblk} blocks demodulated; per field "
+                         f"line location, hsync refine, burst alignment, TBC to uint16), {passes} passes in "
+                         f"{secs:.1f} s wall"}
 
     line = {
         "metric": METRIC,
@@ -350,7 +370,8 @@ def main():
         "config": {
             "workload": f"C2 full NTSC field decode, batched: {nf} fields per step, consecutive steps overlapped "
                         f"({batched['usable']} usable samples, {batched['nblocks']} blocks of 32768), "
-                        "FP32 demod + FP64 hsync line positions + sinc-LUT TBC to 910x263 uint16",
+                        "FP32 demod + line location (GPU sync pulses, host field walk) + FP64 hsync / burst "
+                        "line positions + sinc-LUT TBC to 910x263 uint16",
             "system": "NTSC",
             "blocklen": 32768,
             "fields_per_step": nf,
@@ -360,8 +381,12 @@ def main():
                   "fields_per_step": 1, "blocks_per_step": plain["nblocks"],
                   "x_realtime": round(plain_value / 40.0, 1)},
         "sequential_ms_per_step": round(batched["step_ms"], 4),
-        "stages_ms": {"demod": round(batched["demod_ms"], 4), "hsync": round(batched["hsync_ms"], 4),
-                      "burst_lines": round(batched["lines_ms"], 4), "tbc": round(batched["tbc_ms"], 4)},
+        "stages_ms": {"demod": round(batched["demod_ms"], 4), "pulses": round(batched["pulses_ms"], 4),
+                      "locate_walk_and_hsync": round(batched["locate_hsync_ms"], 4),
+                      "burst_lines": round(batched["lines_ms"], 4), "tbc": round(batched["tbc_ms"], 4),
+                      "facts_verify": round(batched["facts_ms"], 4)},
+        "host_walk_ms": round(batched["walk_host_ms"], 4),
+        "relocated_passes": batched["relocated"] + batched.get("relocated_pipe", 0),
         "roofline": {"bound": "hbm", "achieved": round(demod_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(demod_gbs / HBM_PEAK_GBS, 4), "traffic": demod_traffic,
                      "algorithmic_bytes": batched["demod_bytes"],

@@ -301,8 +301,8 @@ int ld_range_means(ld_handle* h, const float* x, int64_t n, const int64_t* begin
                    int count, float* out, void* stream);
 
 /* What a field leaves for the field after it (getLine0, core.py:2157-2239):
- * ll_linecount = its final linelocs[linecount] + its buffer origin (a
- * stitched index; linecount 263 / 262 NTSC, 312 / 313 PAL) and the eight
+ * ll_linecount = its final linelocs[linecount] (relative to its own buffer;
+ * linecount 263 / 262 NTSC, 312 / 313 PAL) and the eight
  * line medians skip_check (core.py:2122-2155) takes over demod[int(b),
  * int(b + line_samples + 1)) with b = linelocs[outlinecount + lineoffset + j].
  * known = 0 marks facts not measured yet. */
@@ -356,7 +356,7 @@ typedef struct ld_prev_field {
   int32_t linecount;          /* 263 / 262 NTSC, 312 / 313 PAL */
   int32_t sync_confidence;
   int64_t startloc;
-  double ll_linecount;        /* stitched index of its final linelocs[linecount] */
+  double ll_linecount;        /* its final linelocs[linecount], relative to startloc */
   int32_t skip_score;         /* its skip_check() */
   int32_t fields_written;     /* LDdecode.fields_written after it */
 } ld_prev_field;
@@ -368,9 +368,21 @@ enum {
   LD_LOC_NEED_IRE0 = 3        /* getpulses would recalibrate ire0 (first field without pulses) */
 };
 
+/* Flags of ld_field_loc.vb_flags: processVBlank's result for the local
+ * vertical blanking (index 0) and the next one (index 1). */
+enum {
+  LD_VB_FOUND = 1,            /* a line 0 was found */
+  LD_VB_FIRST = 2,            /* ... of a first field */
+  LD_VB_ZEROED = 4,           /* the fallback that zeroes sync_confidence ran */
+  LD_VB_BLANK = 8,            /* getBlankRange found a range (vb_firstblank valid) */
+  LD_VB_ERROR = 16            /* the reference raises here */
+};
+
 /* One field of the walk: decodefield's read and compute_linelocs' results.
  * Positions are relative to the field buffer (stitched index startloc),
- * as the reference's Field sees them. */
+ * as the reference's Field sees them.  The vb_* members keep getLine0's
+ * inputs (processVBlank without a limit for the local and next vertical
+ * blanking), from which ld_locate_recheck re-takes the line-0 decision. */
 typedef struct ld_field_loc {
   double start;               /* raw position decodefield was called with */
   int64_t readloc;
@@ -388,8 +400,23 @@ typedef struct ld_field_loc {
   int32_t skipdetected;
   int32_t npulses;            /* pulses in the field buffer */
   int32_t nlines;             /* proclines written */
+  int32_t analyzed;           /* 1: the pulses were classified and vb_* hold getLine0's inputs */
+  int32_t nvalid;             /* classified pulses kept for ld_locate_recheck (-1: none kept) */
+  int64_t last_start;         /* the buffer's last pulse start (rawpulses[-1].start) */
+  double vb_line0[2];
+  int64_t vb_firstblank[2];
+  int32_t vb_conf[2];         /* -1: None */
+  int32_t vb_flags[2];        /* LD_VB_* */
   int32_t reserved;
 } ld_field_loc;
+
+/* One classified pulse (refinepulses' validpulses entry, core.py:1914-1949). */
+typedef struct ld_vpulse {
+  int64_t start;
+  int32_t len;
+  int16_t kind;               /* 0 hsync, 1 equalising (before), 2 vsync, 3 equalising (after) */
+  int16_t good;               /* pulse_qualitycheck */
+} ld_vpulse;
 
 /* compute_linelocs for one field from its pulse list (HOST arrays; starts
  * relative to the field buffer, as getpulses returns them) and the previous
@@ -411,13 +438,30 @@ int ld_locate_field(const ld_locate_params* p, const int64_t* starts, const int6
  * measured facts.  The walk stops after max_fields fields, before a field
  * whose buffer would pass `total`, or after a field with status
  * LD_LOC_NEED_LEVELS / LD_LOC_NEED_IRE0 (the caller resolves it with
- * ld_range_means / ld_find_pulses and ld_locate_field).  *nfields_out gets
+ * ld_range_means / ld_find_pulses and ld_locate_field).  vpulses (may be
+ * NULL) receives field k's classified pulses at k * max_vpulses (its nvalid
+ * says how many, -1 when they did not fit).  *nfields_out gets
  * the number of fields written, *prev_out the state after the last one.
  * Host only. */
 int ld_locate_fields(const ld_locate_params* p, const int64_t* edges, int64_t nedges, const float* means,
                      int64_t total, double start, const ld_prev_field* prev, const ld_field_facts* facts,
                      int max_fields, ld_field_loc* out, double* linelocs1, uint8_t* linebad, int max_lines,
-                     ld_prev_field* prev_out, int* nfields_out);
+                     ld_vpulse* vpulses, int max_vpulses, ld_prev_field* prev_out, int* nfields_out);
+
+/* Check a walk made with provisional previous-field values against the
+ * measured facts (facts[k] belongs to locs[k]; prev is the state the walk
+ * started from; HOST arrays as ld_locate_fields wrote them, vpulses
+ * included).  Every field's line-0 decision (getLine0, core.py:2157-2239) is
+ * taken again from its kept inputs; where it changes, the line table is
+ * rebuilt from the kept pulses (core.py:2338-2481).  A field whose status,
+ * parity, confidence, next offset, linelocs1 or bad flags change stops the
+ * check: *first_changed gets its index (the walk must be redone from there
+ * with the measured facts).  Otherwise *first_changed is -1 and the records
+ * are updated in place (line0loc, vblank_next, skipdetected), so that they
+ * equal the sequential walk's.  Host only. */
+int ld_locate_recheck(const ld_locate_params* p, ld_field_loc* locs, double* linelocs1, uint8_t* linebad,
+                      int max_lines, const ld_vpulse* vpulses, int max_vpulses, int n, const ld_prev_field* prev,
+                      const ld_field_facts* facts, int* first_changed);
 
 /* skip_check's score (core.py:2139-2155) from a field's eight line medians. */
 int ld_skip_score(const ld_locate_params* p, const double* medians8);

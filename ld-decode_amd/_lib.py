@@ -24,7 +24,7 @@ EXPORTS = (
     "ld_refine_workspace_bytes", "ld_refine_burst_ntsc", "ld_refine_pilot_pal",
     "ld_pulse_workspace_bytes", "ld_find_pulses", "ld_pulse_means", "ld_range_means",
     "ld_field_facts_workspace_bytes", "ld_field_facts_measure", "ld_locate_field", "ld_locate_fields",
-    "ld_skip_score",
+    "ld_skip_score", "ld_locate_recheck",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -107,7 +107,13 @@ class FieldLoc(ctypes.Structure):
                 ("buflen", ctypes.c_int64), ("line0loc", ctypes.c_double), ("meanlinelen", ctypes.c_double),
                 ("nextfieldoffset", ctypes.c_double), ("offset", ctypes.c_double), ("vblank_next", ctypes.c_int64)] + [
         (n, ctypes.c_int32) for n in ("status", "is_first_field", "linecount", "sync_confidence", "skipdetected",
-                                      "npulses", "nlines", "reserved")]
+                                      "npulses", "nlines", "analyzed", "nvalid")] + [
+        ("last_start", ctypes.c_int64), ("vb_line0", ctypes.c_double * 2), ("vb_firstblank", ctypes.c_int64 * 2), ("vb_conf", ctypes.c_int32 * 2),
+        ("vb_flags", ctypes.c_int32 * 2), ("reserved", ctypes.c_int32)]
+
+
+class VPulse(ctypes.Structure):
+    _fields_ = [("start", ctypes.c_int64), ("len", ctypes.c_int32), ("kind", ctypes.c_int16), ("good", ctypes.c_int16)]
 
 
 _lib = None
@@ -151,9 +157,11 @@ def _declare(lib):
         "ld_locate_field": (c_int, [P(LocateParams), c_vp, c_vp, c_int, P(PrevField), c_i64, c_int, P(FieldLoc),
                                     c_vp, c_vp, c_int]),
         "ld_locate_fields": (c_int, [P(LocateParams), c_vp, c_i64, c_vp, c_i64, c_dbl, P(PrevField),
-                                     P(FieldFacts), c_int, P(FieldLoc), c_vp, c_vp, c_int, P(PrevField),
-                                     P(c_int)]),
+                                     P(FieldFacts), c_int, P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int,
+                                     P(PrevField), P(c_int)]),
         "ld_skip_score": (c_int, [P(LocateParams), P(c_dbl)]),
+        "ld_locate_recheck": (c_int, [P(LocateParams), P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int, c_int,
+                                      P(PrevField), P(FieldFacts), P(c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

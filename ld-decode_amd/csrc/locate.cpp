@@ -33,18 +33,47 @@ struct VPulse {
 };
 
 inline bool inrange(double a, double lo, double hi) { return a >= lo && a <= hi; }
-inline int64_t np_round(double x) { return (int64_t)std::nearbyint(x); }   // round half to even
+// round half to even, as np.round: adding and removing 1.5 * 2^52 rounds
+// to an integer in the default rounding mode (|x| < 2^51; beyond that x is
+// already an integer)
+inline int64_t np_round(double x) {
+  constexpr double kMagic = 6755399441055744.0;
+  if (!(std::fabs(x) < 2251799813685248.0)) return (int64_t)x;
+  return (int64_t)((x + kMagic) - kMagic);   // IEEE order kept: no fast-math
+}
+
+// The integers d with lo <= d / L <= hi in double arithmetic (the division
+// correctly rounded, so d / L is non-decreasing in d): [first, last].
+struct Span {
+  int64_t first, last;
+  bool has(int64_t d) const { return d >= first && d <= last; }
+};
+inline Span ratio_span(double L, double lo, double hi) {
+  int64_t a = (int64_t)std::floor(lo * L) - 2, b = (int64_t)std::ceil(hi * L) + 2;
+  while ((double)a / L < lo) ++a;
+  while ((double)(a - 1) / L >= lo) --a;
+  while ((double)b / L > hi) --b;
+  while ((double)(b + 1) / L <= hi) ++b;
+  return {a, b};
+}
 
 struct Geometry {
   const ld_locate_params& p;
   double linefreq;       // Field.get_linefreq() without a line: (freq / linelen) * linelen
   int outlinecount;      // frame_lines // 2 + 1
   int proclines;         // outlinecount + 10 (+3 PAL), lineoffset 0
+  // pulse_qualitycheck's line-length ranges and computeLineLen's, as
+  // integer sample spans (the tests divide integer gaps by linelen)
+  Span q_half, q_full, q_mixed, line_ok;
   explicit Geometry(const ld_locate_params& q)
       : p(q),
         linefreq((q.freq / q.linelen) * q.linelen),
         outlinecount(q.frame_lines / 2 + 1),
-        proclines(q.frame_lines / 2 + 1 + 10 + (q.pal ? 3 : 0)) {}
+        proclines(q.frame_lines / 2 + 1 + 10 + (q.pal ? 3 : 0)),
+        q_half(ratio_span(q.linelen, 0.4, 0.6)),
+        q_full(ratio_span(q.linelen, 0.9, 1.1)),
+        q_mixed(ratio_span(q.linelen, 0.4, 1.1)),
+        line_ok(ratio_span(q.linelen, 0.95, 1.05)) {}
   double us(double x) const { return x * linefreq; }
 };
 
@@ -52,24 +81,44 @@ struct Timings {
   double hsync[2], eq[2], vsync[2];
 };
 
-// Field.get_timings (core.py:1761-1810).
-Timings get_timings(const Geometry& g, const std::vector<Pulse>& pulses) {
+// Per-call scratch, reused from field to field.
+struct Scratch {
+  std::vector<VPulse> valid, vb;
+  std::vector<int64_t> loc, hist;
+  std::vector<double> dist, fill;
+  std::vector<uint8_t> have, err;
+  std::vector<Pulse> pulses;
+  std::vector<float> means;
+};
+
+// Field.get_timings (core.py:1761-1810).  The median of the hsync-like
+// lengths (integers in [cmin, cmax]) comes from a histogram.
+Timings get_timings(const Geometry& g, const std::vector<Pulse>& pulses, Scratch& sc) {
   const ld_locate_params& p = g.p;
   const double typical = g.us(p.hsync_us);
   const double cmin = g.us(p.hsync_us - 1.75), cmax = g.us(p.hsync_us + 2);
-  std::vector<int64_t> hl;
-  for (const Pulse& q : pulses)
-    if (inrange((double)q.len, cmin, cmax)) hl.push_back(q.len);
+  const int64_t lo = (int64_t)std::ceil(cmin), hi = (int64_t)std::floor(cmax);
   double med = p.hsync_us;     // the reference falls back to microseconds here
-  if (!hl.empty()) {
-    const size_t m = hl.size() / 2;
-    std::nth_element(hl.begin(), hl.begin() + m, hl.end());
-    if (hl.size() % 2) {
-      med = (double)hl[m];
-    } else {
-      const int64_t hi = hl[m];
-      const int64_t lo = *std::max_element(hl.begin(), hl.begin() + m);
-      med = ((double)lo + (double)hi) / 2;
+  if (hi >= lo) {
+    std::vector<int64_t>& h = sc.hist;
+    h.assign((size_t)(hi - lo + 1), 0);
+    int64_t cnt = 0;
+    for (const Pulse& q : pulses)
+      if (q.len >= lo && q.len <= hi) {
+        ++h[q.len - lo];
+        ++cnt;
+      }
+    if (cnt) {
+      // the k-th smallest (0-based) for k = cnt / 2 and, for an even count, k - 1
+      auto kth = [&](int64_t k) {
+        int64_t c = 0;
+        for (size_t i = 0; i < h.size(); ++i) {
+          c += h[i];
+          if (c > k) return (double)(lo + (int64_t)i);
+        }
+        return (double)hi;
+      };
+      med = (cnt % 2) ? kth(cnt / 2) : (kth(cnt / 2 - 1) + kth(cnt / 2)) / 2;
     }
   }
   Timings t;
@@ -85,15 +134,10 @@ Timings get_timings(const Geometry& g, const std::vector<Pulse>& pulses) {
 
 // Field.pulse_qualitycheck (core.py:1812-1823).
 bool quality(const Geometry& g, int prev_kind, int64_t prev_start, int kind, int64_t start) {
-  double lo, hi;
-  if (prev_kind > 0 && kind > 0) {
-    lo = 0.4, hi = 0.6;
-  } else if (prev_kind == 0 && kind == 0) {
-    lo = 0.9, hi = 1.1;
-  } else {
-    lo = 0.4, hi = 1.1;
-  }
-  return inrange((double)(start - prev_start) / g.p.linelen, lo, hi);
+  const int64_t d = start - prev_start;
+  if (prev_kind > 0 && kind > 0) return g.q_half.has(d);        // (0.4, 0.6)
+  if (prev_kind == 0 && kind == 0) return g.q_full.has(d);      // (0.9, 1.1)
+  return g.q_mixed.has(d);                                      // (0.4, 1.1)
 }
 
 // Field.run_vblank_state_machine (core.py:1826-1911) over pulses[b, e).
@@ -104,10 +148,12 @@ bool vblank_state_machine(const Geometry& g, const Timings& t, const std::vector
   double state_length = 0;     // 0: None (numPulses / 2 is never 0)
   const double half = g.p.num_pulses / 2.0;
   bool done = false;
+  int last_kind = -1;          // valid[-1] kept in registers
+  int64_t last_start = 0;
   for (size_t i = b; i < e; ++i) {
     const Pulse& q = pulses[i];
     const double n = (double)q.len;
-    const int state = valid.empty() ? -1 : valid.back().kind;
+    const int state = last_kind;
     int kind = -1;
     if (state == -1) {
       if (inrange(n, t.hsync[0], t.hsync[1])) kind = kHsync;
@@ -155,9 +201,10 @@ bool vblank_state_machine(const Geometry& g, const Timings& t, const std::vector
       }
     }
     if (kind >= 0) {
-      const bool good = valid.empty() ? false
-                                      : quality(g, valid.back().kind, valid.back().p.start, kind, q.start);
+      const bool good = last_kind < 0 ? false : quality(g, last_kind, last_start, kind, q.start);
       valid.push_back({kind, q, good});
+      last_kind = kind;
+      last_start = q.start;
     }
     if (done) return true;
   }
@@ -165,22 +212,32 @@ bool vblank_state_machine(const Geometry& g, const Timings& t, const std::vector
 }
 
 // Field.refinepulses (core.py:1914-1949).
-std::vector<VPulse> refinepulses(const Geometry& g, const Timings& t, const std::vector<Pulse>& raw) {
-  std::vector<VPulse> valid, vb;
-  valid.reserve(raw.size());
+void refinepulses(const Geometry& g, const Timings& t, const std::vector<Pulse>& raw, Scratch& sc) {
+  std::vector<VPulse>& valid = sc.valid;
+  std::vector<VPulse>& vb = sc.vb;
+  valid.clear();
+  valid.reserve(raw.size() + 32);
+  // the last entry's kind and start, kept in registers rather than read back
+  int last_kind = -1;
+  int64_t last_start = 0;
   size_t i = 0;
   while (i < raw.size()) {
     const Pulse& cur = raw[i];
     const double n = (double)cur.len;
     if (inrange(n, t.hsync[0], t.hsync[1])) {
-      const bool good =
-          valid.empty() ? false : quality(g, valid.back().kind, valid.back().p.start, kHsync, cur.start);
+      const bool good = last_kind < 0 ? false : quality(g, last_kind, last_start, kHsync, cur.start);
       valid.push_back({kHsync, cur, good});
+      last_kind = kHsync;
+      last_start = cur.start;
       ++i;
-    } else if (i > 2 && inrange(n, t.eq[0], t.eq[1]) && !valid.empty() && valid.back().kind == kHsync) {
+    } else if (i > 2 && inrange(n, t.eq[0], t.eq[1]) && last_kind == kHsync) {
       const bool done = vblank_state_machine(g, t, raw, i - 2, std::min(raw.size(), i + 24), vb);
       if (done) {
         for (size_t k = 2; k < vb.size(); ++k) valid.push_back(vb[k]);
+        if (vb.size() > 2) {
+          last_kind = vb.back().kind;
+          last_start = vb.back().p.start;
+        }
         i += vb.size() - 2;
       } else {
         ++i;
@@ -189,7 +246,6 @@ std::vector<VPulse> refinepulses(const Geometry& g, const Timings& t, const std:
       ++i;
     }
   }
-  return valid;
 }
 
 // Python's seq[i] for -len <= i < len.
@@ -237,20 +293,22 @@ struct VBlank {
   bool found = false;         // line0 is not None
   double line0 = 0;
   bool isfirst = false;
+  bool blank = false;         // getBlankRange found a range (firstblank valid)
   int64_t firstblank = -1;
   int conf = -1;              // -1: None
   bool zeroed = false;        // the fallback that sets sync_confidence = 0 (core.py:2082)
   bool error = false;         // the reference raises (np.max of an empty array)
 };
 
-// Field.processVBlank (core.py:1988-2088).
-VBlank process_vblank(const Geometry& g, const std::vector<VPulse>& v, int64_t start, int64_t limit) {
+// Field.processVBlank (core.py:1988-2088) without a limit; limited() applies
+// one afterwards (the limit only rejects a range that starts too late).
+VBlank process_vblank(const Geometry& g, const std::vector<VPulse>& v, int64_t start) {
   const ld_locate_params& p = g.p;
   VBlank r;
   int64_t fb, lb;
   if (!blank_range(v, start, fb, lb)) return r;
-  const int64_t lastvalid = limit < 0 ? (int64_t)v.size() : start + limit;
-  if (fb > lastvalid) return r;
+  r.blank = true;
+  r.firstblank = fb;
   const int64_t loc_presync = pyat(v, fb - 1).p.start;
   const int64_t m = (int64_t)v.size() - fb;        // entries of pt / pstart / plen
   const int np = p.num_pulses;
@@ -295,7 +353,6 @@ VBlank process_vblank(const Geometry& g, const std::vector<VPulse>& v, int64_t s
     r.found = true;
     r.line0 = std::trunc(line0);
     r.isfirst = isfirst;
-    r.firstblank = fb;
     r.conf = 100;
     return r;
   }
@@ -315,12 +372,18 @@ VBlank process_vblank(const Geometry& g, const std::vector<VPulse>& v, int64_t s
     r.found = true;
     r.line0 = (double)pyat(v, fb - 1).p.start;
     r.isfirst = isfirst;
-    r.firstblank = fb;
     r.conf = 50;
     return r;
   }
   r.conf = 0;
   return r;
+}
+
+// processVBlank(validpulses, start, limit): a range starting past
+// start + limit gives (None, None, None, None) before anything else.
+VBlank limited(const VBlank& vb, int64_t start, int64_t limit) {
+  if (limit >= 0 && vb.blank && vb.firstblank > start + limit) return VBlank();
+  return vb;
 }
 
 // Field.computeLineLen (core.py:2091-2119).
@@ -342,7 +405,7 @@ double compute_line_len(const Geometry& g, const std::vector<VPulse>& v) {
   int64_t cnt = 0;
   for (int64_t i = lr0 + 1; i < lr0 + lr1; ++i) {
     const int64_t ll = v[i].p.start - v[i - 1].p.start;
-    if (inrange((double)ll / g.p.linelen, 0.95, 1.05)) {
+    if (g.line_ok.has(ll)) {        // inrange(ll / linelen, 0.95, 1.05)
       sum += (double)ll;
       ++cnt;
     }
@@ -380,14 +443,16 @@ struct Line0 {
   bool error = false;
 };
 
-// Field.getLine0 (core.py:2157-2239).
-Line0 get_line0(const Geometry& g, const std::vector<VPulse>& v, double meanlinelen, const ld_prev_field* prev,
-                int64_t startloc) {
+// Field.getLine0 (core.py:2157-2239) from processVBlank's unlimited results
+// for the local vertical blanking and the next one (nx, computed from
+// loc0.firstblank + 40 when loc0 was found).
+Line0 get_line0(const Geometry& g, const VBlank& loc0, const VBlank& nx0, double meanlinelen,
+                const ld_prev_field* prev, int64_t startloc) {
   const ld_locate_params& p = g.p;
   Line0 r;
   r.conf = 100;
   const int64_t limit = (prev && prev->valid && prev->skip_score >= 50) ? 100 : -1;
-  const VBlank loc = process_vblank(g, v, 0, limit);
+  const VBlank loc = limited(loc0, 0, limit);
   if (loc.error) {
     r.error = true;
     return r;
@@ -397,7 +462,7 @@ Line0 get_line0(const Geometry& g, const std::vector<VPulse>& v, double meanline
   double l0_next = 0;
   int conf_next = -1;
   if (loc.found) {
-    const VBlank nx = process_vblank(g, v, loc.firstblank + 40, -1);
+    const VBlank& nx = nx0;
     if (nx.error) {
       r.error = true;
       return r;
@@ -418,7 +483,8 @@ Line0 get_line0(const Geometry& g, const std::vector<VPulse>& v, double meanline
   double l0_prev = 0;
   int conf_prev = 0;
   if (prev && prev->valid) {
-    l0_prev = prev->ll_linecount - (double)startloc;
+    // prevfield.linelocs[linecount] - frameoffset (core.py:2199-2203)
+    l0_prev = prev->ll_linecount - (double)(startloc - prev->startloc);
     first_prev = !prev->is_first_field;
     conf_prev = prev->sync_confidence;
     have_prev = true;
@@ -455,23 +521,37 @@ Line0 get_line0(const Geometry& g, const std::vector<VPulse>& v, double meanline
   return r;
 }
 
-// Field.compute_linelocs (core.py:2325-2481) after getpulses.  `last_start`
-// is rawpulses[-1].start.  Returns LD_OK / an LD_E* error; out->status says
-// whether the field is valid.
-int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_field* prev, int64_t startloc,
-           ld_field_loc* out, double* ll_out, uint8_t* bad_out) {
+// getLine0's inputs kept in ld_field_loc, so that a walk can be checked
+// against measured previous-field values without analysing the pulses again.
+void store_vblank(const VBlank& vb, ld_field_loc* out, int i) {
+  out->vb_line0[i] = vb.line0;
+  out->vb_firstblank[i] = vb.firstblank;
+  out->vb_conf[i] = vb.conf;
+  out->vb_flags[i] = (vb.found ? LD_VB_FOUND : 0) | (vb.isfirst ? LD_VB_FIRST : 0) |
+                     (vb.zeroed ? LD_VB_ZEROED : 0) | (vb.blank ? LD_VB_BLANK : 0) | (vb.error ? LD_VB_ERROR : 0);
+}
+
+VBlank load_vblank(const ld_field_loc& f, int i) {
+  VBlank vb;
+  vb.line0 = f.vb_line0[i];
+  vb.firstblank = f.vb_firstblank[i];
+  vb.conf = f.vb_conf[i];
+  vb.found = f.vb_flags[i] & LD_VB_FOUND;
+  vb.isfirst = f.vb_flags[i] & LD_VB_FIRST;
+  vb.zeroed = f.vb_flags[i] & LD_VB_ZEROED;
+  vb.blank = f.vb_flags[i] & LD_VB_BLANK;
+  vb.error = f.vb_flags[i] & LD_VB_ERROR;
+  return vb;
+}
+
+// Field.compute_linelocs (core.py:2325-2481) after getpulses.  Returns
+// LD_OK / an LD_E* error; out->status says whether the field is valid.
+// compute_linelocs from getLine0's result on: the line table, its gap fill
+// and the next field's offset (core.py:2338-2481).  valid: refinepulses'
+// list; last_start: rawpulses[-1].start.
+int assign(const Geometry& g, const std::vector<VPulse>& valid, int64_t last_start, double meanlinelen,
+           const Line0& l0, ld_field_loc* out, double* ll_out, uint8_t* bad_out, Scratch& sc) {
   const ld_locate_params& p = g.p;
-  const Timings t = get_timings(g, pulses);
-  const std::vector<VPulse> valid = refinepulses(g, t, pulses);
-  const double meanlinelen = compute_line_len(g, valid);
-  const Line0 l0 = get_line0(g, valid, meanlinelen, prev, startloc);
-  if (l0.error) {
-    set_error("ld_locate: processVBlank would raise (np.max of an empty array) at buffer %lld",
-              (long long)startloc);
-    return LD_EINVAL;
-  }
-  out->npulses = (int32_t)pulses.size();
-  out->meanlinelen = meanlinelen;
   out->is_first_field = l0.isfirst ? 1 : 0;
   out->linecount = l0.isfirst ? 263 : 262;
   out->sync_confidence = l0.conf;
@@ -488,16 +568,19 @@ int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_fi
     return LD_OK;
   }
   const double line0 = l0.line0;
-  const double lastline = ((double)pulses.back().start - line0) / meanlinelen;
+  const double lastline = ((double)last_start - line0) / meanlinelen;
   if (lastline < proclines) {
     out->nextfieldoffset = line0 - meanlinelen * 20;
     return LD_OK;
   }
   // line number -> (start, distance) for lines [0, proclines); others are
   // recorded by the reference too but never read
-  std::vector<int64_t> loc(proclines, -1);
-  std::vector<double> dist(proclines, 0.0);
-  std::vector<char> have(proclines, 0);
+  std::vector<int64_t>& loc = sc.loc;
+  std::vector<double>& dist = sc.dist;
+  std::vector<uint8_t>& have = sc.have;
+  loc.assign(proclines, -1);
+  dist.assign(proclines, 0.0);
+  have.assign(proclines, 0);
   for (const VPulse& vp : valid) {
     double lineloc = ((double)vp.p.start - line0) / meanlinelen;
     int64_t rl = np_round(lineloc);
@@ -524,7 +607,8 @@ int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_fi
     }
   }
   const int oc = g.outlinecount;
-  std::vector<double> fill(proclines);
+  std::vector<double>& fill = sc.fill;
+  fill.resize(proclines);
   for (int l = 0; l < proclines; ++l) fill[l] = (double)loc[l];
   auto first_valid_from = [&](int l, int stop) {
     for (int i = l; i < stop; ++i)
@@ -543,7 +627,8 @@ int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_fi
       return LD_OK;
     }
   }
-  std::vector<uint8_t> err(proclines, 0);
+  std::vector<uint8_t>& err = sc.err;
+  err.assign(proclines, 0);
   for (int l = 1; l < proclines; ++l) {
     if (fill[l] >= 0) continue;
     err[l] = 1;
@@ -577,6 +662,39 @@ int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_fi
   return LD_OK;
 }
 
+// Field.compute_linelocs (core.py:2325-2481) after getpulses.  Returns
+// LD_OK / an LD_E* error; out->status says whether the field is valid.
+// When keep is non-NULL (room for keep_cap entries) refinepulses' list is
+// copied there for ld_locate_recheck.
+int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_field* prev, int64_t startloc,
+           ld_field_loc* out, double* ll_out, uint8_t* bad_out, Scratch& sc, ld_vpulse* keep, int keep_cap) {
+  const Timings t = get_timings(g, pulses, sc);
+  refinepulses(g, t, pulses, sc);
+  const std::vector<VPulse>& valid = sc.valid;
+  const double meanlinelen = compute_line_len(g, valid);
+  const VBlank loc0 = process_vblank(g, valid, 0);
+  const VBlank nx0 = loc0.found ? process_vblank(g, valid, loc0.firstblank + 40) : VBlank();
+  store_vblank(loc0, out, 0);
+  store_vblank(nx0, out, 1);
+  out->analyzed = 1;
+  out->last_start = pulses.back().start;
+  out->nvalid = -1;
+  if (keep && (int64_t)valid.size() <= keep_cap) {
+    for (size_t i = 0; i < valid.size(); ++i)
+      keep[i] = {valid[i].p.start, (int32_t)valid[i].p.len, (int16_t)valid[i].kind, (int16_t)valid[i].good};
+    out->nvalid = (int32_t)valid.size();
+  }
+  const Line0 l0 = get_line0(g, loc0, nx0, meanlinelen, prev, startloc);
+  if (l0.error) {
+    set_error("ld_locate: processVBlank would raise (np.max of an empty array) at buffer %lld",
+              (long long)startloc);
+    return LD_EINVAL;
+  }
+  out->npulses = (int32_t)pulses.size();
+  out->meanlinelen = meanlinelen;
+  return assign(g, valid, out->last_start, meanlinelen, l0, out, ll_out, bad_out, sc);
+}
+
 int check_params(const ld_locate_params* p, const char* who) {
   if (!p || !(p->freq > 0) || !(p->linelen > 0) || p->num_pulses <= 0 || p->frame_lines <= 0 ||
       !(p->hz_ire != 0) || p->block_stride <= 0 || p->blocklen <= 0 || p->readlen < 0) {
@@ -596,7 +714,91 @@ bool sync_level_ok(const ld_locate_params& p, std::vector<float>& means) {
   return std::fabs(ire - (float)p.vsync_ire) < 5.0f;
 }
 
+// The state field f leaves for its successor (what getLine0 reads of it):
+// measured facts when known, else the provisional estimate.
+void next_prev(const ld_locate_params& p, const ld_field_loc& f, const double* ll, const ld_field_facts* fact,
+               ld_prev_field& st) {
+  const int lc = p.pal ? (f.is_first_field ? 312 : 313) : f.linecount;
+  st.valid = 1;
+  st.is_first_field = f.is_first_field;
+  st.linecount = lc;
+  st.sync_confidence = f.sync_confidence;
+  st.startloc = f.startloc;
+  if (fact && fact->known) {
+    st.ll_linecount = fact->ll_linecount;
+    st.skip_score = skip_score(p, fact->medians);
+  } else {
+    st.ll_linecount = ll[lc] + p.prov_delta;
+    st.skip_score = 100;
+  }
+}
+
 }  // namespace
+
+extern "C" int ld_locate_recheck(const ld_locate_params* p, ld_field_loc* locs, double* linelocs1,
+                                 uint8_t* linebad, int max_lines, const ld_vpulse* vpulses, int max_vpulses, int n,
+                                 const ld_prev_field* prev, const ld_field_facts* facts, int* first_changed) {
+  int rc = check_params(p, "ld_locate_recheck");
+  if (rc != LD_OK) return rc;
+  const Geometry g(*p);
+  if (!locs || n < 0 || !linelocs1 || !linebad || max_lines < g.proclines || !first_changed ||
+      (!vpulses && n > 0) || max_vpulses < 0) {
+    set_error("ld_locate_recheck: bad arguments");
+    return LD_EINVAL;
+  }
+  ld_prev_field st{};
+  if (prev) st = *prev;
+  const ld_prev_field* pv = (prev && prev->valid) ? &st : nullptr;
+  *first_changed = -1;
+  Scratch sc;
+  std::vector<VPulse> valid;
+  std::vector<double> ll(g.proclines);
+  std::vector<uint8_t> bad(g.proclines);
+  for (int k = 0; k < n; ++k) {
+    ld_field_loc& f = locs[k];
+    double* fll = linelocs1 + (int64_t)k * max_lines;
+    uint8_t* fbad = linebad + (int64_t)k * max_lines;
+    if (f.analyzed) {
+      const Line0 l0 = get_line0(g, load_vblank(f, 0), load_vblank(f, 1), f.meanlinelen, pv, f.startloc);
+      const bool found = !std::isnan(f.line0loc);
+      const bool same_decision = !l0.error && l0.found == found && (!found || l0.line0 == f.line0loc) &&
+                                 (int)l0.isfirst == f.is_first_field && l0.conf == f.sync_confidence &&
+                                 (l0.has_next ? (int64_t)l0.vblank_next : -1) == f.vblank_next;
+      if (!same_decision) {
+        if (l0.error || f.nvalid < 0) {
+          *first_changed = k;
+          return LD_OK;
+        }
+        // the line table from the new line 0: same outputs -> keep the walk
+        const ld_vpulse* kv = vpulses + (int64_t)k * max_vpulses;
+        valid.resize(f.nvalid);
+        for (int i = 0; i < f.nvalid; ++i) valid[i] = {kv[i].kind, {kv[i].start, (int64_t)kv[i].len}, kv[i].good != 0};
+        ld_field_loc g2 = f;
+        rc = assign(g, valid, f.last_start, f.meanlinelen, l0, &g2, ll.data(), bad.data(), sc);
+        if (rc != LD_OK) return rc;
+        bool same = g2.status == f.status && g2.is_first_field == f.is_first_field && g2.linecount == f.linecount &&
+                    g2.sync_confidence == f.sync_confidence && g2.nextfieldoffset == f.nextfieldoffset;
+        if (same && f.status == LD_LOC_OK)
+          for (int l = 0; l < g.proclines && same; ++l) same = ll[l] == fll[l] && bad[l] == fbad[l];
+        if (!same) {
+          *first_changed = k;
+          return LD_OK;
+        }
+        f.line0loc = g2.line0loc;
+        f.vblank_next = g2.vblank_next;
+        f.skipdetected = g2.skipdetected;
+      }
+    }
+    if (f.status == LD_LOC_OK) {
+      next_prev(*p, f, fll, facts ? &facts[k] : nullptr, st);
+      pv = &st;
+    } else {
+      st.valid = 0;
+      pv = nullptr;
+    }
+  }
+  return LD_OK;
+}
 
 extern "C" int ld_skip_score(const ld_locate_params* p, const double* medians8) {
   if (check_params(p, "ld_skip_score") != LD_OK || !medians8) return LD_EINVAL;
@@ -625,15 +827,17 @@ extern "C" int ld_locate_field(const ld_locate_params* p, const int64_t* starts,
     out->nextfieldoffset = fields_written ? p->linelen * 200 : (double)(int64_t)(p->freq * 1e6);
     return LD_OK;
   }
+  Scratch sc;
   std::vector<Pulse> pl(npulses);
   for (int i = 0; i < npulses; ++i) pl[i] = {starts[i], lens[i]};
-  return locate(g, pl, prev, startloc, out, linelocs1, linebad);
+  return locate(g, pl, prev, startloc, out, linelocs1, linebad, sc, nullptr, 0);
 }
 
 extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges, int64_t nedges, const float* means,
                                 int64_t total, double start, const ld_prev_field* prev, const ld_field_facts* facts,
                                 int max_fields, ld_field_loc* out, double* linelocs1, uint8_t* linebad,
-                                int max_lines, ld_prev_field* prev_out, int* nfields_out) {
+                                int max_lines, ld_vpulse* vpulses, int max_vpulses, ld_prev_field* prev_out,
+                                int* nfields_out) {
   int rc = check_params(p, "ld_locate_fields");
   if (rc != LD_OK) return rc;
   const Geometry g(*p);
@@ -643,11 +847,6 @@ extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges,
     return LD_EINVAL;
   }
   const int64_t npairs = nedges / 2;
-  for (int64_t k = 1; k < nedges; ++k)
-    if (edges[k] <= edges[k - 1]) {
-      set_error("ld_locate_fields: edges not increasing at %lld", (long long)k);
-      return LD_EINVAL;
-    }
   const double minlength = g.us(10);
   const int64_t S = p->block_stride;
   const int64_t length = (p->readlen / p->blocklen + 2) * p->blocklen;
@@ -656,8 +855,9 @@ extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges,
   const ld_prev_field* pv = (prev && prev->valid) ? &st : nullptr;
   int written = prev ? prev->fields_written : 0;
   int nf = 0;
-  std::vector<Pulse> pl;
-  std::vector<float> vm;
+  Scratch sc;
+  std::vector<Pulse>& pl = sc.pulses;
+  std::vector<float>& vm = sc.means;
   while (nf < max_fields) {
     // decodefield's read (core.py:4030-4041) and DemodCache.read (:1358-1409)
     const int64_t readloc = std::max<int64_t>((int64_t)(start - (double)p->blockcut), 0);
@@ -678,6 +878,10 @@ extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges,
     }
     for (int64_t k = lo; k < npairs && edges[2 * k + 1] < we; ++k) {
       const int64_t len = edges[2 * k + 1] - edges[2 * k];
+      if (len <= 0 || (k > 0 && edges[2 * k] <= edges[2 * k - 1])) {
+        set_error("ld_locate_fields: edges not increasing at %lld", (long long)(2 * k));
+        return LD_EINVAL;
+      }
       if (len > kMaxSyncLen) continue;
       pl.push_back({edges[2 * k] - ws, len});
       if ((double)len > minlength) vm.push_back(means[k]);
@@ -708,7 +912,8 @@ extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges,
         o.vblank_next = -1;
         o.nextfieldoffset = p->linelen * 200;
       } else {
-        rc = locate(g, pl, pv, ws, &o, ll, bad);
+        rc = locate(g, pl, pv, ws, &o, ll, bad, sc, vpulses ? vpulses + (int64_t)nf * max_vpulses : nullptr,
+                    max_vpulses);
         if (rc != LD_OK) return rc;
       }
     }
@@ -720,19 +925,7 @@ extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges,
     if (o.status == LD_LOC_NEED_LEVELS || o.status == LD_LOC_NEED_IRE0) break;
     if (o.status == LD_LOC_OK) {
       o.offset = o.nextfieldoffset - (double)(readloc - ws);
-      const int lc = p->pal ? (o.is_first_field ? 312 : 313) : o.linecount;
-      st.valid = 1;
-      st.is_first_field = o.is_first_field;
-      st.linecount = lc;
-      st.sync_confidence = o.sync_confidence;
-      st.startloc = ws;
-      if (facts && facts[nf - 1].known) {
-        st.ll_linecount = facts[nf - 1].ll_linecount;
-        st.skip_score = skip_score(*p, facts[nf - 1].medians);
-      } else {
-        st.ll_linecount = ll[lc] + (double)ws + p->prov_delta;
-        st.skip_score = 100;
-      }
+      next_prev(*p, o, ll, facts ? &facts[nf - 1] : nullptr, st);
       ++written;
       st.fields_written = written;
       pv = &st;

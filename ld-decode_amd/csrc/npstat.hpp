@@ -52,4 +52,18 @@ __device__ __noinline__ float np_pairwise_sum_any(F x, int off, int n) {
   }
 }
 
+// np.add.reduce of a float32 array: the reduction runs over buffers of 8192
+// elements (NumPy's ufunc buffer size), each summed pairwise, the buffer
+// sums added in order.
+template <class F>
+__device__ float np_sum_f32(F x, long long n) {
+  float r = -0.0f;
+  for (long long i = 0; i < n; i += 8192) {
+    const int m = (int)(n - i < 8192 ? n - i : 8192);
+    auto chunk = [&x, i](int j) { return x(i + j); };
+    r = r + np_pairwise_sum_any<6>(chunk, 0, m);
+  }
+  return r;
+}
+
 }  // namespace ldgpu

@@ -8,7 +8,7 @@
 //       the body of every long pulse, one wavefront per pulse, the samples
 //       staged in LDS
 //   k_range_means   the same mean over caller-listed ranges (black levels)
-//   k_field_facts   skip_check's line medians (bitonic sort in LDS, one
+//   k_field_facts   skip_check's line medians (radix select in LDS, one
 //       workgroup per field line) and the final line location a field's
 //       successor reads
 // Edges and positions are int64; sample comparisons are made in double, as
@@ -183,10 +183,10 @@ __global__ __launch_bounds__(64) void k_range_means(const float* __restrict__ x,
   np_slice(begin[k], end[k], n, a, b);
   const long long m = b - a;
   float r = __builtin_nanf("");
-  if (m > 0 && m <= (128LL << 12)) {
+  if (m > 0) {
     const float* p = x + a;
-    auto at = [p](int i) { return p[i]; };
-    r = np_pairwise_sum_any<12>(at, 0, (int)m) / (float)m;
+    auto at = [p](long long i) { return p[i]; };
+    r = np_sum_f32(at, m) / (float)m;
   }
   out[k] = r;
 }
@@ -200,10 +200,62 @@ struct FactArgs {
   int pal;
 };
 
+// Order-preserving unsigned key of a float (finite values).
+__device__ __forceinline__ unsigned fkey(float v) {
+  const unsigned u = __float_as_uint(v);
+  return (u & 0x80000000u) ? ~u : (u | 0x80000000u);
+}
+__device__ __forceinline__ float fkey_inv(unsigned k) {
+  return __uint_as_float((k & 0x80000000u) ? (k & 0x7fffffffu) : ~k);
+}
+
+// The k-th smallest (0-based) of the m keys in s[], by four 8-bit radix
+// passes over LDS histograms; every thread of the workgroup returns it.
+__device__ unsigned radix_select(const unsigned* s, int m, int k, unsigned* hist, unsigned* shared, int t) {
+  unsigned prefix = 0, mask = 0;
+  for (int shift = 24; shift >= 0; shift -= 8) {
+    hist[t] = 0;
+    __syncthreads();
+    for (int i = t; i < m; i += kFactThreads) {
+      const unsigned v = s[i];
+      if ((v & mask) == prefix) atomicAdd(&hist[(v >> shift) & 255u], 1u);
+    }
+    __syncthreads();
+    if (t < 64) {
+      // each lane owns four bins; find the bin holding rank k
+      const unsigned c0 = hist[4 * t], c1 = hist[4 * t + 1], c2 = hist[4 * t + 2], c3 = hist[4 * t + 3];
+      const unsigned tot = c0 + c1 + c2 + c3;
+      unsigned incl = tot;
+      for (int o = 1; o < 64; o <<= 1) {
+        const unsigned y = __shfl_up(incl, o);
+        if (t >= o) incl += y;
+      }
+      const unsigned excl = incl - tot;
+      if ((unsigned)k >= excl && (unsigned)k < incl) {
+        unsigned r = (unsigned)k - excl, bin = 4 * t;
+        if (r >= c0) { r -= c0; ++bin;
+          if (r >= c1) { r -= c1; ++bin;
+            if (r >= c2) { r -= c2; ++bin; } } }
+        shared[0] = bin;
+        shared[1] = r;
+      }
+    }
+    __syncthreads();
+    prefix |= shared[0] << shift;
+    mask |= 255u << shift;
+    k = (int)shared[1];
+    __syncthreads();
+  }
+  return prefix;
+}
+
 // One workgroup per (field, j): the median of demod over skip_check's
-// line slice j, as numba's np.median; j == 0 also stores ll_linecount.
+// line slice j, as numba's np.median (exact order statistics by radix
+// select); j == 0 also stores ll_linecount.
 __global__ __launch_bounds__(kFactThreads) void k_field_facts(FactArgs a, const ld_line_field* __restrict__ fields) {
-  __shared__ float s[kFactMax];
+  __shared__ unsigned s[kFactMax];
+  __shared__ unsigned hist[kFactThreads];
+  __shared__ unsigned shared[4];
   const int f = blockIdx.x >> 3, j = blockIdx.x & 7;
   const ld_line_field fd = fields[f];
   const int t = threadIdx.x;
@@ -212,7 +264,7 @@ __global__ __launch_bounds__(kFactThreads) void k_field_facts(FactArgs a, const 
   const int line = a.outlinecount + j + lineoffset;
   if (j == 0 && t == 0) {
     const int lc = a.pal ? (first ? 312 : 313) : (first ? 263 : 262);
-    a.facts[f].ll_linecount = lc < fd.nlines ? a.ll[fd.lineloc_offset + lc] + (double)fd.demod_offset : __builtin_nan("");
+    a.facts[f].ll_linecount = lc < fd.nlines ? a.ll[fd.lineloc_offset + lc] : __builtin_nan("");
     a.facts[f].known = 1;
     a.facts[f].reserved = 0;
   }
@@ -226,29 +278,39 @@ __global__ __launch_bounds__(kFactThreads) void k_field_facts(FactArgs a, const 
   }
   const long long m = hi - lo;
   if (m > 0 && m <= kFactMax) {
-    int p2 = 1;
-    while (p2 < m) p2 <<= 1;
     const float* src = a.demod + fd.demod_offset + lo;
-    for (int i = t; i < p2; i += kFactThreads) s[i] = i < m ? src[i] : __builtin_inff();
+    for (int i = t; i < m; i += kFactThreads) s[i] = fkey(src[i]);
     __syncthreads();
-    for (int k = 2; k <= p2; k <<= 1) {
-      for (int jj = k >> 1; jj > 0; jj >>= 1) {
-        for (int i = t; i < p2; i += kFactThreads) {
-          const int l = i ^ jj;
-          if (l > i) {
-            const float u = s[i], v = s[l];
-            const bool up = (i & k) == 0;
-            if (up ? (u > v) : (u < v)) {
-              s[i] = v;
-              s[l] = u;
-            }
-          }
-        }
-        __syncthreads();
-      }
-    }
     const int h = (int)(m / 2);
-    med = (m & 1) ? (double)s[h] : (double)(s[h - 1] + s[h]) / 2.0;
+    if (m & 1) {
+      med = (double)fkey_inv(radix_select(s, (int)m, h, hist, shared, t));
+    } else {
+      // s[h-1] by selection; s[h] is the same key when it repeats past rank
+      // h-1, else the smallest larger key
+      const unsigned k1 = radix_select(s, (int)m, h - 1, hist, shared, t);
+      unsigned le = 0, nxt = 0xffffffffu;
+      for (int i = t; i < m; i += kFactThreads) {
+        const unsigned v = s[i];
+        le += v <= k1;
+        if (v > k1 && v < nxt) nxt = v;
+      }
+      for (int o = 32; o > 0; o >>= 1) {
+        le += __shfl_xor(le, o);
+        nxt = min(nxt, (unsigned)__shfl_xor((int)nxt, o));
+      }
+      if ((t & 63) == 0) {
+        hist[t >> 6] = le;
+        hist[8 + (t >> 6)] = nxt;
+      }
+      __syncthreads();
+      unsigned tle = 0, tn = 0xffffffffu;
+      for (int w = 0; w < kFactThreads / 64; ++w) {
+        tle += hist[w];
+        tn = min(tn, hist[8 + w]);
+      }
+      const unsigned k2 = tle > (unsigned)h ? k1 : tn;
+      med = (double)(fkey_inv(k1) + fkey_inv(k2)) / 2.0;
+    }
   }
   if (t == 0) a.facts[f].medians[j] = med;
 }

@@ -151,6 +151,17 @@ def compute_deriv_error(linelocs, baserr):
     return np.asarray(baserr, bool) | d1 | d2
 
 
+def compute_deriv_error_rows(linelocs, baserr):
+    """Field.compute_deriv_error (core.py:2553-2564) for each row of a
+    (fields, lines) array."""
+    ll = np.asarray(linelocs, dtype=np.float64)
+    d = np.abs(np.diff(np.diff(ll, axis=1), axis=1)) > 4
+    out = np.array(baserr, dtype=bool, copy=True)
+    out[:, 1:-1] |= d
+    out[:, 2:] |= d
+    return out
+
+
 def line_params(rf):
     """ld_line_params of an RFDecode: rf.freq, rf.linelen, fsc/pilot MHz, hz_ire."""
     sp = rf.SysParams
@@ -272,6 +283,108 @@ class Field:
             self.phase_adjust_median = float(phase[0].item())
             self.fieldPhaseID = int(fpid[0].item())
         return ll
+
+    def compute_linelocs(self, prevfield=None, fields_written=1, startloc=0):
+        """Field.compute_linelocs (core.py:2325-2481) over this field's
+        buffer: the GPU finds the sync pulses of demod_05 and their levels
+        (getpulses, with its re-threshold and ire0 retries), the native state
+        machine places the lines.  prevfield: a located Field before this one
+        (its final `linelocs`, `linecount`, `isFirstField`, `sync_confidence`,
+        `startloc` and buffers are read, as getLine0 reads them) or None.
+        startloc: this buffer's stitched origin (for frameoffset).  Returns
+        (linelocs1, linebad, nextfieldoffset) like the reference (None, None,
+        offset when no field was found) and sets isFirstField, linecount,
+        sync_confidence, vblank_next, valid."""
+        from . import locate as L
+        rf = self.rf
+        torch = rf.torch
+        d05 = self.data["video"]["demod_05"]
+        n = d05.numel()
+        self.startloc = int(startloc)
+        pf = L.PulseFinder(rf, n)
+        pf.launch(d05, n)
+        edges, means = pf.result()
+        loc0 = _lib.FieldLoc()
+        loc0.startloc, loc0.buflen = 0, n
+
+        def buffer_pulses(edges, means):
+            # findpulses on the buffer: runs that end inside it, not starting
+            # at sample 0, at most 5000 samples long
+            s_all, e_all = edges[0:len(edges) - 1:2], edges[1::2]
+            s_all = s_all[:len(e_all)]
+            keep = (s_all != 0) & ((e_all - s_all) <= 5000)
+            return s_all[keep], (e_all - s_all)[keep], means[:len(e_all)][keep]
+
+        starts, lens, lmeans = buffer_pulses(edges, means)
+        prev = None
+        if prevfield is not None and getattr(prevfield, "valid", False):
+            prev = prevfield._as_prev(fields_written)
+        P = L.locate_params(rf)
+        if len(starts) == 0 and not fields_written:
+            rf.DecoderParams["ire0"] = L.recalibrated_ire0(rf, d05, loc0)
+            P = L.locate_params(rf)
+            pf.launch(d05, n)
+            edges, means = pf.result()
+            starts, lens, lmeans = buffer_pulses(edges, means)
+        if len(starts):
+            long = lens > 10.0 * ((rf.freq / rf.linelen) * rf.linelen)    # usectoinpx(10)
+            if not long.any():
+                starts = None       # getpulses returns None: no vertical sync seen
+            else:
+                m = lmeans[long].astype(np.float32)
+                level = np.float32(np.median(m))
+                dp = rf.DecoderParams
+                ire = np.float32((level - np.float32(dp["ire0"])) / np.float32(dp["hz_ire"]))
+                if not abs(ire - np.float32(dp["vsync_ire"])) < 5:
+                    starts, lens = L.relevel_pulses(rf, d05, edges, means, loc0)
+        if starts is None:
+            off = rf.linelen * 200 if fields_written else int(rf.freq * 1e6)
+            self.valid = False
+            self.nextfieldoffset = off
+            return None, None, off
+        loc, ll1, bad = L.locate_one(rf, starts, lens, prev, self.startloc, fields_written, P)
+        self.isFirstField = bool(loc.is_first_field)
+        self.linecount = int(loc.linecount)
+        self.sync_confidence = int(loc.sync_confidence)
+        self.vblank_next = None if loc.vblank_next < 0 else int(loc.vblank_next)
+        self.line0loc = loc.line0loc
+        self.nextfieldoffset = loc.nextfieldoffset
+        self.valid = loc.status == _lib.LD_LOC_OK
+        if not self.valid:
+            return None, None, loc.nextfieldoffset
+        self.linelocs1 = ll1
+        self.linebad = bad.astype(bool)
+        return ll1, self.linebad, loc.nextfieldoffset
+
+    def _as_prev(self, fields_written):
+        """The state getLine0 reads of this (located, aligned) field."""
+        from . import locate as L
+        rf = self.rf
+        torch = rf.torch
+        pal = rf.system == "PAL"
+        lc = (312 if self.isFirstField else 313) if pal else self.linecount
+        ll = self.linelocs
+        if not isinstance(ll, torch.Tensor):
+            ll = torch.as_tensor(np.asarray(ll, dtype=np.float64))
+        ll = ll.to(rf.device, torch.float64).contiguous()
+        demod = self.data["video"]["demod"]
+        fd = dict(demod_offset=0, demod_len=demod.numel(), lineloc_offset=0, nlines=ll.numel(),
+                  linecount=self.linecount, first_field=int(bool(self.isFirstField)))
+        facts = torch.empty(ctypes.sizeof(_lib.FieldFacts), dtype=torch.uint8, device=rf.device)
+        ws = torch.empty(rf.lib.ld_field_facts_workspace_bytes(rf.handle, 1), dtype=torch.uint8, device=rf.device)
+        length = rf.SysParams["line_period"] * ((rf.freq / rf.linelen) * rf.linelen)
+        _lib.check(rf.lib.ld_field_facts_measure(
+            rf.handle, demod.data_ptr(), demod.numel(), _line_fields([fd]), 1, ll.data_ptr(), ll.numel(),
+            1 if pal else 0, rf.SysParams["frame_lines"] // 2 + 1, float(length), facts.data_ptr(),
+            ws.data_ptr(), ws.numel(), ctypes.c_void_p(rf._stream())))
+        f = _lib.FieldFacts.from_buffer_copy(facts.cpu().numpy().tobytes())
+        pv = _lib.PrevField()
+        pv.valid, pv.is_first_field, pv.linecount = 1, int(bool(self.isFirstField)), lc
+        pv.sync_confidence, pv.startloc = int(self.sync_confidence), int(getattr(self, "startloc", 0))
+        pv.ll_linecount = f.ll_linecount
+        pv.skip_score = L.skip_scores(rf, [f])[0]
+        pv.fields_written = fields_written
+        return pv
 
     def downscale(self, lineinfo=None, linesout=None, outwidth=None, channel="demod", audio=0,
                   final=False, lastfieldwritten=None):

@@ -98,7 +98,8 @@ class PulseFinder:
         _lib.check(rf.lib.ld_find_pulses(rf.handle, demod05.data_ptr(), n, float(hi), self.edges.data_ptr(),
                                          self.cap, self.count.data_ptr(), self.ws.data_ptr(), self.ws.numel(), sp))
         _lib.check(rf.lib.ld_pulse_means(rf.handle, demod05.data_ptr(), n, self.edges.data_ptr(),
-                                         self.count.data_ptr(), self.cap // 2, float(rf.freq * 10.0),
+                                         self.count.data_ptr(), self.cap // 2,
+                                         10.0 * ((rf.freq / rf.linelen) * rf.linelen),    # usectoinpx(10)
                                          float(rf.freq), self.means.data_ptr(), sp))
         with torch.cuda.stream(st):
             self.h_count.copy_(self.count, non_blocking=True)
@@ -130,29 +131,76 @@ def _facts_array(facts, n):
     return arr
 
 
-def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None):
-    """ld_locate_fields: returns (locs, linelocs1, linebad, prev_out) with
-    locs a list of _lib.FieldLoc, linelocs1 / linebad arrays of shape
-    (fields, proclines)."""
+MAX_VPULSES = 2048     # classified pulses kept per field for the re-check
+VPULSE_DTYPE = np.dtype([("start", np.int64), ("len", np.int32), ("kind", np.int16), ("good", np.int16)])
+
+
+class WalkBuffers:
+    """Host arrays a walk writes into: field records, linelocs1 / linebad
+    rows and the classified pulses kept for the re-check."""
+
+    def __init__(self, rf, cap):
+        L = proclines(rf)
+        self.cap = int(cap)
+        self.out = (_lib.FieldLoc * self.cap)()
+        self.ll = np.empty((self.cap, L), dtype=np.float64)
+        self.bad = np.empty((self.cap, L), dtype=np.uint8)
+        self.vp = np.empty((self.cap, MAX_VPULSES), dtype=VPULSE_DTYPE)
+        self.vp[:] = 0          # touch the pages once
+
+
+def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None, bufs=None, at=0):
+    """ld_locate_fields: returns (locs, linelocs1, linebad, prev_out,
+    vpulses) with locs a list of _lib.FieldLoc, linelocs1 / linebad arrays
+    of shape (fields, proclines) and vpulses the fields' classified pulses
+    (fields, MAX_VPULSES).  With bufs (WalkBuffers), the results are written
+    from row `at` on and the arrays returned are views of those rows."""
     P = params or locate_params(rf)
     L = proclines(rf)
-    out = (_lib.FieldLoc * max(max_fields, 1))()
-    ll = np.empty((max(max_fields, 1), L), dtype=np.float64)
-    bad = np.empty((max(max_fields, 1), L), dtype=np.uint8)
+    m = max(max_fields, 1)
+    if bufs is None:
+        bufs = WalkBuffers(rf, m)
+        at = 0
+    if at + max_fields > bufs.cap:
+        raise ValueError("walk buffers too small")
+    fsz = ctypes.sizeof(_lib.FieldLoc)
+    out = ctypes.cast(ctypes.addressof(bufs.out) + at * fsz, ctypes.POINTER(_lib.FieldLoc))
+    ll, bad, vp = bufs.ll[at:], bufs.bad[at:], bufs.vp[at:]
     po = _lib.PrevField()
     nf = ctypes.c_int(0)
     edges = np.ascontiguousarray(edges, dtype=np.int64)
     means = np.ascontiguousarray(means, dtype=np.float32)
-    fa = _facts_array(facts, max(max_fields, 1))
+    fa = _facts_array(facts, m)
     _lib.check(rf.lib.ld_locate_fields(ctypes.byref(P), _ptr(edges), len(edges), _ptr(means), int(total),
                                        float(start), ctypes.byref(prev) if prev is not None else None, fa,
-                                       max_fields, out, _ptr(ll), _ptr(bad), L, ctypes.byref(po), ctypes.byref(nf)))
+                                       max_fields, out, _ptr(ll), _ptr(bad), L, _ptr(vp), MAX_VPULSES,
+                                       ctypes.byref(po), ctypes.byref(nf)))
     n = nf.value
     for k in range(n):
         if out[k].status != _lib.LD_LOC_OK:
             ll[k] = np.nan
             bad[k] = 0
-    return list(out[:n]), ll[:n], bad[:n], po
+    return [_lib.FieldLoc.from_buffer_copy(out[k]) for k in range(n)], ll[:n], bad[:n], po, vp[:n]
+
+
+def recheck(rf, locs, linelocs1, linebad, vpulses, facts, prev=None, params=None):
+    """ld_locate_recheck: (first changed field or -1, the records updated
+    to the measured line-0 decisions).  linelocs1 / linebad are checked in
+    place."""
+    P = params or locate_params(rf)
+    L = proclines(rf)
+    n = len(locs)
+    if n == 0:
+        return -1, []
+    arr = (_lib.FieldLoc * n)(*locs)
+    assert linelocs1.shape == (n, L) and linelocs1.flags.c_contiguous and linebad.flags.c_contiguous
+    vp = np.ascontiguousarray(vpulses)
+    fa = _facts_array(facts, n)
+    first = ctypes.c_int(-1)
+    _lib.check(rf.lib.ld_locate_recheck(ctypes.byref(P), arr, _ptr(linelocs1), _ptr(linebad), L, _ptr(vp),
+                                        vp.shape[1], n, ctypes.byref(prev) if prev is not None else None, fa,
+                                        ctypes.byref(first)))
+    return first.value, list(arr)
 
 
 def window_pulses(rf, edges, startloc, buflen):
@@ -196,7 +244,7 @@ def relevel_pulses(rf, demod05, edges, means, loc, stream=None):
     s = s[:len(e)]
     k = np.flatnonzero((s > ws) & (e < ws + n) & ((e - s) <= 5000))
     st, ln = s[k] - ws, (e - s)[k]
-    minlength = rf.freq * 10.0
+    minlength = 10.0 * ((rf.freq / rf.linelen) * rf.linelen)      # usectoinpx(10)
     vl = np.flatnonzero(ln > minlength)
     synclevel = _f32_median(means[k[vl]])
     r = [i for i in list(range(vl[0] - 5, vl[0])) + list(range(vl[-1] + 1, vl[-1] + 6)) if 0 <= i < len(st)]
@@ -248,7 +296,8 @@ def skip_scores(rf, facts, params=None):
 
 class FieldWalker:
     """The decodefield walk over one stitched demod_05 buffer, resolving
-    the getpulses retries on the GPU.  walk() returns every field visited
+    the getpulses retries on the GPU.  walk() visits fields until max_fields
+    of them are valid (or the buffer ends) and returns every field visited
     (valid or not) with its linelocs1 / linebad."""
 
     def __init__(self, rf, total, max_fields=64):
@@ -257,26 +306,36 @@ class FieldWalker:
         self.max_fields = max_fields
         self.finder = PulseFinder(rf, total)
         self.prov_delta = 0.0
+        self.bufs = WalkBuffers(rf, max_fields + 16)
 
     def pulses(self, demod05, stream=None):
         self.finder.launch(demod05, self.total, stream)
 
     def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None):
+        """Returns (locs, linelocs1, linebad, prev_out, next start,
+        vpulses); the arrays are views of this walker's buffers, valid until
+        its next walk."""
         rf = self.rf
         edges, means = self.finder.result()
         P = locate_params(rf, self.prov_delta)
-        locs, lls, bads = [], [], []
-        while len(locs) < self.max_fields:
-            fk = None if facts is None else facts[len(locs):]
-            requested = self.max_fields - len(locs)
-            got, ll, bad, po = walk(rf, edges, means, self.total, start, prev, fk, requested, P)
+        B = self.bufs
+        locs = []
+        nvalid = 0
+        while nvalid < self.max_fields:
+            n = len(locs)
+            if n + (self.max_fields - nvalid) > B.cap:
+                old, B = B, WalkBuffers(rf, 2 * B.cap + self.max_fields)
+                B.ll[:n], B.bad[:n], B.vp[:n] = old.ll[:n], old.bad[:n], old.vp[:n]
+                self.bufs = B
+            fk = None if facts is None else facts[n:]
+            requested = self.max_fields - nvalid
+            got, _, _, po, _ = walk(rf, edges, means, self.total, start, prev, fk, requested, P, B, n)
             if not got:
                 break
             tail = got[-1]
             if tail.status in (_lib.LD_LOC_NEED_LEVELS, _lib.LD_LOC_NEED_IRE0):
                 locs += got[:-1]
-                lls.append(ll[:-1])
-                bads.append(bad[:-1])
+                nvalid += sum(c.status == _lib.LD_LOC_OK for c in got[:-1])
                 written = po.fields_written
                 pv = po if len(got) > 1 else prev
                 if tail.status == _lib.LD_LOC_NEED_IRE0:
@@ -293,24 +352,23 @@ class FieldWalker:
                 loc.start, loc.readloc, loc.startloc, loc.buflen = tail.start, tail.readloc, tail.startloc, tail.buflen
                 loc.offset = (loc.nextfieldoffset - (loc.readloc - loc.startloc)
                               if loc.status == _lib.LD_LOC_OK else loc.nextfieldoffset)
+                k = len(locs)
+                B.ll[k], B.bad[k] = l1, b1
+                B.vp[k] = 0
                 locs.append(loc)
-                lls.append(l1[None])
-                bads.append(b1[None])
+                nvalid += loc.status == _lib.LD_LOC_OK
                 prev = _prev_from(loc, l1, rf, P, written, None if facts is None else
-                                  (facts[len(locs) - 1] if len(locs) - 1 < len(facts) else None))
+                                  (facts[k] if k < len(facts) else None))
                 start = loc.start + loc.offset
                 continue
             locs += got
-            lls.append(ll)
-            bads.append(bad)
+            nvalid += sum(c.status == _lib.LD_LOC_OK for c in got)
             prev = po
             start = got[-1].start + got[-1].offset
             if len(got) < requested:
                 break       # the next field's buffer passes the end
-        L = proclines(rf)
-        ll = np.concatenate(lls) if lls else np.empty((0, L))
-        bad = np.concatenate(bads) if bads else np.empty((0, L), np.uint8)
-        return locs, ll, bad, prev, start
+        n = len(locs)
+        return locs, B.ll[:n], B.bad[:n], prev, start, B.vp[:n]
 
 
 def _prev_from(loc, ll1, rf, P, written, fact):
@@ -326,6 +384,6 @@ def _prev_from(loc, ll1, rf, P, written, fact):
         pv.ll_linecount = fact.ll_linecount
         pv.skip_score = skip_scores(rf, [fact], P)[0]
     else:
-        pv.ll_linecount = ll1[lc] + loc.startloc + P.prov_delta
+        pv.ll_linecount = ll1[lc] + P.prov_delta
         pv.skip_score = 100
     return pv

@@ -1,0 +1,212 @@
+"""GPU parity of line location: the pulse edges, sync levels, range means
+and skip-check medians against numpy / the oracle bit for bit, and the whole
+located decode (GPU pulses + host walk + verification) against the oracle's
+sequential walk over the same demodulated buffers.
+"""
+
+import ctypes
+
+import numpy as np
+import pytest
+
+from lddecode_amd import _lib, capture, locate, synth
+from oracle import dsp as O
+from oracle import linelocs as LL
+
+pytestmark = pytest.mark.gpu
+
+
+def _edges_np(x, high):
+    below = x.astype(np.float64) <= high
+    prev = np.concatenate([[False], below[:-1]])
+    return np.flatnonzero(below != prev).astype(np.int64)
+
+
+def _means_np(x, edges, min_len, trim):
+    s, e = edges[0:len(edges) - 1:2], edges[1::2]
+    s = s[:len(e)]
+    out = np.full(len(s), np.nan, np.float32)
+    for k in range(len(s)):
+        n = e[k] - s[k]
+        if n > min_len and n <= 5000:
+            a, b = int(s[k] + trim), int(s[k] + n - trim)
+            if b > a:
+                out[k] = LL.np_mean_f32(x[a:b])
+    return out
+
+
+def _pulses_gpu(rf, x, high, cap=None):
+    import torch
+    xt = torch.from_numpy(np.ascontiguousarray(x, dtype=np.float32)).cuda()
+    pf = locate.PulseFinder(rf, len(x), cap_edges=cap)
+    pf.launch(xt, len(x), high=high)
+    e, m = pf.result()
+    return e.copy(), m.copy()
+
+
+_CAP = {}
+
+
+def _demod(system, nfields=5, seed=3):
+    key = (system, nfields, seed)
+    if key not in _CAP:
+        orf = O.OracleRF(system, decode_analog_audio=False)
+        fl = 667_334 if system == "NTSC" else 800_000
+        cap = synth.make_capture(system, nfields * fl, seed=seed, snr_db=40.0, jitter_ns=50, audio=False)
+        nbt = (len(cap.data) - 32768) // 31712 + 1
+        ref, _ = O.demod_span(orf, cap.data.astype(np.float64), 0, nbt * 31712 - 1)
+        _CAP[key] = (cap, {k: v.astype(np.float32) for k, v in ref.items()})
+    return _CAP[key]
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_pulse_edges_and_levels_exact(gpu_rf, system):
+    rf = gpu_rf(system, audio=False)
+    _, v = _demod(system)
+    d05 = v["demod_05"]
+    high = locate.pulse_threshold(rf)
+    e, m = _pulses_gpu(rf, d05, high)
+    want = _edges_np(d05, high)
+    np.testing.assert_array_equal(e, want)
+    wm = _means_np(d05, want, rf.freq * 10.0, rf.freq)
+    np.testing.assert_array_equal(np.isnan(m), np.isnan(wm))
+    np.testing.assert_array_equal(m[~np.isnan(m)], wm[~np.isnan(wm)])
+    assert (~np.isnan(m)).sum() >= 3 * 6      # every field's vertical sync pulses
+
+
+def test_pulse_edges_adversarial(gpu_rf):
+    """Thresholds met exactly, dense noise edges, a buffer starting inside a
+    pulse, lengths that are not multiples of a tile, and a capacity too small
+    for the first pass (the finder grows and searches again)."""
+    rf = gpu_rf("NTSC", audio=False)
+    rng = np.random.default_rng(5)
+    for n in (5, 17, 4095, 4096, 4097, 100_003):
+        x = rng.normal(size=n).astype(np.float32)
+        x[::7] = np.float32(0.25)
+        x[0] = -1.0
+        for high in (0.25, 0.0, float(np.float32(0.25)) + 1e-12):
+            e, _ = _pulses_gpu(rf, x, high, cap=64)
+            np.testing.assert_array_equal(e, _edges_np(x, high))
+
+
+def test_range_means_exact(gpu_rf):
+    import torch
+    rf = gpu_rf("NTSC", audio=False)
+    rng = np.random.default_rng(9)
+    x = (rng.normal(size=200_000) * 1e5 + 7.6e6).astype(np.float32)
+    b = rng.integers(0, 190_000, size=40)
+    ln = rng.integers(0, 20000, size=40)
+    ln[:5] = (0, 1, 129, 8192, 16385)
+    end = b + ln
+    xt = torch.from_numpy(x).cuda()
+    out = torch.empty(40, dtype=torch.float32, device="cuda")
+    bt, et = torch.from_numpy(b.astype(np.int64)).cuda(), torch.from_numpy(end.astype(np.int64)).cuda()
+    _lib.check(rf.lib.ld_range_means(rf.handle, xt.data_ptr(), len(x), bt.data_ptr(), et.data_ptr(), 40,
+                                     out.data_ptr(), None))
+    got = out.cpu().numpy()
+    for k in range(40):
+        want = LL.np_mean_f32(x[b[k]:end[k]]) if end[k] > b[k] else np.float32(np.nan)
+        assert (np.isnan(got[k]) and np.isnan(want)) or got[k] == want, (k, got[k], want)
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_located_decode_matches_oracle_walk(gpu_rf, system):
+    """CaptureDecoder(locate=True) against the oracle's sequential walk over
+    the GPU's own demod buffers, each oracle field finished with the GPU's
+    final line locations for it: the same fields, parity, confidence, line 0,
+    next-field offsets and linelocs1, bit for bit; and the located lines sit
+    at the synthetic truth."""
+    import torch
+    rf = gpu_rf(system, audio=False)
+    cap, _ = _demod(system)
+    dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).cuda(), locate=True, max_fields=8)
+    dec.step()
+    torch.cuda.synchronize()
+    locs, ll1, bad1 = dec.walk_state[:3]
+    d05 = dec.buffers["demod_05"].cpu().numpy()
+    dm = dec.buffers["demod"].cpu().numpy()
+    final = dec.linelocs.cpu().numpy()
+    L = locate.proclines(rf)
+    valid = [k for k, c in enumerate(locs) if c.status == _lib.LD_LOC_OK]
+    assert len(valid) >= 3
+    fin = {k: final[j * L:(j + 1) * L] for j, k in enumerate(valid)}
+    p = LL.LocParams.from_rf(O.OracleRF(system, decode_analog_audio=False))
+    ora = LL.walk_fields(p, d05, len(locs), 0, lambda k, res, org: fin[k], demod=dm)
+    assert len(ora) == len(locs)
+    for k, (c, o) in enumerate(zip(locs, ora)):
+        assert c.startloc == o["startloc"]
+        assert (c.status == _lib.LD_LOC_OK) == o["valid"]
+        if o["valid"]:
+            r = o["res"]
+            assert bool(c.is_first_field) == r.isFirstField and c.sync_confidence == r.sync_confidence
+            assert c.line0loc == r.line0loc and c.nextfieldoffset == r.nextfieldoffset
+            np.testing.assert_array_equal(ll1[k], r.linelocs1)
+    # truth: even-numbered raster fields are first fields; line starts trail
+    # the true starts by demod_05's edge delay
+    hpf = 525 if system == "NTSC" else 625
+    hs = cap.halfline_starts
+    for j, k in enumerate(valid):
+        c = locs[k]
+        raw = ll1[k] + c.startloc + 1024
+        f = int(round(raw[0] / (hpf * rf.SysParams["line_period"] * 20)))
+        idx = synth.field_line0(system, f) + 2 * np.arange(L)
+        d = raw - hs[idx]
+        assert 9.5 < d.min() and d.max() < 14.5
+        assert bool(c.is_first_field) == (f % 2 == 0)
+
+
+def test_pipelined_located_equals_sequential(gpu_rf):
+    """PipelinedDecoder(locate=True) over four passes gives the sequential
+    CaptureDecoder's fields and output codes (the last three passes each
+    still hold their output: three buffer sets)."""
+    import torch
+    rf = gpu_rf("NTSC", audio=False)
+    cap, _ = _demod("NTSC")
+    x = torch.from_numpy(cap.data).cuda()
+    seq = capture.CaptureDecoder(rf, x, locate=True, max_fields=8)
+    want = seq.step().clone()
+    torch.cuda.synchronize()
+    nf = len(seq.fields)
+    pipe = capture.PipelinedDecoder(rf, x, locate=True, max_fields=8)
+    outs = [pipe.step() for _ in range(4)]
+    assert outs[0] is None and outs[1] is None
+    outs = outs[2:] + pipe.join()
+    torch.cuda.synchronize()
+    assert len(outs) == 4
+    for d in outs[-3:]:
+        assert [f["demod_offset"] for f in d.fields] == [f["demod_offset"] for f in seq.fields]
+        n = nf * seq.linesout * seq.W
+        np.testing.assert_array_equal(d.out_u16[:n].cpu().numpy(), want[:n].cpu().numpy())
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_field_compute_linelocs(gpu_rf, system):
+    """Field.compute_linelocs on one field buffer (the reference's per-field
+    call) against the oracle's compute_linelocs on the same buffer, without
+    and with a previous field (its final line locations and buffers)."""
+    import torch
+    from lddecode_amd import field as fieldmod
+    rf = gpu_rf(system, audio=False)
+    _, v = _demod(system)
+    p = LL.LocParams.from_rf(O.OracleRF(system, decode_analog_audio=False))
+    ora = LL.walk_fields(p, v["demod_05"], 4, 0, lambda k, res, org: res.linelocs1 + 2.5, demod=v["demod"])
+    valid = [o for o in ora if o["valid"]]
+    a, b = valid[0], valid[1]
+    S = 31712
+
+    def buf(o):
+        sl = slice(o["startloc"], o["startloc"] + o["nblocks"] * S)
+        return {"video": {k: torch.from_numpy(np.ascontiguousarray(v[k][sl])).cuda() for k in ("demod", "demod_05")}}
+
+    fa = fieldmod.Field(rf, buf(a))
+    ll, bad, off = fa.compute_linelocs(None, fields_written=1, startloc=a["startloc"])
+    ra = LL.compute_linelocs(p, v["demod_05"][a["startloc"]:a["startloc"] + a["nblocks"] * S], None, a["startloc"], 1)
+    np.testing.assert_array_equal(ll, ra.linelocs1)
+    np.testing.assert_array_equal(bad, ra.linebad)
+    assert off == ra.nextfieldoffset and fa.isFirstField == ra.isFirstField
+    fa.linelocs = ll + 2.5                       # stands in for its aligned positions
+    fb = fieldmod.Field(rf, buf(b))
+    ll, bad, off = fb.compute_linelocs(fa, fields_written=1, startloc=b["startloc"])
+    np.testing.assert_array_equal(ll, b["res"].linelocs1)
+    assert fb.line0loc == b["res"].line0loc and fb.sync_confidence == b["res"].sync_confidence
+    assert off == b["res"].nextfieldoffset

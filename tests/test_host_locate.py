@@ -94,11 +94,11 @@ def test_walk_matches_oracle(system):
         if o["valid"]:
             llc, meds = LL.field_facts(p, dm[o["startloc"]:o["startloc"] + o["nblocks"] * 31712], o["final"],
                                        o["res"].isFirstField)
-            f.ll_linecount = llc + o["startloc"]
+            f.ll_linecount = llc
             f.medians[:] = meds
             f.known = 1
         facts.append(f)
-    locs, ll, bad, _ = locate.walk(rf, edges, means, len(d05), 0.0, None, facts, max_fields=5)
+    locs, ll, bad, _, _ = locate.walk(rf, edges, means, len(d05), 0.0, None, facts, max_fields=5)
     assert len(locs) == len(ora)
     for k, (c, o) in enumerate(zip(locs, ora)):
         _loc_equal(c, o, o["startloc"])
@@ -107,7 +107,7 @@ def test_walk_matches_oracle(system):
             np.testing.assert_array_equal(bad[k].astype(bool), o["res"].linebad)
     assert sum(o["valid"] for o in ora) == 3
     # the walk with provisional facts reaches the same fields here
-    locs2, ll2, _, _ = locate.walk(rf, edges, means, len(d05), 0.0, None, None, max_fields=5)
+    locs2, ll2, _, _, _ = locate.walk(rf, edges, means, len(d05), 0.0, None, None, max_fields=5)
     assert [c.startloc for c in locs2] == [c.startloc for c in locs]
     np.testing.assert_array_equal(ll2, ll)
 
@@ -132,11 +132,11 @@ def test_prev_field_dependence(system):
             f = _lib.FieldFacts()
             if o["valid"]:
                 lc = o["res"].linecount if system == "NTSC" else (312 if o["res"].isFirstField else 313)
-                f.ll_linecount = o["final"][lc] + o["startloc"]
+                f.ll_linecount = o["final"][lc]
                 f.medians[:] = [p.ire0] * 8 if quiet else [p.iretohz(p.vsync_ire)] * 8
                 f.known = 1
             facts.append(f)
-        locs, ll, bad, _ = locate.walk(rf, edges, means, len(d05), 0.0, None, facts, max_fields=5)
+        locs, ll, bad, _, _ = locate.walk(rf, edges, means, len(d05), 0.0, None, facts, max_fields=5)
         assert len(locs) == len(ora)
         for k, (c, o) in enumerate(zip(locs, ora)):
             _loc_equal(c, o, o["startloc"])
@@ -185,7 +185,7 @@ def test_damaged_pulse_trains(system):
         if prev is not None:
             cprev = _lib.PrevField(valid=1, is_first_field=int(prev.isFirstField), linecount=prev.linecount,
                                    sync_confidence=prev.sync_confidence, startloc=prev.startloc,
-                                   ll_linecount=prev.linelocs[prev.linecount] + prev.startloc,
+                                   ll_linecount=prev.linelocs[prev.linecount],
                                    skip_score=prev.skip_score, fields_written=3)
         if res is None:
             with pytest.raises(_lib.LdGpuError):
@@ -225,3 +225,48 @@ def test_skip_score_closed_forms():
         f = _lib.FieldFacts()
         f.medians[:] = meds
         assert locate.skip_scores(rf, [f], P)[0] == want == LL.skip_score_from(p, meds)
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_recheck_settles_to_sequential_walk(system):
+    """The batched decoder's scheme on the host: walk on provisional
+    previous-field values, measure each field's facts from its line table
+    (here final = linelocs1 + a shift, standing in for the GPU alignment),
+    re-check, and walk again from the first changed field until the check
+    passes.  The settled records equal the oracle's sequential walk, line 0
+    included, for shifts that leave the line tables alone and for ones that
+    move line 0 across a pulse."""
+    rf, orf = _rf(system)
+    p = LL.LocParams.from_rf(orf)
+    d05, _ = _capture(system)
+    edges, means = _edges_means(p, d05)
+    for shift in (1.7, 0.3 * p.linelen, -0.45 * p.linelen):
+        def facts_for(locs, ll):
+            out = []
+            for k, c in enumerate(locs):
+                f = _lib.FieldFacts()
+                if c.status == _lib.LD_LOC_OK:
+                    lc = c.linecount if system == "NTSC" else (312 if c.is_first_field else 313)
+                    f.ll_linecount = ll[k][lc] + shift
+                    f.medians[:] = [p.iretohz(p.vsync_ire)] * 8
+                    f.known = 1
+                out.append(f)
+            return out
+
+        locs, ll, bad, _, vp = locate.walk(rf, edges, means, len(d05), 0.0, None, None, max_fields=5)
+        for _ in range(6):
+            facts = facts_for(locs, ll)
+            first, checked = locate.recheck(rf, locs, ll, bad, vp, facts)
+            if first < 0:
+                locs = checked
+                break
+            keep = facts[:first] + [_lib.FieldFacts() for _ in facts[first:]]
+            locs, ll, bad, _, vp = locate.walk(rf, edges, means, len(d05), 0.0, None, keep, max_fields=5)
+        else:
+            raise AssertionError("did not settle")
+        ora = LL.walk_fields(p, d05, 5, 0, lambda k, res, org: res.linelocs1 + shift)
+        assert len(ora) == len(locs)
+        for k, (c, o) in enumerate(zip(locs, ora)):
+            _loc_equal(c, o, o["startloc"])
+            if o["valid"]:
+                np.testing.assert_array_equal(ll[k], o["res"].linelocs1)
