@@ -52,9 +52,10 @@ def capture_samples(nfields):
     return (nfields + 2) * NTSC_FIELD_SAMPLES + 100_000
 
 
-def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None):
+def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=False):
     """Time the located decode.  single: (start, startloc, buflen) of one
-    field read -- the plain figure decodes just that field per step."""
+    field read -- the plain figure decodes just that field per step.
+    packed: the capture in the packed 10-bit format (4 samples in 5 bytes)."""
     from lddecode_amd import capture
     data = cap.data
     start = 0.0
@@ -62,6 +63,9 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None):
         st, org, n = single
         data = data[org: org + n + rf.blocklen - rf.block_stride]
         start = st - org
+    if packed:
+        from lddecode_amd import synth
+        data = synth.pack_10bit(data)
     x = torch.from_numpy(np.ascontiguousarray(data)).to(rf.device)
     audio = True
     dec = capture.CaptureDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start)
@@ -70,10 +74,15 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None):
     for _ in range(warmup):
         dec.step()
     torch.cuda.synchronize()
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(steps)]
+    if packed:
+        # the pipelined figure only
+        steps_seq = 0
+    else:
+        steps_seq = steps
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(max(steps_seq, 1))]
     walk_s = []
     t0 = time.perf_counter()
-    for i in range(steps):
+    for i in range(len(ev)):
         e = ev[i]
         e[0].record(stream)
         dec.demod()
@@ -104,7 +113,7 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None):
     if single is None:
         # consecutive passes overlapped on two streams: stream A demodulates
         # pass k and finds its pulses while the host walks pass k-1 and
-        # stream B aligns and resamples it, two buffer sets
+        # stream B aligns and resamples it; three buffer sets
         pipe = capture.PipelinedDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start)
         for _ in range(warmup):
             pipe.step()
@@ -333,6 +342,8 @@ def main():
     if batched["nfields"] != nf:
         raise RuntimeError(f"located {batched['nfields']} fields, expected {nf}")
     plain = run_gpu(args, torch, rf, cap, 1, args.steps, args.warmup, single=batched["first_field"])
+    packed = run_gpu(args, torch, rf, cap, nf, args.steps, args.warmup, packed=True)
+    print(f"# packed 10-bit input: {packed['pipe_ms']:.3f} ms/step overlapped", file=sys.stderr, flush=True)
     print(f"# plain: {plain['step_ms']:.3f} ms/step", file=sys.stderr, flush=True)
 
     value = batched["usable"] / (batched["pipe_ms"] / 1e3) / 1e6
@@ -380,6 +391,10 @@ def main():
         "plain": {"value": round(plain_value, 2), "unit": "Msamples/s", "ms_per_step": round(plain["step_ms"], 4),
                   "fields_per_step": 1, "blocks_per_step": plain["nblocks"],
                   "x_realtime": round(plain_value / 40.0, 1)},
+        "packed_input": {"value": round(batched["usable"] / (packed["pipe_ms"] / 1e3) / 1e6, 2),
+                         "unit": "Msamples/s", "ms_per_step": round(packed["pipe_ms"], 4),
+                         "input": "the same capture in the packed 10-bit format (1.25 B/sample), read by "
+                                  "k_demod_block directly"},
         "sequential_ms_per_step": round(batched["step_ms"], 4),
         "stages_ms": {"demod": round(batched["demod_ms"], 4), "pulses": round(batched["pulses_ms"], 4),
                       "locate_walk_and_hsync": round(batched["locate_hsync_ms"], 4),

@@ -51,7 +51,12 @@ enum {
   LD_NFILT = 6
 };
 
-enum { LD_IN_INT16 = 0, LD_IN_FLOAT32 = 1 };
+/* Input sample formats: int16 (10-bit codes << 6, utils.py:388), float32,
+ * and the packed 10-bit capture format (4 samples in 5 bytes, big-endian
+ * bit order, utils.py:356-390); input_len counts samples in every format,
+ * and a packed buffer holds (input_len + 3) / 4 * 5 bytes and must be
+ * readable up to the next multiple of 4 bytes (device allocations are). */
+enum { LD_IN_INT16 = 0, LD_IN_FLOAT32 = 1, LD_IN_PACKED10 = 2 };
 
 typedef struct ld_handle ld_handle;
 
@@ -126,6 +131,13 @@ typedef struct ld_demod_outputs {
  * Replaces RFDecode.demodblock(data=..., mtf_level, cut), core.py:747-879 (the
  * MTF level is folded into LD_FILT_RFVIDEO by the host). */
 size_t ld_demod_workspace_bytes(const ld_handle* h, int nblocks);
+
+/* Unpack samples [first, first + n) of a packed 10-bit capture (DEVICE
+ * bytes, packed_bytes long) to int16 (code - 512) << 6 in DEVICE `out`:
+ * unpack_data_4_40 / load_packed_data_4_40 (utils.py:371-410), whose
+ * `sample` is `first` here.  LD_ERANGE when the samples pass the buffer. */
+int ld_unpack_10bit(ld_handle* h, const uint8_t* packed, int64_t packed_bytes, int64_t first, int64_t n,
+                    int16_t* out, void* stream);
 int ld_demod_blocks(ld_handle* h, const void* input, int input_dtype, int64_t input_len,
                     int64_t first, int64_t step, int nblocks, int cut,
                     const ld_demod_outputs* out, void* workspace, size_t workspace_bytes,

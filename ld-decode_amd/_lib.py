@@ -13,7 +13,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().with_name("libldgpu.so")
 
 LD_OK, LD_EINVAL, LD_ERANGE, LD_EHIP, LD_ESTATE = 0, -1, -2, -3, -4
 LD_FILT_RFHPF, LD_FILT_RFVIDEO, LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT = range(6)
-LD_IN_INT16, LD_IN_FLOAT32 = 0, 1
+LD_IN_INT16, LD_IN_FLOAT32, LD_IN_PACKED10 = 0, 1, 2
 ABI_VERSION = 1
 
 # every symbol include/ldgpu.h declares
@@ -24,7 +24,7 @@ EXPORTS = (
     "ld_refine_workspace_bytes", "ld_refine_burst_ntsc", "ld_refine_pilot_pal",
     "ld_pulse_workspace_bytes", "ld_find_pulses", "ld_pulse_means", "ld_range_means",
     "ld_field_facts_workspace_bytes", "ld_field_facts_measure", "ld_locate_field", "ld_locate_fields",
-    "ld_skip_score", "ld_locate_recheck",
+    "ld_skip_score", "ld_locate_recheck", "ld_unpack_10bit",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -160,6 +160,7 @@ def _declare(lib):
                                      P(FieldFacts), c_int, P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int,
                                      P(PrevField), P(c_int)]),
         "ld_skip_score": (c_int, [P(LocateParams), P(c_dbl)]),
+        "ld_unpack_10bit": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
         "ld_locate_recheck": (c_int, [P(LocateParams), P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int, c_int,
                                       P(PrevField), P(FieldFacts), P(c_int)]),
     }

@@ -58,7 +58,9 @@ class CaptureDecoder:
         self.torch = torch
         self.data = data
         N, S = rf.blocklen, rf.block_stride
-        self.nblocks = (data.numel() - N) // S + 1
+        # samples in the capture (uint8: the packed 10-bit format, 4 per 5 bytes)
+        nsamp = data.numel() // 5 * 4 if data.dtype == torch.uint8 else data.numel()
+        self.nblocks = (nsamp - N) // S + 1
         self.total = self.nblocks * S
         dev = rf.device
         names = list(rf.video_fields) + ["rfhpf"]

@@ -109,7 +109,8 @@ struct C2rOut {
 struct DemodArgs {
   const void* in;
   int64_t first, step;
-  int in_bytes;            // bytes per input sample (2: int16, 4: float32)
+  int in_bytes;            // bytes per input sample (2: int16, 4: float32, 0: packed 10-bit)
+  uint32_t in_total;       // packed input: bytes of the whole input buffer
   const float2* tw_rows;   // Stockham pass twiddles of the audio transform (TwLayout)
   const float2* tw_lds;    // TwLds image, copied to LDS by every workgroup
   const float2* h_rfv;
@@ -361,7 +362,10 @@ __device__ __forceinline__ void load_half(const float2* table, float2 (&h)[kP], 
 //   5..8     D * post filter -> video outputs   (c2r)
 enum { kPhIn = 0, kPhHpf = 1, kPhEven = 2, kPhOdd = 3, kPhDemod = 4, kPhPost = 5 };
 
-__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
+// kPacked: the input is the packed 10-bit format (a separate instantiation,
+// so that the int16 / float32 kernel's register allocation is not touched).
+template <bool kPacked>
+__device__ __forceinline__ void demod_block(const DemodArgs& a) {
   __shared__ Lds s;
   __shared__ float2 aslice[2][kNA / 2 + 1];
   __shared__ TwLds twl;
@@ -396,6 +400,38 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
   const int last = kPhPost + a.npost;
   const bool has_hpf = a.hpf.out != nullptr;
 
+  if constexpr (kPacked) {
+    // packed 10-bit input (utils.py:371-390): sample j is the big-endian
+    // 16-bit word at byte 5 (j >> 2) + (j & 3), shifted right by
+    // 6 - 2 (j & 3); a sample pair always lies in the 4 bytes from its first
+    // byte.  The block's bytes are staged in LDS (the exchange area is free
+    // before the first transform) as dwords from the one holding its first
+    // byte, then each thread decodes its 32 pairs into v, as phase 0 of the
+    // int16 path loads them.
+    const rsrc_t pk = make_rsrc(a.in, a.in_total);
+    const uint32_t q5 = (uint32_t)(5 * (s0 >> 2));
+    const uint32_t r = (uint32_t)(s0 & 3);
+    const uint32_t A0 = q5 & ~3u;
+    const uint32_t lastb = q5 + 5u * ((r + (uint32_t)kN - 1u) >> 2) + ((r + (uint32_t)kN - 1u) & 3u) + 1u;
+    const int nwords = (int)((lastb - A0) / 4u + 1u);
+    uint32_t* w32 = reinterpret_cast<uint32_t*>(s.re);
+    for (int i = tid; i < nwords; i += kT) w32[i] = __builtin_bit_cast(uint32_t, bload1(pk, A0 + 4u * (uint32_t)i, 0u));
+    __syncthreads();
+    static_for<0, kP>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const uint32_t jr = r + 2u * (uint32_t)(tid + kT * m);
+      const uint32_t B = q5 + 5u * (jr >> 2) + (jr & 3u) - A0;
+      const uint32_t W = __builtin_amdgcn_alignbyte(w32[(B >> 2) + 1u], w32[B >> 2], B & 3u);
+      const uint32_t r0 = jr & 3u;
+      const uint32_t b0 = W & 0xffu, b1 = (W >> 8) & 0xffu, b2 = (W >> 16) & 0xffu, b3 = W >> 24;
+      const uint32_t v0 = (((b0 << 8) | b1) >> (6u - 2u * r0)) & 0x3ffu;
+      const uint32_t v1 = r0 < 3u ? ((((b1 << 8) | b2) >> (4u - 2u * r0)) & 0x3ffu) : ((((b2 << 8) | b3) >> 6) & 0x3ffu);
+      // (code - 512) << 6 as the int16 path sees it, exactly
+      v[m] = make_float2((float)((int)v0 - 512) * 64.0f, (float)((int)v1 - 512) * 64.0f);
+    });
+    __syncthreads();    // the first transform reuses the exchange area
+  }
+
   for (int ph = 0; ph < last; ++ph) {
     // the split twiddles derived from wt are loop-invariant; keep the
     // compiler from hoisting (and spilling) all 32 of them out of the loop
@@ -406,7 +442,9 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
     if (ph == kPhIn) {
       // spectrum of the block (core.py:761): y[j] = x[2j] + i x[2j+1]
       const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(kN * a.in_bytes));
-      if (a.in_bytes == 4) {
+      if constexpr (kPacked) {
+        // decoded before the phase loop (see the prologue)
+      } else if (a.in_bytes == 4) {
 #pragma unroll
         for (int m = 0; m < kP; ++m) v[m] = bload2(in, vo, m * kStrideB);
       } else if ((s0 & 1) == 0) {
@@ -564,6 +602,9 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) {
       audio_wave(a, c, aslice[c], s.re + c * LdsSize<kNA>::floats, s.im + c * LdsSize<kNA>::floats, b, tg);
   }
 }
+
+__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false>(a); }
+__global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true>(a); }
 
 }  // namespace ldgpu
 
@@ -739,7 +780,7 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
     set_error("ld_demod_blocks: null argument or nblocks <= 0");
     return LD_EINVAL;
   }
-  if (dtype != LD_IN_INT16 && dtype != LD_IN_FLOAT32) {
+  if (dtype != LD_IN_INT16 && dtype != LD_IN_FLOAT32 && dtype != LD_IN_PACKED10) {
     set_error("ld_demod_blocks: unknown input dtype %d", dtype);
     return LD_EINVAL;
   }
@@ -825,10 +866,25 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.dc = (float)h->dc_offset;
   a.notch_lo = h->notch_lo;
   a.notch_hi = h->notch_hi;
-  a.in_bytes = dtype == LD_IN_INT16 ? 2 : 4;
+  a.in_bytes = dtype == LD_IN_INT16 ? 2 : (dtype == LD_IN_FLOAT32 ? 4 : 0);
+  a.in_total = 0;
+  if (dtype == LD_IN_PACKED10) {
+    const int64_t bytes = (input_len + 3) / 4 * 5;
+    if (bytes > 0xfffffff0LL) {
+      set_error("ld_demod_blocks: packed input over 4 GiB (pass a shorter span)");
+      return LD_EINVAL;
+    }
+    // dword loads cover the pairs' bytes: every dword holding a sample byte
+    // lies below the byte count rounded up to 4 (the buffer's allocation
+    // covers that round-up)
+    a.in_total = (uint32_t)((bytes + 3) / 4 * 4);
+  }
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
-  hipLaunchKernelGGL(k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
+  if (dtype == LD_IN_PACKED10)
+    hipLaunchKernelGGL(k_demod_block_packed, dim3(nblocks), dim3(kT), 0, st, a);
+  else
+    hipLaunchKernelGGL(k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

@@ -254,19 +254,38 @@ class RFDecode:
         return self._ws
 
     def _as_device_input(self, data):
+        """(device tensor, LD_IN_* format, samples).  uint8 data is the packed
+        10-bit capture format (4 samples in 5 bytes, utils.py:356-390)."""
         torch = self.torch
         if isinstance(data, torch.Tensor):
             t = data if data.device == self.device else data.to(self.device)
         else:
             a = np.asarray(data)
-            if a.dtype not in (np.int16, np.float32):
+            if a.dtype not in (np.int16, np.float32, np.uint8):
                 a = a.astype(np.float32)
             t = torch.from_numpy(np.ascontiguousarray(a)).to(self.device)
+        if t.dtype == torch.uint8:
+            return t.contiguous(), _lib.LD_IN_PACKED10, t.numel() // 5 * 4
         if t.dtype == torch.int16:
-            return t.contiguous(), _lib.LD_IN_INT16
+            return t.contiguous(), _lib.LD_IN_INT16, t.numel()
         if t.dtype != torch.float32:
             t = t.float()
-        return t.contiguous(), _lib.LD_IN_FLOAT32
+        return t.contiguous(), _lib.LD_IN_FLOAT32, t.numel()
+
+    def unpack_10bit(self, packed, first=0, n=None, stream=None):
+        """load_packed_data_4_40's unpacking (utils.py:371-410) on the GPU:
+        samples [first, first + n) of a packed capture as int16 (code - 512)
+        << 6."""
+        torch = self.torch
+        x, dt, total = self._as_device_input(packed)
+        if dt != _lib.LD_IN_PACKED10:
+            raise TypeError("unpack_10bit takes packed bytes (uint8)")
+        n = total - first if n is None else n
+        out = torch.empty(n, dtype=torch.int16, device=self.device)
+        st = stream if stream is not None else self._stream()
+        _lib.check(self.lib.ld_unpack_10bit(self.handle, x.data_ptr(), x.numel(), int(first), int(n),
+                                            out.data_ptr(), ctypes.c_void_p(st)))
+        return out
 
     def demod_blocks(self, data, first=0, nblocks=None, mtf_level=0, cut=True, step=None,
                      audio=None, channels=None, out=None, stream=None, _raw_level=False):
@@ -276,12 +295,12 @@ class RFDecode:
         stitched DemodCache arrays: element i is raw sample first + 1024 + i.
         Returns {'video': Channels, 'rfhpf': tensor, 'audio': Channels}."""
         torch = self.torch
-        x, dt = self._as_device_input(data)
+        x, dt, nsamp = self._as_device_input(data)
         N = self.blocklen
         S = self.block_stride
         step = S if step is None else step
         if nblocks is None:
-            nblocks = (x.numel() - first - N) // step + 1
+            nblocks = (nsamp - first - N) // step + 1
         if nblocks <= 0:
             raise ValueError("not enough input for one block")
         self._upload_rfvideo(mtf_level if _raw_level else self._level(mtf_level))
@@ -322,7 +341,7 @@ class RFDecode:
         o.audio_stride = alen
         ws = self._workspace(nblocks)
         st = stream if stream is not None else self._stream()
-        _lib.check(self.lib.ld_demod_blocks(self.handle, x.data_ptr(), dt, x.numel(), first, step, nblocks,
+        _lib.check(self.lib.ld_demod_blocks(self.handle, x.data_ptr(), dt, nsamp, first, step, nblocks,
                                             1 if cut else 0, ctypes.byref(o), ws.data_ptr(), ws.numel(),
                                             ctypes.c_void_p(st)))
         rv = {"video": Channels((n, res[n]) for n in self.video_fields if n in res)}

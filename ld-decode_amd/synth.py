@@ -173,3 +173,19 @@ def make_capture(system="NTSC", nsamples=700_000, seed=1, snr_db=40.0, jitter_ns
     code = np.clip(np.round(512 + 511 * sig / np.max(np.abs(sig))), 0, 1023).astype(np.int32)
     data = ((code - 512) << 6).astype(np.int16)
     return Capture(data, line_starts, hstarts, system, sysp, fs, snr_db)
+
+
+def pack_10bit(samples):
+    """An int16 capture ((code - 512) << 6) in the packed 10-bit format of
+    the Domesday Duplicator: four 10-bit codes in five bytes, most
+    significant bits first (the layout utils.py:356-390 unpacks); a partial
+    last group is padded with code 0."""
+    c = (np.asarray(samples, dtype=np.int32) >> 6) + 512
+    c = np.concatenate([c, np.zeros((-len(c)) % 4, np.int32)]).astype(np.uint32).reshape(-1, 4)
+    out = np.empty((len(c), 5), dtype=np.uint8)
+    out[:, 0] = c[:, 0] >> 2
+    out[:, 1] = ((c[:, 0] & 0x03) << 6) | (c[:, 1] >> 4)
+    out[:, 2] = ((c[:, 1] & 0x0F) << 4) | (c[:, 2] >> 6)
+    out[:, 3] = ((c[:, 2] & 0x3F) << 2) | (c[:, 3] >> 8)
+    out[:, 4] = c[:, 3] & 0xFF
+    return out.reshape(-1)
