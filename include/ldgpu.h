@@ -420,6 +420,7 @@ typedef struct ld_field_loc {
   int32_t vb_conf[2];         /* -1: None */
   int32_t vb_flags[2];        /* LD_VB_* */
   int32_t reserved;
+  double hsync_max;           /* get_timings' LT['hsync'][1] (dropout detection's hsync length) */
 } ld_field_loc;
 
 /* One classified pulse (refinepulses' validpulses entry, core.py:1914-1949). */
@@ -474,6 +475,63 @@ int ld_locate_fields(const ld_locate_params* p, const int64_t* edges, int64_t ne
 int ld_locate_recheck(const ld_locate_params* p, ld_field_loc* locs, double* linelocs1, uint8_t* linebad,
                       int max_lines, const ld_vpulse* vpulses, int max_vpulses, int n, const ld_prev_field* prev,
                       const ld_field_facts* facts, int* first_changed);
+
+/* ---------------------------------------------------------------------
+ * Dropout detection (Field.dropout_detect, core.py:2808-2996).
+ * ------------------------------------------------------------------- */
+
+/* One field: its buffer in the stitched channels, its final line locations
+ * (entries [lineloc_offset, + nlines) of the DEVICE array), lineoffset and
+ * linecount (0 / 263|262 NTSC, 2|3 / 312|313 PAL), isFirstField and
+ * get_timings' LT['hsync'][1] (ld_field_loc.hsync_max). */
+typedef struct ld_dropout_field {
+  int64_t demod_offset;
+  int64_t demod_len;
+  int64_t lineloc_offset;
+  int32_t nlines;
+  int32_t lineoffset;
+  int32_t linecount;
+  int32_t first_field;
+  double hsync_max;
+} ld_dropout_field;
+
+/* The levels of dropout_detect_demod as float32 (np.full_like of the
+ * float32 buffers): default / sync-area minimum and maximum of demod and
+ * demod_05, freq_hz / 2 for demod_raw, rf.delays['video_rot']. */
+typedef struct ld_dropout_params {
+  double freq_hz_half;
+  float vmin, vmax, vmin05, vmax05, sync_min, sync_min05;
+  int32_t video_rot;
+  int32_t pal;
+} ld_dropout_params;
+
+size_t ld_dropout_workspace_bytes(const ld_handle* h, const ld_dropout_field* fields, int nfields);
+
+/* dropout_detect_demod (core.py:2830-2887) for every field of a batch:
+ * the error samples of each field buffer (rfhpf beyond 3 std of the buffer,
+ * shifted by video_rot; demod_raw above freq_hz / 2; demod / demod_05 out of
+ * range, with the sync-area minimum over each line's hsync length and over
+ * the vertical-sync lines; only between int(linelocs[lineoffset + 1]) and
+ * int(linelocs[lineoffset + linecount + 1])), compacted in order:
+ * positions (DEVICE, buffer-relative, at most cap) hold field 0's errors,
+ * then field 1's, ...; field_start (DEVICE, nfields + 1) each field's first
+ * index and the total.  `fields` is a HOST array; the channels and linelocs
+ * are DEVICE arrays.  The buffer's std is a double reduction (the
+ * reference's numba std sums in order: the float32 threshold agrees).
+ * Line locations are taken to increase along each field. */
+int ld_dropout_detect(ld_handle* h, const float* rfhpf, const float* demod, const float* demod_raw,
+                      const float* demod_05, int64_t total, const ld_dropout_field* fields, int nfields,
+                      const double* linelocs, int64_t nlinelocs, const ld_dropout_params* params,
+                      int64_t* positions, int64_t cap, int64_t* field_start, void* workspace,
+                      size_t workspace_bytes, void* stream);
+
+/* build_errlist + dropout_errlist_to_tbc + dropout_detect's packing
+ * (core.py:2890-2996) for one field from its sorted error positions and
+ * final line locations (HOST arrays): lines (field line - 1), starts, ends
+ * in output pixels, at most cap; *count gets the number.  Host only. */
+int ld_dropout_list(const int64_t* errmap, int64_t n, const double* linelocs, int nlines, int lineoffset,
+                    int linecount, int outlinelen, double freq, int32_t* lines, int32_t* starts, int32_t* ends,
+                    int cap, int* count);
 
 /* skip_check's score (core.py:2139-2155) from a field's eight line medians. */
 int ld_skip_score(const ld_locate_params* p, const double* medians8);

@@ -25,6 +25,7 @@ EXPORTS = (
     "ld_pulse_workspace_bytes", "ld_find_pulses", "ld_pulse_means", "ld_range_means",
     "ld_field_facts_workspace_bytes", "ld_field_facts_measure", "ld_locate_field", "ld_locate_fields",
     "ld_skip_score", "ld_locate_recheck", "ld_unpack_10bit",
+    "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -109,11 +110,23 @@ class FieldLoc(ctypes.Structure):
         (n, ctypes.c_int32) for n in ("status", "is_first_field", "linecount", "sync_confidence", "skipdetected",
                                       "npulses", "nlines", "analyzed", "nvalid")] + [
         ("last_start", ctypes.c_int64), ("vb_line0", ctypes.c_double * 2), ("vb_firstblank", ctypes.c_int64 * 2), ("vb_conf", ctypes.c_int32 * 2),
-        ("vb_flags", ctypes.c_int32 * 2), ("reserved", ctypes.c_int32)]
+        ("vb_flags", ctypes.c_int32 * 2), ("reserved", ctypes.c_int32), ("hsync_max", ctypes.c_double)]
 
 
 class VPulse(ctypes.Structure):
     _fields_ = [("start", ctypes.c_int64), ("len", ctypes.c_int32), ("kind", ctypes.c_int16), ("good", ctypes.c_int16)]
+
+
+class DropoutField(ctypes.Structure):
+    _fields_ = [("demod_offset", ctypes.c_int64), ("demod_len", ctypes.c_int64), ("lineloc_offset", ctypes.c_int64),
+                ("nlines", ctypes.c_int32), ("lineoffset", ctypes.c_int32), ("linecount", ctypes.c_int32),
+                ("first_field", ctypes.c_int32), ("hsync_max", ctypes.c_double)]
+
+
+class DropoutParams(ctypes.Structure):
+    _fields_ = [("freq_hz_half", ctypes.c_double)] + [
+        (n, ctypes.c_float) for n in ("vmin", "vmax", "vmin05", "vmax05", "sync_min", "sync_min05")] + [
+        ("video_rot", ctypes.c_int32), ("pal", ctypes.c_int32)]
 
 
 _lib = None
@@ -163,6 +176,11 @@ def _declare(lib):
         "ld_unpack_10bit": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
         "ld_locate_recheck": (c_int, [P(LocateParams), P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int, c_int,
                                       P(PrevField), P(FieldFacts), P(c_int)]),
+        "ld_dropout_workspace_bytes": (c_sz, [c_vp, P(DropoutField), c_int]),
+        "ld_dropout_detect": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, P(DropoutField), c_int, c_vp, c_i64,
+                                      P(DropoutParams), c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
+        "ld_dropout_list": (c_int, [c_vp, c_i64, c_vp, c_int, c_int, c_int, c_int, c_dbl, c_vp, c_vp, c_vp, c_int,
+                                    P(c_int)]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)

@@ -13,6 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from . import dropouts as dropoutmod
 from . import field as fieldmod
 from . import locate as locatemod
 
@@ -108,6 +109,7 @@ class CaptureDecoder:
         self.coord_f32 = coord_f32
         self.smoothing = smoothing
         self.fields = []
+        self.dropout_det = None
         if self.locating:
             fsz = ctypes.sizeof(_lib.FieldFacts)
             self.walker = locatemod.FieldWalker(rf, self.total, self.max_fields)
@@ -181,7 +183,8 @@ class CaptureDecoder:
             first = bool(c.is_first_field)
             lo = (2 if first else 3) if rf.system == "PAL" else self.default_lineoffset
             fields.append(dict(demod_offset=int(c.startloc), demod_len=int(c.buflen), nlines=L,
-                               linecount=int(c.linecount), first_field=int(first), lineoffset=lo))
+                               linecount=int(c.linecount), first_field=int(first), lineoffset=lo,
+                               hsync_max=float(c.hsync_max)))
         n = len(valid) * L
         if valid:
             ll = ll1[valid]
@@ -277,6 +280,19 @@ class CaptureDecoder:
             self.tbc()
             self.measure_facts()
         return rounds
+
+    def dropouts(self, stream=None):
+        """Enqueue dropout detection (Field.dropout_detect, core.py:2978-2996)
+        for every field of the pass, over its buffers and final line
+        locations; dropout_lists() returns the results."""
+        if self.dropout_det is None:
+            self.dropout_det = dropoutmod.DropoutDetector(self.rf)
+        n = sum(f["nlines"] for f in self.fields)
+        self.dropout_det.detect(self.buffers, self.fields, self.linelocs[:n], stream=stream)
+
+    def dropout_lists(self):
+        """Per field of the pass, the reference's (lines, starts, ends)."""
+        return self.dropout_det.lists() if self.dropout_det is not None else []
 
     def _facts_list(self):
         n = len(self.fields)

@@ -669,6 +669,7 @@ int assign(const Geometry& g, const std::vector<VPulse>& valid, int64_t last_sta
 int locate(const Geometry& g, const std::vector<Pulse>& pulses, const ld_prev_field* prev, int64_t startloc,
            ld_field_loc* out, double* ll_out, uint8_t* bad_out, Scratch& sc, ld_vpulse* keep, int keep_cap) {
   const Timings t = get_timings(g, pulses, sc);
+  out->hsync_max = t.hsync[1];
   refinepulses(g, t, pulses, sc);
   const std::vector<VPulse>& valid = sc.valid;
   const double meanlinelen = compute_line_len(g, valid);
@@ -940,5 +941,69 @@ extern "C" int ld_locate_fields(const ld_locate_params* p, const int64_t* edges,
   }
   *nfields_out = nf;
   if (prev_out) *prev_out = st;
+  return LD_OK;
+}
+
+// ------------------------------------------------------------- dropouts
+extern "C" int ld_dropout_list(const int64_t* errmap, int64_t n, const double* ll, int nlines, int lineoffset,
+                               int linecount, int outlinelen, double freq, int32_t* lines, int32_t* starts,
+                               int32_t* ends, int cap, int* count) {
+  if (!count || n < 0 || (n > 0 && !errmap) || !ll || lineoffset < 0 || linecount < 1 ||
+      lineoffset + linecount + 1 > nlines || outlinelen <= 0 || cap < 0 || (cap > 0 && (!lines || !starts || !ends))) {
+    set_error("ld_dropout_list: bad arguments");
+    return LD_EINVAL;
+  }
+  *count = 0;
+  if (n == 0 || !((double)errmap[n - 1] > ll[lineoffset])) return LD_OK;
+  // build_errlist (core.py:2890-2908)
+  int64_t fi = 0;
+  while ((double)errmap[fi] < ll[lineoffset]) ++fi;
+  const double firsterr = (double)errmap[fi];
+  std::vector<std::pair<double, double>> el;
+  double c0 = firsterr, c1 = firsterr;
+  for (int64_t i = 0; i < n; ++i) {
+    const double e = (double)errmap[i];
+    if (e > c0 && e <= c1 + 20) {
+      double pad = (e - c0) * 1.7;
+      pad = std::min(pad, freq * 12);
+      c1 = c0 + pad;
+    } else if (e > firsterr) {
+      el.push_back({c0 - 8, c1 + 4});
+      c0 = c1 = e;
+    }
+  }
+  el.push_back({c0, c1});
+  // dropout_errlist_to_tbc (core.py:2911-2975)
+  size_t idx = 0;
+  std::pair<double, double> cur = el[idx++];
+  bool have = true;
+  while (idx < el.size() && cur.first < ll[lineoffset]) cur = el[idx++];
+  int k = 0;
+  auto emit = [&](int64_t line, int64_t s, int64_t e) {
+    if (k < cap) {
+      lines[k] = (int32_t)(line - 1);
+      starts[k] = (int32_t)s;
+      ends[k] = (int32_t)e;
+    }
+    ++k;
+  };
+  for (int line = lineoffset; line < linecount + lineoffset; ++line) {
+    while (have && cur.first >= ll[line] && cur.first <= ll[line + 1]) {
+      const double span = ll[line + 1] - ll[line];
+      const int64_t sp = (int64_t)(((cur.first - ll[line]) / span) * outlinelen);
+      const int64_t ep = np_round(((cur.second - ll[line]) / span) * outlinelen);
+      const int64_t first_line = line + 1 - lineoffset;
+      if (ep > outlinelen) {
+        const int64_t num = ep / outlinelen;
+        emit(first_line, sp, outlinelen);
+        for (int64_t q = 0; q < num - 1; ++q) emit(first_line + q + 1, 0, outlinelen);
+        emit(first_line + num, 0, ep % outlinelen);
+      } else {
+        emit(first_line, sp, ep);
+      }
+      if (idx < el.size()) cur = el[idx++]; else have = false;
+    }
+  }
+  *count = k;
   return LD_OK;
 }

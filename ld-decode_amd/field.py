@@ -348,6 +348,7 @@ class Field:
         self.sync_confidence = int(loc.sync_confidence)
         self.vblank_next = None if loc.vblank_next < 0 else int(loc.vblank_next)
         self.line0loc = loc.line0loc
+        self.hsync_max = loc.hsync_max
         self.nextfieldoffset = loc.nextfieldoffset
         self.valid = loc.status == _lib.LD_LOC_OK
         if not self.valid:
@@ -355,6 +356,32 @@ class Field:
         self.linelocs1 = ll1
         self.linebad = bad.astype(bool)
         return ll1, self.linebad, loc.nextfieldoffset
+
+    def dropout_detect(self, linecount=None, first_field=None):
+        """Field.dropout_detect (core.py:2978-2996) on this field's buffers
+        (decode['video'] demod / demod_raw / demod_05 and decode['rfhpf'])
+        with its final `linelocs`: returns (lines, starts, ends) as lists.
+        linecount / first_field default to what compute_linelocs found;
+        the hsync length is the one its get_timings measured (hsync_max)."""
+        from . import dropouts as D
+        rf = self.rf
+        torch = rf.torch
+        ll = self.linelocs
+        if not isinstance(ll, torch.Tensor):
+            ll = torch.as_tensor(np.asarray(ll, dtype=np.float64))
+        ll = ll.to(rf.device, torch.float64).contiguous()
+        video = self.data["video"]
+        bufs = {n: video[n] for n in ("demod", "demod_raw", "demod_05")}
+        bufs["rfhpf"] = self.data["rfhpf"]
+        first = self.isFirstField if first_field is None else first_field
+        fd = dict(demod_offset=0, demod_len=bufs["demod"].numel(), lineloc_offset=0, nlines=ll.numel(),
+                  lineoffset=self.lineoffset, linecount=self.linecount if linecount is None else linecount,
+                  first_field=int(bool(first)))
+        if getattr(self, "hsync_max", None) is not None:
+            fd["hsync_max"] = self.hsync_max
+        det = D.DropoutDetector(rf, cap=1 << 16)
+        det.detect(bufs, [fd], ll)
+        return det.lists()[0]
 
     def _as_prev(self, fields_written):
         """The state getLine0 reads of this (located, aligned) field."""

@@ -130,10 +130,11 @@ def ire_raster(system, sysp, nsamples, fs, jitter_samples, rng, start_halfline=0
 
 
 def make_capture(system="NTSC", nsamples=700_000, seed=1, snr_db=40.0, jitter_ns=50.0,
-                 audio=True, fs=FS, start_halfline=0, pilot_ire=None):
+                 audio=True, fs=FS, start_halfline=0, pilot_ire=None, dropouts=()):
     """Generate one capture.  Returns :class:`Capture`.  pilot_ire: peak
     amplitude of the PAL sync-tip pilot (default 10 IRE for PAL, none for
-    NTSC)."""
+    NTSC).  dropouts: (start, length) sample spans where the RF carrier is
+    lost (the signal drops to the noise floor), as on a damaged disc."""
     rng = np.random.default_rng(seed)
     sysp, decp = system_tables(system)
     jitter = jitter_ns * 1e-9 * fs
@@ -167,6 +168,8 @@ def make_capture(system="NTSC", nsamples=700_000, seed=1, snr_db=40.0, jitter_ns
         for fc, fa, dev in carriers:
             sig += 0.1 * np.sin(2 * np.pi * fc * t - (dev / fa) * np.cos(2 * np.pi * fa * t))
         del t
+    for a0, n in dropouts:
+        sig[a0:a0 + n] = 0.0
     if snr_db is not None:
         sigma = np.sqrt(0.5 / 10 ** (snr_db / 10))
         sig += rng.normal(0.0, sigma, nsamples)

@@ -109,6 +109,10 @@ class LocParams:
                    line_period=sp["line_period"], blocklen=getattr(rf, "blocklen", 32768))
 
     @property
+    def freq_hz_half(self):
+        return self.freq * 1000000 / 2
+
+    @property
     def linefreq(self):
         return (self.freq / self.linelen) * self.linelen
 
@@ -474,6 +478,7 @@ class FieldLines:
     skipdetected: bool = False
     new_ire0: Optional[float] = None
     validpulses: list = dc_field(default_factory=list)
+    LT: Optional[dict] = None
 
 
 def get_line0(p, valid, meanlinelen, prev: Optional[PrevField], startloc):
@@ -527,7 +532,7 @@ def compute_linelocs(p, demod_05, prev: Optional[PrevField] = None, startloc=0, 
     proclines = p.proclines
     res = FieldLines(None, None, None, isFirstField=isfirst, linecount=linecount, sync_confidence=conf,
                      vblank_next=lastlineloc, line0loc=line0loc, meanlinelen=meanlinelen, new_ire0=new_ire0,
-                     validpulses=valid)
+                     validpulses=valid, LT=LT)
     if lastlineloc is not None:
         skipdetected = (lastlineloc - line0loc) / p.linelen < (linecount - 5)
     else:
