@@ -73,14 +73,16 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
 
     for _ in range(warmup):
         dec.step()
+        dec.dropouts()
+        dec.dropout_lists()
     torch.cuda.synchronize()
     if packed:
         # the pipelined figure only
         steps_seq = 0
     else:
         steps_seq = steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(7)] for _ in range(max(steps_seq, 1))]
-    walk_s = []
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(8)] for _ in range(max(steps_seq, 1))]
+    walk_s, lists_s = [], []
     t0 = time.perf_counter()
     for i in range(len(ev)):
         e = ev[i]
@@ -101,6 +103,13 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
         dec.measure_facts()
         dec.verify()
         e[6].record(stream)
+        # dropout detection (on by default in the reference's decode): the
+        # GPU error search, then the host grouping of the positions
+        dec.dropouts()
+        e[7].record(stream)
+        t = time.perf_counter()
+        drops = dec.dropout_lists()
+        lists_s.append(time.perf_counter() - t)
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
 
@@ -108,7 +117,10 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
         return float(np.mean([e[a].elapsed_time(e[b]) for e in ev]))
     out = dict(wall_s=wall, step_ms=span(0, 6), demod_ms=span(0, 1), pulses_ms=span(1, 2),
                locate_hsync_ms=span(2, 3), lines_ms=span(3, 4), tbc_ms=span(4, 5), facts_ms=span(5, 6),
-               walk_host_ms=1e3 * float(np.mean(walk_s)), relocated=dec.relocated)
+               walk_host_ms=1e3 * float(np.mean(walk_s)), relocated=dec.relocated, dropouts_ms=span(6, 7),
+               dropout_lists_host_ms=1e3 * float(np.mean(lists_s)),
+               dropout_errors=sum(len(p) for p in dec.dropout_det.positions_per_field()),
+               dropout_entries=sum(len(d[0]) for d in drops))
     pipe_ms = None
     if single is None:
         # consecutive passes overlapped on two streams: stream A demodulates
@@ -401,6 +413,12 @@ def main():
                       "burst_lines": round(batched["lines_ms"], 4), "tbc": round(batched["tbc_ms"], 4),
                       "facts_verify": round(batched["facts_ms"], 4)},
         "host_walk_ms": round(batched["walk_host_ms"], 4),
+        "dropouts": {"gpu_ms": round(batched["dropouts_ms"], 4),
+                     "host_lists_ms": round(batched["dropout_lists_host_ms"], 4),
+                     "error_samples": batched["dropout_errors"], "entries": batched["dropout_entries"],
+                     "note": "Field.dropout_detect for every field of the step, after the decode (not in "
+                             "ms_per_step): GPU time of the error search; host time of the positions' copy "
+                             "and the native grouping into (line, start, end) entries"},
         "relocated_passes": batched["relocated"] + batched.get("relocated_pipe", 0),
         "roofline": {"bound": "hbm", "achieved": round(demod_gbs, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                      "frac": round(demod_gbs / HBM_PEAK_GBS, 4), "traffic": demod_traffic,
