@@ -33,6 +33,7 @@ import __graft_entry__
 
 METRIC = "RF Msamples/s (×realtime @40 MSPS) demod+TBC, batched vs plain, % HBM roofline"
 HBM_PEAK_GBS = 8000.0
+DEMOD_THREADS = 1024      # threads per k_demod_block workgroup (the traffic file is keyed by grid size)
 NTSC_FIELD_SAMPLES = 667_334
 
 
@@ -364,7 +365,7 @@ def main():
     tbc_gbs = batched["tbc_bytes"] / (batched["tbc_ms"] / 1e3) / 1e9
     path_gbs = (batched["demod_bytes"] + batched["tbc_bytes"]) / (batched["pipe_ms"] / 1e3) / 1e9
 
-    demod_traffic = load_traffic("ldgpu::k_demod_block", batched["nblocks"] * 512)
+    demod_traffic = load_traffic("ldgpu::k_demod_block", batched["nblocks"] * DEMOD_THREADS)
     tbc_traffic = load_traffic("ldgpu::k_tbc_pixels", nf * (rf.SysParams["frame_lines"] // 2 + 1) * 4 * 256)
     cpu = None
     if args.cpu_fields > 0:

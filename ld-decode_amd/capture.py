@@ -131,7 +131,7 @@ class CaptureDecoder:
         flags) into the device buffers."""
         torch = self.torch
         rf = self.rf
-        out, off, lo = [], 0, 0
+        out, off, lo = fieldmod.FieldPlan(), 0, 0
         for f in fields:
             fd = dict(f)
             fd.setdefault("lineoffset", self.default_lineoffset)
@@ -356,6 +356,20 @@ def truth_fields(rf, cap, nfields, first_field=0, margin_lines=2, delay=13.0):
     return fields, np.concatenate(lls)
 
 
+_STREAMS = {}
+
+
+def _pipeline_streams(torch, device):
+    """The two default-priority streams every PipelinedDecoder on a device
+    shares.  The runtime spreads streams over a few hardware queues in
+    creation order; streams made later (another decoder's) can share one
+    queue and then run one after the other, so the pair is made once."""
+    key = torch.device(device).index or 0
+    if key not in _STREAMS:
+        _STREAMS[key] = (torch.cuda.Stream(device=device), torch.cuda.Stream(device=device))
+    return _STREAMS[key]
+
+
 class PipelinedDecoder:
     """CaptureDecoder buffer sets on two streams, so that consecutive passes
     over a capture (or consecutive batches) overlap.  Both streams have the
@@ -383,8 +397,7 @@ class PipelinedDecoder:
         nsets = 3 if self.locating else 2
         self.sets = [CaptureDecoder(rf, data, fields, linelocs1, **kw) for _ in range(nsets)]
         self.nblocks = self.sets[0].nblocks
-        self.stream_a = torch.cuda.Stream(device=rf.device)
-        self.stream_b = torch.cuda.Stream(device=rf.device)
+        self.stream_a, self.stream_b = _pipeline_streams(torch, rf.device)
         self.free = [None] * nsets  # event: the set's previous pass is finished (and verified)
         self.k = 0
         self.pending_b = None       # (set, ready event): pass waiting for its walk and stage B
@@ -459,9 +472,15 @@ class PipelinedDecoder:
         j = self.pending_v
         p = self.sets[j]
         with torch.cuda.stream(self.stream_b):
-            p.verify()
-            done = torch.cuda.Event()
-            done.record(self.stream_b)
+            rounds = p.verify()
+            if rounds == 0 and p.facts_done is not None:
+                # nothing was re-run: the set is free once its own stage B
+                # (ending with the facts copy) is; an event recorded now
+                # would also wait for the next pass's stage B, queued since
+                done = p.facts_done
+            else:
+                done = torch.cuda.Event()
+                done.record(self.stream_b)
         self.free[j] = done
         self.pending_v = None
         return p

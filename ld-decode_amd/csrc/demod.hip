@@ -1,6 +1,6 @@
 // RF demodulation kernel (FP32) and the handle / demod part of the C ABI.
 //
-// k_demod_block: one workgroup of 512 threads per block of N = 32768 real
+// k_demod_block: one workgroup of kT threads per block of N = 32768 real
 // samples (M = 16384 complex points) computes the reference's
 // RFDecode.demodblock_cpu (core.py:755-879) end to end:
 //
@@ -24,6 +24,7 @@
 
 #include <mutex>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 #include "fft.hpp"
@@ -49,13 +50,15 @@ int hip_fail(hipError_t e, const char* what) {
 
 constexpr int kN = 32768;      // block length handled by these kernels
 constexpr int kM = kN / 2;     // complex points per transform
-constexpr int kT = 512;        // threads per workgroup for the M-point FFT
-constexpr int kP = kM / kT;    // points per thread (32)
+// Threads per workgroup for the M-point FFT: 1024 (16 points per thread,
+// radix 16 x 16 x 16 x 4, four wavefronts per SIMD to hide LDS and barrier
+// latency) or 512 (32 points, radix 32 x 32 x 16, two per SIMD).
+constexpr int kT = 1024;
+constexpr int kP = kM / kT;    // points per thread
 constexpr int kNA = 1024;      // audio slice bins at N = 32768
 constexpr float kTwoPi = 6.283185307179586f;
 
-using FwdM = WgFft<kM, -1>;
-using InvM = WgFft<kM, +1>;
+using FwdM = std::conditional_t<kT == 512, WgFft<kM, -1>, WgFft16kT1024<-1>>;
 using InvA = WgFft<kNA, +1, kSyncWave>;
 
 struct Lds {
@@ -63,37 +66,84 @@ struct Lds {
   float im[LdsSize<kM>::floats];
 };
 
-// Twiddles of the M-point transform and of the real-transform split, kept in
-// LDS for the whole kernel (copied once from the global table):
-//   p2[k][i] = exp(-2 pi i 2^i k / 1024), k < 32, i < 5   (second pass)
-//   a[h][i]  = exp(-2 pi i 2^i 32 h / 16384), h < 32, i < 4 (third pass,
-//   b[l][i]  = exp(-2 pi i 2^i l / 16384),    l < 32       k = 32 h + l)
-//   split[t] = exp(-2 pi i t / N), t < T: W_N^(t + T m) = split[t] * exp(-2 pi i m / 64)
-struct TwLds {
+// Pass twiddles of the M-point transform, kept in LDS for the whole kernel
+// (copied once from the global table).  Row k of a pass after NS points holds
+// exp(-2 pi i 2^i k / (NS R)), i < log2 R; long rows are products of two
+// short tables (k = F h + l).
+template <int T>
+struct TwLdsT;
+template <>
+struct TwLdsT<512> {   // passes (R 32, NS 32), (R 16, NS 1024 = 32 h + l)
   float2 p2[32 * 5];
   float2 a[32 * 4];
   float2 b[32 * 4];
-  float2 split[kT];
+  static constexpr int count = 32 * 5 + 32 * 4 + 32 * 4;
+  template <int NS>
+  __device__ __forceinline__ float2 get(int k, int i) const {
+    if constexpr (NS == 32) {
+      return p2[k * 5 + i];
+    } else {
+      static_assert(NS == 1024, "M-point transform passes");
+      return cmul(a[(k >> 5) * 4 + i], b[(k & 31) * 4 + i]);
+    }
+  }
+  static void fill(float2* t, float2 (*cexp)(double, double)) {
+    for (int k = 0; k < 32; ++k)
+      for (int i = 0; i < 5; ++i) *t++ = cexp((double)(1 << i) * k, 1024);
+    for (int h = 0; h < 32; ++h)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 32 * h, 16384);
+    for (int l = 0; l < 32; ++l)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * l, 16384);
+  }
 };
-constexpr int kTwLdsCount = 32 * 5 + 32 * 4 + 32 * 4 + kT;
+template <>
+struct TwLdsT<1024> {  // passes (R 16, NS 16), (R 16, NS 256 = 16 h + l), (R 4, NS 4096 = 64 h + l)
+  float2 p16[16 * 4];
+  float2 a256[16 * 4];
+  float2 b256[16 * 4];
+  float2 a4k[64 * 2];
+  float2 b4k[64 * 2];
+  static constexpr int count = 16 * 4 * 3 + 64 * 2 * 2;
+  template <int NS>
+  __device__ __forceinline__ float2 get(int k, int i) const {
+    if constexpr (NS == 16) {
+      return p16[k * 4 + i];
+    } else if constexpr (NS == 256) {
+      return cmul(a256[(k >> 4) * 4 + i], b256[(k & 15) * 4 + i]);
+    } else {
+      static_assert(NS == 4096, "M-point transform passes");
+      return cmul(a4k[(k >> 6) * 2 + i], b4k[(k & 63) * 2 + i]);
+    }
+  }
+  static void fill(float2* t, float2 (*cexp)(double, double)) {
+    for (int k = 0; k < 16; ++k)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * k, 256);
+    for (int h = 0; h < 16; ++h)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 16 * h, 4096);
+    for (int l = 0; l < 16; ++l)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * l, 4096);
+    for (int h = 0; h < 64; ++h)
+      for (int i = 0; i < 2; ++i) *t++ = cexp((double)(1 << i) * 64 * h, 16384);
+    for (int l = 0; l < 64; ++l)
+      for (int i = 0; i < 2; ++i) *t++ = cexp((double)(1 << i) * l, 16384);
+  }
+};
+using TwLds = TwLdsT<kT>;
+constexpr int kTwLdsCount = TwLds::count;
 
 struct TwSrcLds {
   const TwLds* t;
   template <int NS>
   __device__ __forceinline__ float2 get(int k, int i) const {
-    if constexpr (NS == 32) {
-      return t->p2[k * 5 + i];
-    } else {
-      static_assert(NS == 1024, "M-point transform passes");
-      return cmul(t->a[(k >> 5) * 4 + i], t->b[(k & 31) * 4 + i]);
-    }
+    return t->template get<NS>(k, i);
   }
 };
 
-// W_N^(tid + T m) from the thread's split[tid] and a compile-time constant
+// W_N^(tid + T m) from the thread's split[tid] = exp(-2 pi i tid / N) and a
+// compile-time constant exp(-2 pi i m T / N)
 template <int M_>
 __device__ __forceinline__ float2 split_tw(float2 wt) {
-  return cmul(wt, make_float2(Tw<64, M_, -1>::re, Tw<64, M_, -1>::im));
+  return cmul(wt, make_float2(Tw<kN / kT, M_, -1>::re, Tw<kN / kT, M_, -1>::im));
 }
 
 // Everything one fused launch needs.  Filters are half spectra (bins 0..M)
@@ -113,6 +163,7 @@ struct DemodArgs {
   uint32_t in_total;       // packed input: bytes of the whole input buffer
   const float2* tw_rows;   // Stockham pass twiddles of the audio transform (TwLayout)
   const float2* tw_lds;    // TwLds image, copied to LDS by every workgroup
+  const float2* tw_split;  // split[t] = exp(-2 pi i t / N), t < kT
   const float2* h_rfv;
   C2rOut hpf;              // out == nullptr: skip
   C2rOut post[4];
@@ -382,7 +433,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
   const uint32_t vo = tid * 8u;
   {  // twiddle tables into LDS
     const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
-    float2* dst = &twl.p2[0];
+    float2* dst = reinterpret_cast<float2*>(&twl);
     for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
     if (tid < 2 + a.npost) {
       const float2* t = tid == 0 ? a.hpf.h : (tid == 1 ? a.h_rfv : a.post[tid - 2].h);
@@ -391,7 +442,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
   }
   __syncthreads();
   const TwSrcLds twf{&twl};
-  const float2 wt0 = twl.split[tid];
+  const float2 wt0 = bload2(make_rsrc(a.tw_split, kT * 8u), vo, 0u);
   // v: the transform's data; between phases it carries the next phase's
   // filter (bins tid + T m), loaded after the transform and before that
   // phase's output stores (see stage_own)
@@ -632,10 +683,10 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   h->device = device;
   h->blocklen = blocklen;
   // [0, TwLayout::total): Stockham pass rows (TwLayout), then the TwLds
-  // image (second-pass rows, third-pass factor tables, split twiddles)
+  // image (pass rows and factor tables), then the split twiddles
   const int M = 0;
-  std::vector<float2> tw(TwLayout::total + kTwLdsCount);
-  auto cexp = [](double num, double den) {
+  std::vector<float2> tw(TwLayout::total + kTwLdsCount + kT);
+  float2 (*cexp)(double, double) = [](double num, double den) {
     double a = -2.0 * M_PI * num / den;
     return make_float2((float)cos(a), (float)sin(a));
   };
@@ -649,16 +700,8 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   rows(TwLayout::k16k_p3, 16, 1024);
   rows(TwLayout::k1k_p2, 16, 16);
   rows(TwLayout::k1k_p3, 4, 256);
-  {
-    float2* t = tw.data() + TwLayout::total;
-    for (int k = 0; k < 32; ++k)
-      for (int i = 0; i < 5; ++i) *t++ = cexp((double)(1 << i) * k, 1024);
-    for (int hh = 0; hh < 32; ++hh)
-      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 32 * hh, 16384);
-    for (int l = 0; l < 32; ++l)
-      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * l, 16384);
-    for (int t2 = 0; t2 < kT; ++t2) *t++ = cexp(t2, blocklen);
-  }
+  TwLds::fill(tw.data() + TwLayout::total, cexp);
+  for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, blocklen);
   hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
@@ -825,6 +868,7 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.step = step;
   a.tw_rows = h->twiddle;
   a.tw_lds = h->twiddle + TwLayout::total;
+  a.tw_split = h->twiddle + TwLayout::total + kTwLdsCount;
   a.h_rfv = h->filt[LD_FILT_RFVIDEO];
   a.hpf.h = h->filt[LD_FILT_RFHPF];
   a.hpf.out = out->rfhpf;

@@ -341,6 +341,20 @@ struct WgFft<16384, DIR, SYNC> {
   }
 };
 
+// 16384 points, 1024 threads (16 per thread): radix 16 x 16 x 16 x 4
+template <int DIR, int SYNC = kSyncBlock>
+struct WgFft16kT1024 {
+  static constexpr int T = 1024;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<16384, 1024, 16, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<16384, 1024, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<16384, 1024, 16, 256, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<16384, 1024, 4, 4096, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
+  }
+};
+
 // 1024 points, 64 threads (one wavefront): radix 16 x 16 x 4
 template <int DIR, int SYNC>
 struct WgFft<1024, DIR, SYNC> {

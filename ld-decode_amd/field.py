@@ -88,12 +88,46 @@ def tbc_fields(rf, demod, fields, linelocs, coord_f32=True, smoothing=0.0, thres
     return out_f32, out_u16
 
 
+class FieldPlan(list):
+    """A pass's field dicts (a list) that builds each C struct array of
+    them once: the batched calls of one pass share it."""
+
+    def struct_array(self, ctype, build):
+        cache = self.__dict__.setdefault("_arrays", {})
+        if ctype not in cache:
+            cache[ctype] = build(self)
+        return cache[ctype]
+
+
+_DTYPES = {}
+
+
+def _struct_array(ctype, fields, keys, default=None):
+    """(ctype * n) over a numpy structured array of the same layout, filled
+    column by column (numpy reads the struct layout from ctypes)."""
+    dt = _DTYPES.get(ctype)
+    if dt is None:
+        dt = _DTYPES[ctype] = np.dtype(ctype)
+    n = len(fields)
+    a = np.zeros(max(n, 1), dt)
+    for k in keys:
+        a[k][:n] = [f[k] if default is None else f.get(k, default) for f in fields]
+    return (ctype * n).from_buffer(a)
+
+
+def _build_tbc(fields):
+    return _struct_array(_lib.TbcField, fields, [k for k, _ in _lib.TbcField._fields_], default=0)
+
+
+def _build_lines(fields):
+    return _struct_array(_lib.LineField, fields,
+                         ("demod_offset", "demod_len", "lineloc_offset", "nlines", "linecount", "first_field"))
+
+
 def _field_array(fields):
-    arr = (_lib.TbcField * len(fields))()
-    for i, f in enumerate(fields):
-        for k, _ in _lib.TbcField._fields_:
-            setattr(arr[i], k, f.get(k, 0))
-    return arr
+    if isinstance(fields, FieldPlan):
+        return fields.struct_array(_lib.TbcField, _build_tbc)
+    return _build_tbc(fields)
 
 
 def refine_linelocs_hsync_fields(rf, demod_05, fields, linelocs1, linebad=None, stream=None, out=None,
@@ -170,11 +204,9 @@ def line_params(rf):
 
 
 def _line_fields(fields):
-    arr = (_lib.LineField * len(fields))()
-    for i, f in enumerate(fields):
-        for k in ("demod_offset", "demod_len", "lineloc_offset", "nlines", "linecount", "first_field"):
-            setattr(arr[i], k, int(f[k]))
-    return arr
+    if isinstance(fields, FieldPlan):
+        return fields.struct_array(_lib.LineField, _build_lines)
+    return _build_lines(fields)
 
 
 def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust=0.0, stream=None, out=None,
