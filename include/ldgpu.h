@@ -88,6 +88,12 @@ int ld_set_filter(ld_handle* h, int filter_id, const double* host_reim, int nbin
 int ld_set_audio(ld_handle* h, int channel, const double* host_reim, int nbins, int lowbin,
                  double a1_freq, double low_freq);
 
+/* Audio stage-2 channel (0 = left, 1 = right): the filter audio2_filter
+ * (20 kHz low-pass times de-emphasis, core.py:727-734) as its half spectrum,
+ * bins 0..blocklen/2, interleaved float64 pairs, and the channel's carrier
+ * frequency (audio_lfreq / audio_rfreq) removed before it and added after. */
+int ld_set_audio2(ld_handle* h, int channel, const double* host_reim, int nbins, double center_freq);
+
 /* Scalar demodulation parameters: sample rate (Hz), the clip applied to
  * demod_raw before the second FFT (core.py:809), the constant subtracted from
  * the clipped signal before that FFT (float32 range protection; added back
@@ -532,6 +538,73 @@ int ld_dropout_detect(ld_handle* h, const float* rfhpf, const float* demod, cons
 int ld_dropout_list(const int64_t* errmap, int64_t n, const double* linelocs, int nlines, int lineoffset,
                     int linecount, int outlinelen, double freq, int32_t* lines, int32_t* starts, int32_t* ends,
                     int cap, int* count);
+
+/* ---------------------------------------------------------------------
+ * Analogue audio stage 2 and its output rate (RFDecode.audio_phase2,
+ * core.py:884-950; downscale_audio, core.py:1423-1578).
+ * ------------------------------------------------------------------- */
+
+/* One field read: its stage-1 audio (entries [audio_offset, + audio_len)
+ * of the stitched stage-1 arrays), where its stage-2 output goes
+ * (a2_offset, audio_len entries), its final
+ * line locations (entries [lineloc_offset, + nlines) of the DEVICE array)
+ * and linecount, and for the output: the first tick's time (timeoffset,
+ * seconds; 0 for a line-locked rate), nout output sample pairs written from
+ * pair out_offset on. */
+typedef struct ld_audio_field {
+  int64_t audio_offset;
+  int64_t audio_len;
+  int64_t a2_offset;
+  int64_t lineloc_offset;
+  int64_t out_offset;
+  double timeoffset;
+  int32_t nlines;
+  int32_t linecount;
+  int32_t nout;
+  int32_t reserved;
+} ld_audio_field;
+
+/* downscale_audio's constants: SysParams line_period (us), rf.linelen, the
+ * output rate as Hz (already resolved for a line-locked rate), the stage-1
+ * decimation audio_fdiv, and the two carriers. */
+typedef struct ld_audio_out_params {
+  double line_period;
+  double linelen;
+  double freq;
+  double scale;
+  double lfreq, rfreq;
+} ld_audio_out_params;
+
+size_t ld_audio2_workspace_bytes(const ld_handle* h, const ld_audio_field* fields, int nfields);
+
+/* audio_phase2 for every field read of a batch (core.py:884-950): per
+ * blocklen segment and channel the stage-1 audio minus its carrier, zeroed
+ * within 8 samples of the left channel's clip peaks in that segment
+ * (findpeaks over 500 kHz, utils.py:1207), filtered by the stage-2 filter
+ * (FP32 transforms), plus the carrier.  A read of at most blocklen samples
+ * is one zero-padded segment (float64 result, as the reference returns it);
+ * a longer read is assembled from the reference's overlapping segments
+ * (at 0, every blocklen - 512 samples, and at len - blocklen - 1, placed
+ * one sample late as the reference places it), each output sample taken
+ * from the last segment that writes it and rounded to float32 as the
+ * reference's array of that case holds it.  Written as float64 to
+ * out_left / out_right (DEVICE, out_total entries each).  `fields` is a
+ * HOST array. */
+int ld_audio2_fields(ld_handle* h, const float* audio_left, const float* audio_right, int64_t total,
+                     const ld_audio_field* fields, int nfields, double* out_left, double* out_right,
+                     int64_t out_total, void* workspace, size_t workspace_bytes, void* stream);
+
+/* _downscale_audio_compute_locs_and_swow + _downscale_audio_to_output
+ * (core.py:1423-1535) for every field: nout interleaved int16 pairs per
+ * field into out (DEVICE, out_pairs pairs), failed[f] (DEVICE) set when a
+ * tick had no audio under it (the samples stay 0, as in the reference).
+ * a2_left / a2_right: the stage-2 output; linelocs: DEVICE.  Each field's
+ * nout must equal its tick count minus one (numba's arange length) and be
+ * at most 4095. */
+int ld_audio_downscale(ld_handle* h, const double* a2_left, const double* a2_right, int64_t a2_total,
+                       const ld_audio_field* fields, int nfields, const double* linelocs, int64_t nlinelocs,
+                       const ld_audio_out_params* params, int16_t* out, int64_t out_pairs, int32_t* failed,
+                       void* workspace, size_t workspace_bytes, void* stream);
 
 /* skip_check's score (core.py:2139-2155) from a field's eight line medians. */
 int ld_skip_score(const ld_locate_params* p, const double* medians8);

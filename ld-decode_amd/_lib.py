@@ -26,6 +26,7 @@ EXPORTS = (
     "ld_field_facts_workspace_bytes", "ld_field_facts_measure", "ld_locate_field", "ld_locate_fields",
     "ld_skip_score", "ld_locate_recheck", "ld_unpack_10bit",
     "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
+    "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -129,6 +130,17 @@ class DropoutParams(ctypes.Structure):
         ("video_rot", ctypes.c_int32), ("pal", ctypes.c_int32)]
 
 
+class AudioField(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_int64) for n in ("audio_offset", "audio_len", "a2_offset", "lineloc_offset",
+                                              "out_offset")] + [
+        ("timeoffset", ctypes.c_double)] + [(n, ctypes.c_int32) for n in ("nlines", "linecount", "nout",
+                                                                          "reserved")]
+
+
+class AudioOutParams(ctypes.Structure):
+    _fields_ = [(n, ctypes.c_double) for n in ("line_period", "linelen", "freq", "scale", "lfreq", "rfreq")]
+
+
 _lib = None
 _lock = threading.Lock()
 
@@ -176,6 +188,12 @@ def _declare(lib):
         "ld_unpack_10bit": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
         "ld_locate_recheck": (c_int, [P(LocateParams), P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int, c_int,
                                       P(PrevField), P(FieldFacts), P(c_int)]),
+        "ld_set_audio2": (c_int, [c_vp, c_int, P(c_dbl), c_int, c_dbl]),
+        "ld_audio2_workspace_bytes": (c_sz, [c_vp, P(AudioField), c_int]),
+        "ld_audio2_fields": (c_int, [c_vp, c_vp, c_vp, c_i64, P(AudioField), c_int, c_vp, c_vp, c_i64, c_vp, c_sz,
+                                     c_vp]),
+        "ld_audio_downscale": (c_int, [c_vp, c_vp, c_vp, c_i64, P(AudioField), c_int, c_vp, c_i64,
+                                       P(AudioOutParams), c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
         "ld_dropout_workspace_bytes": (c_sz, [c_vp, P(DropoutField), c_int]),
         "ld_dropout_detect": (c_int, [c_vp, c_vp, c_vp, c_vp, c_vp, c_i64, P(DropoutField), c_int, c_vp, c_i64,
                                       P(DropoutParams), c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),

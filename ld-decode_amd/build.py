@@ -14,7 +14,7 @@ import subprocess
 HERE = pathlib.Path(__file__).resolve().parent
 CSRC = HERE / "csrc"
 LIB = HERE / "libldgpu.so"
-SOURCES = ["demod.hip", "tbc.hip", "lines.hip", "pulses.hip", "input.hip", "dropouts.hip", "locate.cpp"]
+SOURCES = ["demod.hip", "tbc.hip", "lines.hip", "pulses.hip", "input.hip", "dropouts.hip", "audio.hip", "locate.cpp"]
 ARCH = os.environ.get("LDGPU_ARCH", "gfx950")
 
 

@@ -13,6 +13,7 @@ import ctypes
 import numpy as np
 
 from . import _lib
+from . import audio as audiomod
 from . import dropouts as dropoutmod
 from . import field as fieldmod
 from . import locate as locatemod
@@ -110,6 +111,7 @@ class CaptureDecoder:
         self.smoothing = smoothing
         self.fields = []
         self.dropout_det = None
+        self.audio_out = None
         if self.locating:
             fsz = ctypes.sizeof(_lib.FieldFacts)
             self.walker = locatemod.FieldWalker(rf, self.total, self.max_fields)
@@ -280,6 +282,24 @@ class CaptureDecoder:
             self.tbc()
             self.measure_facts()
         return rounds
+
+    def audio_output(self, freq=44100, timeoffsets=None, stream=None):
+        """Enqueue analogue audio stage 2 (RFDecode.audio_phase2,
+        core.py:884-950) for every field read of the pass and its int16
+        output at `freq` (downscale_audio, core.py:1538-1578) from the
+        fields' final line locations; audio_results() returns it.
+        timeoffsets: each field's first tick offset (audio.time_offset)."""
+        if not self.audio:
+            raise ValueError("this decoder was built without analogue audio")
+        if self.audio_out is None:
+            self.audio_out = audiomod.AudioOutput(self.rf, self.max_fields)
+        self.audio_out.stage2(self.buffers["audio_left"], self.buffers["audio_right"], self.fields, stream)
+        n = sum(f["nlines"] for f in self.fields)
+        self.audio_out.downscale(self.fields, self.linelocs[:max(n, 1)], freq, timeoffsets, stream)
+
+    def audio_results(self):
+        """Per field: (int16 pairs, next time offset, failed)."""
+        return self.audio_out.results() if self.audio_out is not None else []
 
     def dropouts(self, stream=None):
         """Enqueue dropout detection (Field.dropout_detect, core.py:2978-2996)

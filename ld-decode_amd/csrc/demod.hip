@@ -22,6 +22,7 @@
 #include <stdlib.h>
 #include <string.h>
 
+#include <algorithm>
 #include <mutex>
 #include <string>
 #include <type_traits>
@@ -657,6 +658,118 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
 __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false>(a); }
 __global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true>(a); }
 
+// ------------------------------------------------------- audio stage 2
+// One blocklen segment of a field read's stage-1 audio: y = the filtered
+// segment (index 0 = its first sample); output sample out_base + i takes
+// y[i] for i in [r0, r1) and [r2, r3) (the samples no later segment writes).
+struct Audio2Seg {
+  int64_t in_off;    // stitched stage-1 index of the segment's first sample
+  int64_t out_base;  // stage-2 index that y[0] lands on
+  int32_t len;       // samples read (the rest of the block is zero)
+  int32_t f32;       // round to float32 (the reference's long-read array)
+  int32_t r0, r1, r2, r3;
+};
+
+struct Audio2Args {
+  const float* in[2];           // stitched stage-1 audio, left / right
+  const Audio2Seg* segs;
+  const float2* h[2];           // stage-2 half spectra (bins 0..M), scale 1/M
+  double center[2];
+  double* out[2];
+  const float2* tw_lds;
+  const float2* tw_split;
+};
+
+constexpr float kClipLow = 500000.0f;  // findpeaks threshold, core.py:894
+
+// runfilter_audio_phase2 (core.py:884-915) for one (segment, channel):
+// raw = stage-1 audio - carrier (float32); the left channel's findpeaks
+// clips in the segment (utils.py:1207-1217: t[i] > t[last] and t[i+1] > t[i]
+// on the copy t with values under 500 kHz zeroed; clip l = i - 1) zero
+// raw[l-8, l+8) in both channels; the zero-padded block is filtered over N
+// points as the demodulator's c2r outputs are (one forward M-point
+// transform, the real split, the filter product, the conjugated inverse)
+// and the carrier is added back in float64.
+__global__ __launch_bounds__(kT) void k_audio2_block(Audio2Args a) {
+  __shared__ Lds s;
+  __shared__ TwLds twl;
+  const int tid = threadIdx.x;
+  const int g = blockIdx.x >> 1, c = blockIdx.x & 1;
+  const Audio2Seg sg = a.segs[g];
+  const int len = sg.len;
+  const uint32_t vo = tid * 8u;
+  {
+    const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
+    float2* dst = reinterpret_cast<float2*>(&twl);
+    for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
+  }
+  const rsrc_t lin = make_rsrc(a.in[0] + sg.in_off, (uint32_t)len * 4u);
+  const float cl = (float)a.center[0];
+  // clip candidates: flag[i] for i < len - 1 (bytes over the exchange area)
+  uint8_t* flag = reinterpret_cast<uint8_t*>(&s);
+  float tlast = bload1(lin, (uint32_t)(len - 1) * 4u, 0u) - cl;
+  tlast = tlast < kClipLow ? 0.0f : tlast;
+  int any = 0;
+  for (int i = tid; i < kN; i += kT) {
+    uint8_t fl = 0;
+    if (i + 1 < len) {
+      float t0 = bload1(lin, (uint32_t)i * 4u, 0u) - cl, t1 = bload1(lin, (uint32_t)(i + 1) * 4u, 0u) - cl;
+      t0 = t0 < kClipLow ? 0.0f : t0;
+      t1 = t1 < kClipLow ? 0.0f : t1;
+      fl = (t0 > tlast) && (t1 > t0);
+    }
+    flag[i] = fl;
+    any |= fl;
+  }
+  any = __syncthreads_or(any);
+  // raw samples of this channel as pairs (2n, 2n + 1), n = tid + T m
+  const rsrc_t in = make_rsrc(a.in[c] + sg.in_off, (uint32_t)len * 4u);
+  const float cc = (float)a.center[c];
+  float2 v[kP], x[kP];
+#pragma unroll
+  for (int m = 0; m < kP; ++m) {
+    const int j = 2 * (tid + kT * m);
+    float r0 = j < len ? bload1(in, (uint32_t)j * 4u, 0u) - cc : 0.0f;
+    float r1 = j + 1 < len ? bload1(in, (uint32_t)(j + 1) * 4u, 0u) - cc : 0.0f;
+    if (any) {
+      // sample q is zeroed when a flagged i lies in [q - 6, q + 9]
+      bool z0 = false, z1 = false;
+      for (int i = j - 6; i <= j + 10; ++i) {
+        if (i < 0 || i >= len - 1 || !flag[i]) continue;
+        z0 |= i <= j + 9;
+        z1 |= i >= j - 5;
+      }
+      if (z0) r0 = 0.0f;
+      if (z1) r1 = 0.0f;
+    }
+    v[m] = make_float2(r0, r1);
+  }
+  __syncthreads();   // the flags share LDS with the transform
+  const TwSrcLds twf{&twl};
+  const float2 wt = bload2(make_rsrc(a.tw_split, kT * 8u), vo, 0u);
+  float xM = 0.0f;
+  FwdM::run(v, s.re, s.im, twf, tid);
+  r2c_split(v, x, xM, s, wt, tid);
+  load_half(a.h[c], v, tid);
+  const float2 hM = bload2(make_rsrc(a.h[c], kHalfBytes), 0u, kM * 8u);
+  c2r_pre(x, xM, hM, v, s, wt, tid);
+  FwdM::run(v, s.re, s.im, twf, tid);
+  double* out = a.out[c] + sg.out_base;
+  const double center = a.center[c];
+  auto put = [&](int i, float y) {
+    if ((i >= sg.r0 && i < sg.r1) || (i >= sg.r2 && i < sg.r3)) {
+      const double o = (double)y + center;
+      out[i] = sg.f32 ? (double)(float)o : o;
+    }
+  };
+#pragma unroll
+  for (int m = 0; m < kP; ++m) {
+    const int j = 2 * (tid + kT * m);
+    put(j, v[m].x);
+    put(j + 1, -v[m].y);
+  }
+}
+
 }  // namespace ldgpu
 
 using namespace ldgpu;
@@ -720,6 +833,7 @@ void ld_destroy(ld_handle* h) {
   hipFree(h->sinc_lut);
   for (auto& f : h->filt) hipFree(f);
   for (auto& f : h->audio_filt) hipFree(f);
+  for (auto& f : h->audio2_filt) hipFree(f);
   delete h;
 }
 
@@ -750,6 +864,131 @@ int ld_set_filter(ld_handle* h, int id, const double* reim, int nbins, double sc
   if (rc) return rc;
   h->filt_dc[id] = dc_gain;
   h->filt_shift[id] = shift;
+  return LD_OK;
+}
+
+int ld_set_audio2(ld_handle* h, int ch, const double* reim, int nbins, double center_freq) {
+  if (!h || !reim || ch < 0 || ch > 1) {
+    set_error("ld_set_audio2: bad handle, table or channel %d", ch);
+    return LD_EINVAL;
+  }
+  if (nbins != h->blocklen / 2 + 1) {
+    set_error("ld_set_audio2: nbins %d != blocklen/2+1 = %d", nbins, h->blocklen / 2 + 1);
+    return LD_EINVAL;
+  }
+  int rc = upload_complex(&h->audio2_filt[ch], reim, nbins, 1.0 / (h->blocklen / 2), h->device);
+  if (rc) return rc;
+  h->audio2_center[ch] = center_freq;
+  return LD_OK;
+}
+
+// The segments of one field read as audio_phase2 (core.py:917-950) cuts
+// it: one zero-padded block up to blocklen samples; longer reads: blocks at
+// 0 and every blocklen - 512 samples while below len - (blocklen - 512)
+// (written from their sample 512 on), and the block at len - blocklen - 1
+// written one sample late; each output sample belongs to the last block
+// that writes it.
+static void audio2_segments(const ld_audio_field& fd, std::vector<Audio2Seg>& out) {
+  const int64_t n = fd.audio_len, N = kN, askip = 512, sjump = kN - 512;
+  if (n <= N) {
+    out.push_back(Audio2Seg{fd.audio_offset, fd.a2_offset, (int32_t)n, 0, 0, (int32_t)n, 0, 0});
+    return;
+  }
+  struct W { int64_t in, base, len, w0, w1; };   // writes output [w0, w1) = y[w0 - base ..]
+  std::vector<W> ws;
+  ws.push_back({0, 0, N, 0, N});
+  for (int64_t smp = sjump; smp < n - sjump; smp += sjump) {
+    const int64_t m = std::min(N, n - smp);
+    ws.push_back({smp, smp, m, smp + askip, smp + m});
+  }
+  ws.push_back({n - N - 1, n - N, N, n - (N - askip), n});
+  // ownership: subtract the union of later writes
+  std::vector<std::pair<int64_t, int64_t>> later;
+  std::vector<Audio2Seg> segs;
+  for (int j = (int)ws.size() - 1; j >= 0; --j) {
+    std::vector<std::pair<int64_t, int64_t>> runs{{ws[j].w0, ws[j].w1}};
+    for (auto& L : later) {
+      std::vector<std::pair<int64_t, int64_t>> next;
+      for (auto& r : runs) {
+        if (L.second <= r.first || L.first >= r.second) {
+          next.push_back(r);
+          continue;
+        }
+        if (r.first < L.first) next.push_back({r.first, L.first});
+        if (L.second < r.second) next.push_back({L.second, r.second});
+      }
+      runs.swap(next);
+    }
+    later.push_back({ws[j].w0, ws[j].w1});
+    if (runs.empty()) continue;
+    Audio2Seg g{fd.audio_offset + ws[j].in, fd.a2_offset + ws[j].base, (int32_t)ws[j].len, 1, 0, 0, 0, 0};
+    g.r0 = (int32_t)(runs[0].first - ws[j].base);
+    g.r1 = (int32_t)(runs[0].second - ws[j].base);
+    if (runs.size() > 1) {
+      g.r2 = (int32_t)(runs[1].first - ws[j].base);
+      g.r3 = (int32_t)(runs[1].second - ws[j].base);
+    }
+    segs.push_back(g);
+  }
+  out.insert(out.end(), segs.rbegin(), segs.rend());
+}
+
+size_t ld_audio2_workspace_bytes(const ld_handle*, const ld_audio_field* fields, int nfields) {
+  if (!fields || nfields <= 0) return 0;
+  size_t nseg = 0;
+  for (int f = 0; f < nfields; ++f) {
+    const int64_t n = fields[f].audio_len;
+    nseg += n <= kN ? 1 : (size_t)((n + kN - 513) / (kN - 512) + 2);
+  }
+  return (sizeof(Audio2Seg) * nseg + 255) & ~(size_t)255;
+}
+
+int ld_audio2_fields(ld_handle* h, const float* left, const float* right, int64_t total,
+                     const ld_audio_field* fields, int nfields, double* out_left, double* out_right,
+                     int64_t out_total, void* workspace, size_t ws_bytes, void* stream) {
+  if (!h || !left || !right || !fields || nfields <= 0 || !out_left || !out_right || !workspace) {
+    set_error("ld_audio2_fields: bad arguments");
+    return LD_EINVAL;
+  }
+  if (!h->audio2_filt[0] || !h->audio2_filt[1]) {
+    set_error("ld_audio2_fields: stage-2 filters not set (ld_set_audio2)");
+    return LD_ESTATE;
+  }
+  std::vector<Audio2Seg> segs;
+  for (int f = 0; f < nfields; ++f) {
+    const ld_audio_field& fd = fields[f];
+    if (fd.audio_len < 2 || fd.audio_len > (int64_t)1 << 30) {
+      set_error("ld_audio2_fields: field %d: %lld audio samples", f, (long long)fd.audio_len);
+      return LD_EINVAL;
+    }
+    if (fd.audio_offset < 0 || fd.audio_offset + fd.audio_len > total || fd.a2_offset < 0 ||
+        fd.a2_offset + fd.audio_len > out_total) {
+      set_error("ld_audio2_fields: field %d outside the arrays", f);
+      return LD_ERANGE;
+    }
+    audio2_segments(fd, segs);
+  }
+  if (ws_bytes < sizeof(Audio2Seg) * segs.size()) {
+    set_error("ld_audio2_fields: workspace %zu bytes < %zu needed", ws_bytes, sizeof(Audio2Seg) * segs.size());
+    return LD_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  LD_HIP(hipMemcpyAsync(workspace, segs.data(), sizeof(Audio2Seg) * segs.size(), hipMemcpyHostToDevice, st));
+  Audio2Args a{};
+  a.in[0] = left;
+  a.in[1] = right;
+  a.segs = (const Audio2Seg*)workspace;
+  a.h[0] = h->audio2_filt[0];
+  a.h[1] = h->audio2_filt[1];
+  a.center[0] = h->audio2_center[0];
+  a.center[1] = h->audio2_center[1];
+  a.out[0] = out_left;
+  a.out[1] = out_right;
+  a.tw_lds = h->twiddle + TwLayout::total;
+  a.tw_split = h->twiddle + TwLayout::total + kTwLdsCount;
+  hipLaunchKernelGGL(k_audio2_block, dim3(2 * (unsigned)segs.size()), dim3(kT), 0, st, a);
+  LD_HIP(hipGetLastError());
   return LD_OK;
 }
 

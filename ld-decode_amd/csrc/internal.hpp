@@ -19,6 +19,8 @@ struct ld_handle {
   double filt_dc[LD_NFILT] = {};
   int filt_shift[LD_NFILT] = {};
   float2* audio_filt[2] = {};
+  float2* audio2_filt[2] = {};   // stage-2 half spectra, scaled for the c2r transform
+  double audio2_center[2] = {};
   int audio_nbins = 0;
   int audio_lowbin[2] = {};
   double audio_rate[2] = {};

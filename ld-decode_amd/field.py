@@ -445,12 +445,54 @@ class Field:
         pv.fields_written = fields_written
         return pv
 
+    def downscale_audio(self, freq=44100, lineinfo=None, lastfieldwritten=None, timeoffset=None):
+        """downscale_audio (core.py:1538-1578) of this field: interleaved
+        int16 pairs at `freq` (negative: a multiple of the line rate).
+        decode['audio'] holds the field read's stage-1 audio (float32, as
+        RFDecode.demod_blocks returns it; stage 2 runs first) or its stage-2
+        audio (float64).  The first tick's offset follows Field.downscale
+        (core.py:2644-2670) from lastfieldwritten = (field number, readloc)
+        and this field's readloc, unless timeoffset is given.  Sets
+        audio_next_offset; returns the int16 array."""
+        from . import audio as A
+        rf = self.rf
+        torch = rf.torch
+        au = self.data["audio"]
+        left, right = au["audio_left"], au["audio_right"]
+        out = A.AudioOutput(rf, 1)
+        n = min(left.numel(), right.numel())
+        fd = dict(demod_offset=0, demod_len=n * out.fdiv)
+        if left.dtype == torch.float64:
+            out.a2 = (left.contiguous(), right.contiguous())
+            fd.update(a2_offset=0, audio_len=n)
+        else:
+            out.stage2(left.contiguous(), right.contiguous(), [fd])
+        ll = self.linelocs if lineinfo is None else lineinfo
+        if not isinstance(ll, torch.Tensor):
+            ll = torch.as_tensor(np.asarray(ll, dtype=np.float64))
+        ll = ll.to(rf.device, torch.float64).contiguous()
+        if timeoffset is None:
+            timeoffset = 0.0
+            if lastfieldwritten and freq >= 16000:
+                per_field = rf.freq_hz / rf.SysParams["FPS"] / 2
+                gap = (getattr(self, "readloc", 0) - lastfieldwritten[1]) / per_field
+                number = int(np.round(lastfieldwritten[0] + gap))
+                timeoffset = A.time_offset(rf.SysParams, number, bool(getattr(self, "isFirstField", True)), freq)
+        fd.update(lineloc_offset=0, nlines=ll.numel(), linecount=int(self.linecount))
+        out.downscale([fd], ll, freq, [timeoffset])
+        pcm, nxt, failed = out.results()[0]
+        self.audio_next_offset = nxt
+        self.audio_failed = failed
+        return pcm
+
     def downscale(self, lineinfo=None, linesout=None, outwidth=None, channel="demod", audio=0,
                   final=False, lastfieldwritten=None):
-        """core.py:2626-2722 (video): returns (dsout, None, None); dsout is the
-        float32 Hz field, or the uint16 codes when final=True."""
-        if audio:
-            raise NotImplementedError("analogue audio resampling is not part of the GPU path")
+        """core.py:2626-2722: returns (dsout, dsaudio, None); dsout is the
+        float32 Hz field, or the uint16 codes when final=True; dsaudio the
+        int16 audio pairs at rate `audio` (None when audio is 0)."""
+        dsaudio = None
+        if audio and self.rf.decode_analog_audio:
+            dsaudio = self.downscale_audio(audio, lineinfo, lastfieldwritten)
         torch = self.rf.torch
         ll = self.linelocs if lineinfo is None else lineinfo
         if not isinstance(ll, torch.Tensor):
@@ -467,4 +509,4 @@ class Field:
         dsout = u16 if final else f32
         if final:
             self.dspicture = dsout
-        return dsout, None, None
+        return dsout, dsaudio, None

@@ -54,6 +54,26 @@ def audio_bins(center, min_bw, freq_hz, nbins_in):
     return cbin - nbins // 4, nbins, bw * nbins
 
 
+def audio2_filter(a1_freq, n):
+    """Stage-2 audio filter at the stage-1 rate a1_freq (core.py:727-734): the
+    lowest-order Butterworth low-pass with at most 1 dB ripple to 20 kHz and
+    9 dB attenuation from 24 kHz, times the 75 us / 5.3 us de-emphasis; whole
+    circle at n bins."""
+    order, wn = sps.buttord(20000 / (a1_freq / 2), 24000 / (a1_freq / 2), 1, 9)
+    return freq_response(*sps.butter(order, wn), n) * freq_response(*emphasis(5.3e-6, 75e-6, a1_freq), n)
+
+
+def audio2_half(h):
+    """The half spectrum (bins 0..n/2) that a real inverse transform applies
+    for np.fft.ifft(X * h).real with X a real signal's spectrum: the
+    Hermitian part (h[k] + conj(h[n-k])) / 2.  The stage-2 low-pass in b/a
+    form is conjugate-symmetric only to ~1e-4, so this is not just the first
+    half of h."""
+    n = len(h)
+    k = np.arange(n // 2 + 1)
+    return 0.5 * (h[k] + np.conj(h[(-k) % n]))
+
+
 def gd_equaliser(system, lpf, fs, lpf_freq):
     """Unit-magnitude all-pass whose group delay is the IEC 60856/60857 target
     curve minus the video LPF's own group delay (core.py:480-542)."""
@@ -156,7 +176,9 @@ class FilterBank:
                 sl = np.concatenate([resp[lowbin:lowbin + h], resp[n - lowbin - h:n - lowbin]])
                 self.audio[ch] = {"lowbin": lowbin, "nbins": nb, "a1_freq": rate,
                                   "low_freq": fs * (lowbin / n),
-                                  "filt1": sl * analytic_mask(nb)}
+                                  "filt1": sl * analytic_mask(nb),
+                                  "audio2_filter": audio2_filter(rate, n),
+                                  "center": sysp[key]}
 
     def half(self, name):
         """Bins 0..blocklen/2 of a filter."""

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from .field import upload_sinc_lut
-from .filters import FilterBank
+from .filters import FilterBank, audio2_half
 from .hostdsp import calczc, genwave
 from .params import BLOCKSIZE, system_tables
 
@@ -193,6 +193,9 @@ class RFDecode:
                 ptr = t.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
                 _lib.check(self.lib.ld_set_audio(self.handle, ci, ptr, len(t), a["lowbin"], a["a1_freq"],
                                                  a["low_freq"]))
+                h2 = np.ascontiguousarray(audio2_half(a["audio2_filter"]), dtype=np.complex128)
+                p2 = h2.view(np.float64).ctypes.data_as(ctypes.POINTER(ctypes.c_double))
+                _lib.check(self.lib.ld_set_audio2(self.handle, ci, p2, len(h2), float(a["center"])))
             self.Filters["audio_fdiv"] = self.blocklen // bank.audio["left"]["nbins"]
         rot = self.delays["video_rot"] if self.delays else 0
         self._set_params(rot)
@@ -351,6 +354,20 @@ class RFDecode:
             rv["audio"] = Channels((n, res[n]) for n in ("audio_left", "audio_right"))
         rv["setupcount"] = self.setupcount
         return rv
+
+    def audio_phase2(self, field_audio):
+        """RFDecode.audio_phase2 (core.py:917-950) for one field read's
+        stage-1 audio (Channels or a dict of float32 device tensors, the
+        stitched audio of whole blocks): clip removal and the stage-2 filter
+        per channel, returned as Channels of float64 tensors (a read longer
+        than a block holds float32-rounded values, as the reference's array
+        does in that case)."""
+        from . import audio as A
+        left, right = field_audio["audio_left"], field_audio["audio_right"]
+        n = min(left.numel(), right.numel())
+        out = A.AudioOutput(self, 1)
+        out.stage2(left.contiguous(), right.contiguous(), [dict(demod_offset=0, demod_len=n * out.fdiv)])
+        return Channels((("audio_left", out.a2[0][:n]), ("audio_right", out.a2[1][:n])))
 
     def demodblock(self, data=None, mtf_level=0, fftdata=None, cut=False, _raw_level=False):
         """RFDecode.demodblock (core.py:747-879) on one block.
