@@ -69,7 +69,7 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
         data = synth.pack_10bit(data)
     x = torch.from_numpy(np.ascontiguousarray(data)).to(rf.device)
     audio = True
-    dec = capture.CaptureDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start)
+    dec = capture.CaptureDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start, audio_rate=44100)
     stream = torch.cuda.current_stream(rf.device)
 
     for _ in range(warmup):
@@ -82,7 +82,7 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
         steps_seq = 0
     else:
         steps_seq = steps
-    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(8)] for _ in range(max(steps_seq, 1))]
+    ev = [[torch.cuda.Event(enable_timing=True) for _ in range(9)] for _ in range(max(steps_seq, 1))]
     walk_s, lists_s = [], []
     t0 = time.perf_counter()
     for i in range(len(ev)):
@@ -101,6 +101,8 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
         e[4].record(stream)
         dec.tbc()
         e[5].record(stream)
+        dec.audio_stage()
+        e[8].record(stream)
         dec.measure_facts()
         dec.verify()
         e[6].record(stream)
@@ -117,7 +119,8 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
     def span(a, b):
         return float(np.mean([e[a].elapsed_time(e[b]) for e in ev]))
     out = dict(wall_s=wall, step_ms=span(0, 6), demod_ms=span(0, 1), pulses_ms=span(1, 2),
-               locate_hsync_ms=span(2, 3), lines_ms=span(3, 4), tbc_ms=span(4, 5), facts_ms=span(5, 6),
+               locate_hsync_ms=span(2, 3), lines_ms=span(3, 4), tbc_ms=span(4, 5), audio_ms=span(5, 8),
+               facts_ms=span(8, 6),
                walk_host_ms=1e3 * float(np.mean(walk_s)), relocated=dec.relocated, dropouts_ms=span(6, 7),
                dropout_lists_host_ms=1e3 * float(np.mean(lists_s)),
                dropout_errors=sum(len(p) for p in dec.dropout_det.positions_per_field()),
@@ -127,7 +130,8 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
         # consecutive passes overlapped on two streams: stream A demodulates
         # pass k and finds its pulses while the host walks pass k-1 and
         # stream B aligns and resamples it; three buffer sets
-        pipe = capture.PipelinedDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start)
+        pipe = capture.PipelinedDecoder(rf, x, audio=audio, locate=True, max_fields=nfields, start=start,
+                                        audio_rate=44100)
         for _ in range(warmup):
             pipe.step()
         pipe.join()
@@ -160,12 +164,15 @@ def cpu_baseline(cap, walk, nsample, workers, min_seconds, max_seconds):
     (one per task) is located (compute_linelocs on its buffer), its lines are
     refined (hsync edge, colour-burst alignment, bad-line repair, subcarrier
     shift) and it is resampled to uint16 (the resampler loop is the oracle's
-    compiled C restatement, as the reference runs it compiled with numba).
+    compiled C restatement, as the reference runs it compiled with numba);
+    its audio goes through stage 2 (numpy FFTs) and to 44.1 kHz int16 (the
+    oracle's numpy form of the numba-compiled output loop).
     Passes over this fixed sample repeat until `min_seconds` have elapsed
     (stopping early past `max_seconds`).  Returns (Msamples/s, seconds,
     passes, blocks per pass)."""
     from concurrent.futures import ThreadPoolExecutor
 
+    from oracle import audio as OA
     from oracle import ctbc
     from oracle import dsp as O
     from oracle import lines as L
@@ -188,12 +195,16 @@ def cpu_baseline(cap, walk, nsample, workers, min_seconds, max_seconds):
     data = cap.data[b0 * S: b1 * S + N - S].astype(np.float64)
     blocks = list(range(b1 - b0))
 
+    afilt = {ch: OA.audio2_filter(orf, ch) for ch in ("left", "right")}
+    fdiv = orf.Filters["audio_fdiv"]
+
     def demod(b):
         r = orf.demodblock(data=data[b * S:b * S + N], cut=True)
-        return r["video"]["demod"], r["video"]["demod_05"], r["video"]["demod_burst"]
+        return (r["video"]["demod"], r["video"]["demod_05"], r["video"]["demod_burst"], r["audio"]["audio_left"],
+                r["audio"]["audio_right"])
 
     def field(args):
-        off, n, demod_all, d05_all, burst_all = args
+        off, n, demod_all, d05_all, burst_all, al, ar = args
         res = LL.compute_linelocs(pp, d05_all[off:off + n], None, off, 1)
         bad = L.compute_deriv_error(res.linelocs1, res.linebad)
         ll2, bad = T.refine_linelocs_hsync(d05_all[off:], res.linelocs1, bad, "NTSC", 40.0, orf.iretohz, vs)
@@ -202,7 +213,10 @@ def cpu_baseline(cap, walk, nsample, workers, min_seconds, max_seconds):
                                           res.isFirstField, 0.0)
         locs, wow = T.computewow_scaled(ll4, orf.linelen, W, lines, 0)
         ds = ctbc.scale_field(demod_all[off:], locs, wow, lut, 0, W, lines * W)
-        return T.hz_to_output_array(ds.astype(np.float32), dp["ire0"], dp["hz_ire"], sp["outputZero"], vs, osc)
+        seg = {"audio_left": al[off // fdiv:(off + n) // fdiv], "audio_right": ar[off // fdiv:(off + n) // fdiv]}
+        a2 = OA.audio_phase2(orf, seg, afilt)
+        pcm = OA.downscale_audio_np(a2, ll4, sp, orf.linelen, fdiv, res.linecount, 0.0, 44100)
+        return T.hz_to_output_array(ds.astype(np.float32), dp["ire0"], dp["hz_ire"], sp["outputZero"], vs, osc), pcm
 
     passes = 0
     t0 = time.perf_counter()
@@ -212,7 +226,9 @@ def cpu_baseline(cap, walk, nsample, workers, min_seconds, max_seconds):
             demod_all = np.concatenate([o[0] for o in outs]).astype(np.float32)
             d05_all = np.concatenate([o[1] for o in outs]).astype(np.float32)
             burst_all = np.concatenate([o[2] for o in outs]).astype(np.float32)
-            list(ex.map(field, [(org - b0 * S, n, demod_all, d05_all, burst_all) for org, n in plan]))
+            al = np.concatenate([o[3] for o in outs]).astype(np.float32)
+            ar = np.concatenate([o[4] for o in outs]).astype(np.float32)
+            list(ex.map(field, [(org - b0 * S, n, demod_all, d05_all, burst_all, al, ar) for org, n in plan]))
             passes += 1
             dt = time.perf_counter() - t0
             if dt >= min_seconds or dt >= max_seconds:
@@ -375,7 +391,8 @@ def main():
         cpu = {"value": round(rate, 3), "unit": "Msamples/s", "cores": workers, "kind": "port",
                "sample": f"FP64 numpy/scipy oracle on a {workers}-thread pool, the same step on the first "
                          f"{args.cpu_fields} located fields of the batch ({nblk} blocks demodulated; per field "
-                         f"line location, hsync refine, burst alignment, TBC to uint16), {passes} passes in "
+                         f"line location, hsync refine, burst alignment, TBC to uint16, audio stage 2 and "
+                         f"44.1 kHz output), {passes} passes in "
                          f"{secs:.1f} s wall"}
 
     line = {
@@ -395,7 +412,8 @@ def main():
             "workload": f"C2 full NTSC field decode, batched: {nf} fields per step, consecutive steps overlapped "
                         f"({batched['usable']} usable samples, {batched['nblocks']} blocks of 32768), "
                         "FP32 demod + line location (GPU sync pulses, host field walk) + FP64 hsync / burst "
-                        "line positions + sinc-LUT TBC to 910x263 uint16",
+                        "line positions + sinc-LUT TBC to 910x263 uint16 + analogue audio stage 2 and 44.1 kHz "
+                        "int16 output",
             "system": "NTSC",
             "blocklen": 32768,
             "fields_per_step": nf,
@@ -412,6 +430,7 @@ def main():
         "stages_ms": {"demod": round(batched["demod_ms"], 4), "pulses": round(batched["pulses_ms"], 4),
                       "locate_walk_and_hsync": round(batched["locate_hsync_ms"], 4),
                       "burst_lines": round(batched["lines_ms"], 4), "tbc": round(batched["tbc_ms"], 4),
+                      "audio_stage2_44k1": round(batched["audio_ms"], 4),
                       "facts_verify": round(batched["facts_ms"], 4)},
         "host_walk_ms": round(batched["walk_host_ms"], 4),
         "dropouts": {"gpu_ms": round(batched["dropouts_ms"], 4),

@@ -54,7 +54,8 @@ class CaptureDecoder:
     """
 
     def __init__(self, rf, data, fields=None, linelocs1=None, lineoffset=0, coord_f32=True, smoothing=0.0,
-                 audio=False, lines=True, prev_phase_adjust=0.0, locate=False, max_fields=64, start=0.0):
+                 audio=False, lines=True, prev_phase_adjust=0.0, locate=False, max_fields=64, start=0.0,
+                 audio_rate=None, first_field_number=0):
         torch = rf.torch
         self.rf = rf
         self.torch = torch
@@ -112,6 +113,11 @@ class CaptureDecoder:
         self.fields = []
         self.dropout_det = None
         self.audio_out = None
+        # audio_rate (e.g. 44100): every pass also runs audio stage 2 and the
+        # output-rate audio after the TBC; the fields of a pass are numbered
+        # from first_field_number for the first tick's offset
+        self.audio_rate = audio_rate if audio else None
+        self.first_field_number = int(first_field_number)
         if self.locating:
             fsz = ctypes.sizeof(_lib.FieldFacts)
             self.walker = locatemod.FieldWalker(rf, self.total, self.max_fields)
@@ -280,6 +286,7 @@ class CaptureDecoder:
             self.refine()
             self.lines()
             self.tbc()
+            self.audio_stage()
             self.measure_facts()
         return rounds
 
@@ -296,6 +303,15 @@ class CaptureDecoder:
         self.audio_out.stage2(self.buffers["audio_left"], self.buffers["audio_right"], self.fields, stream)
         n = sum(f["nlines"] for f in self.fields)
         self.audio_out.downscale(self.fields, self.linelocs[:max(n, 1)], freq, timeoffsets, stream)
+
+    def audio_stage(self, stream=None):
+        """audio_output at audio_rate for the pass's fields (when set)."""
+        if self.audio_rate is None or not self.fields:
+            return
+        sp = self.rf.SysParams
+        offs = [audiomod.time_offset(sp, self.first_field_number + k, bool(f["first_field"]), self.audio_rate)
+                for k, f in enumerate(self.fields)]
+        self.audio_output(self.audio_rate, offs, stream)
 
     def audio_results(self):
         """Per field: (int16 pairs, next time offset, failed)."""
@@ -344,6 +360,7 @@ class CaptureDecoder:
         self.refine()
         self.lines()
         self.tbc()
+        self.audio_stage()
         if self.locating:
             self.measure_facts()
             self.verify()
@@ -442,6 +459,7 @@ class PipelinedDecoder:
             self.stream_b.wait_event(ready)
             d.lines()
             d.tbc()
+            d.audio_stage()
             done = torch.cuda.Event()
             done.record(self.stream_b)
         self.free[i] = done
@@ -481,6 +499,7 @@ class PipelinedDecoder:
             p.refine()
             p.lines()
             p.tbc()
+            p.audio_stage()
             p.measure_facts()
         self.pending_b = None
         return j

@@ -1,7 +1,8 @@
 // Analogue audio output rate (downscale_audio, core.py:1423-1578), batched
 // over fields: one workgroup per field computes the tick positions and the
 // wow factors of _downscale_audio_compute_locs_and_swow (the 1.5% wow clamp
-// is a chain over the ticks and runs on one thread from registers), then
+// is a chain over the ticks: one thread runs it, and only when some step of
+// the computed wow exceeds 1.5%), then
 // every tick's output pair as _downscale_audio_to_output does: the in-order
 // float64 mean of the stage-2 audio under it, times the wow, minus the
 // carrier, rescaled to int16 with round-half-even and clipped, negated.
@@ -13,8 +14,9 @@
 
 namespace ldgpu {
 
-constexpr int kDsThreads = 256;
+constexpr int kDsThreads = 1024;
 constexpr int kDsMaxTicks = 4096;   // ticks per field (nout + 1)
+constexpr int kDsBatch = 8;         // stage-2 samples loaded together per mean
 
 struct DsArgs {
   const double* a2[2];
@@ -31,6 +33,21 @@ struct DsArgs {
 __device__ __forceinline__ int dsa_rescale(double x) {
   const double v = rint(x * 32767.0 / 371081.0);
   return v < -32766.0 ? -32766 : (v > 32766.0 ? 32766 : (int)v);
+}
+
+// In-order float64 sum of x[s..e) (numba's np.mean before the division),
+// kDsBatch loads in flight at a time.
+__device__ __forceinline__ double ordered_sum(const double* __restrict__ x, long long s, long long e) {
+  double acc = 0.0;
+  for (long long k = s; k < e; k += kDsBatch) {
+    double v[kDsBatch];
+#pragma unroll
+    for (int q = 0; q < kDsBatch; ++q) v[q] = k + q < e ? x[k + q] : 0.0;
+#pragma unroll
+    for (int q = 0; q < kDsBatch; ++q)
+      if (k + q < e) acc += v[q];
+  }
+  return acc;
 }
 
 __global__ __launch_bounds__(kDsThreads) void k_audio_downscale(DsArgs a) {
@@ -69,8 +86,13 @@ __global__ __launch_bounds__(kDsThreads) void k_audio_downscale(DsArgs a) {
     locs[i] = loc / a.p.scale;
   }
   __syncthreads();
-  // the wow clamp: swow[i] = swow[i-1] when they differ by more than 1.5%
-  if (threadIdx.x == 0) {
+  // the wow clamp: swow[i] = swow[i-1] when they differ by more than 1.5%.
+  // Without a jump that large between the computed values it never fires
+  // (each previous value is then its own computed value): the chain runs
+  // only when some jump exceeds 1.5%.
+  int jump = 0;
+  for (int i = 1 + threadIdx.x; i < nt; i += kDsThreads) jump |= fabs(swow[i] - swow[i - 1]) > 0.015;
+  if (__syncthreads_or(jump) && threadIdx.x == 0) {
     double prev = swow[0];
     for (int i0 = 1; i0 < nt; i0 += 16) {
       double r[16];
@@ -98,14 +120,9 @@ __global__ __launch_bounds__(kDsThreads) void k_audio_downscale(DsArgs a) {
     const long long s = (long long)locs[i], e = (long long)locs[i + 1];
     int16_t ol = 0, orr = 0;
     if (e > s && e < len && s >= 0) {
-      double sl = 0.0, sr = 0.0;
-      for (long long k = s; k < e; ++k) {
-        sl += L[k];
-        sr += R[k];
-      }
       const double n = (double)(e - s);
-      const double vl = ((sl / n) * swow[i]) - a.p.lfreq;
-      const double vr = ((sr / n) * swow[i]) - a.p.rfreq;
+      const double vl = ((ordered_sum(L, s, e) / n) * swow[i]) - a.p.lfreq;
+      const double vr = ((ordered_sum(R, s, e) / n) * swow[i]) - a.p.rfreq;
       ol = (int16_t)(-dsa_rescale(vl));
       orr = (int16_t)(-dsa_rescale(vr));
     } else {
@@ -114,8 +131,7 @@ __global__ __launch_bounds__(kDsThreads) void k_audio_downscale(DsArgs a) {
     out[2 * i] = ol;
     out[2 * i + 1] = orr;
   }
-  if (mybad) bad = 1;    // every writer stores the same value
-  __syncthreads();
+  if (__syncthreads_or(mybad) && threadIdx.x == 0) bad = 1;
   if (threadIdx.x == 0) a.failed[f] = bad;
 }
 

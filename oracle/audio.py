@@ -174,6 +174,51 @@ def downscale_audio(audio, lineinfo, sysp, linelen, scale, linecount, timeoffset
     return out, arange[-1] - frametime, failed
 
 
+def downscale_audio_np(audio, lineinfo, sysp, linelen, scale, linecount, timeoffset=0, freq=44100):
+    """downscale_audio with numpy array operations (the means from running
+    sums, so in a different order): the CPU baseline's
+    timing form of the numba-compiled reference, equal to downscale_audio
+    except where a mean lies within rounding of an int16 step."""
+    lp = sysp["line_period"]
+    if freq < 0:
+        timeoffset = 0
+        freq = (1000000 / lp) * -freq
+    frametime = linecount / (1000000 / lp)
+    soundgap = 1 / freq
+    arange = nb_arange(timeoffset, frametime + (soundgap / 2), soundgap)
+    li = np.asarray(lineinfo, dtype=np.float64)
+    linenum = ((arange * 1000000) / lp) + 1
+    il = linenum.astype(np.int64)
+    inside = (linenum >= 0) & (len(li) > linenum + 2)
+    cur = np.where(inside, li[np.clip(il, 0, len(li) - 2)], li[-2])
+    nxt = np.where(inside, li[np.clip(il + 1, 0, len(li) - 1)], li[-2] + linelen)
+    neg = linenum < 0
+    cur = np.where(neg, np.trunc(li[0] + linelen * linenum), cur)
+    nxt = np.where(neg, cur + linelen, nxt)
+    locs = (cur + (nxt - cur) * (linenum - np.floor(linenum))) / scale
+    w = ((nxt - cur) / linelen - 1) + 1
+    for i in range(1, len(w)):
+        if abs(w[i] - w[i - 1]) > 0.015:
+            w[i] = w[i - 1]
+    s = locs[:-1].astype(np.int64)
+    e = locs[1:].astype(np.int64)
+    left, right = np.asarray(audio["audio_left"]), np.asarray(audio["audio_right"])
+    ok = (e > s) & (e < len(left)) & (s >= 0)
+    out = np.zeros(2 * (len(arange) - 1), dtype=np.int16)
+    if ok.any():
+        cnt = (e - s)[ok]
+        cl = np.concatenate([[0.0], np.cumsum(left, dtype=np.float64)])
+        cr = np.concatenate([[0.0], np.cumsum(right, dtype=np.float64)])
+        ml = cl[e[ok]] - cl[s[ok]]
+        mr = cr[e[ok]] - cr[s[ok]]
+        vl = (ml / cnt) * w[:-1][ok] - sysp["audio_lfreq"]
+        vr = (mr / cnt) * w[:-1][ok] - sysp["audio_rfreq"]
+        sc = lambda v: -np.clip(np.round(v * 32767.0 / 371081.0), -32766, 32766).astype(np.int16)
+        idx = np.flatnonzero(ok)
+        out[2 * idx], out[2 * idx + 1] = sc(vl), sc(vr)
+    return out, arange[-1] - frametime, not ok.all()
+
+
 def audio_offset(sysp, freq_hz, readloc, is_first, freq, lastfieldwritten):
     """Field.downscale's audio time offset (core.py:2644-2670) for output
     rate `freq` given lastfieldwritten = (field number, readloc)."""

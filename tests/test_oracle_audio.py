@@ -150,6 +150,24 @@ def test_overlap_save_branch_assembly():
     assert out["audio_left"].dtype == np.float32
 
 
+def test_timing_form_of_the_output():
+    """downscale_audio_np (the CPU baseline's numpy form) gives the loop
+    form's pairs, next offset and failure flag on a wobbly field."""
+    orf = _orf()
+    sp = orf.SysParams
+    rng = np.random.default_rng(0)
+    n = 29 * 991
+    aud = {"audio_left": sp["audio_lfreq"] + rng.normal(size=n) * 30000,
+           "audio_right": sp["audio_rfreq"] + rng.normal(size=n) * 30000}
+    ll = 1000 + np.cumsum(rng.normal(2541, 0.5, size=273))
+    ll[120] += 80                      # a jump that the 1.5% wow clamp holds
+    for t0 in (-1e-5, 0.0, 2e-5):
+        a = OA.downscale_audio(aud, ll, sp, orf.linelen, 32, 263, t0, 44100)
+        b = OA.downscale_audio_np(aud, ll, sp, orf.linelen, 32, 263, t0, 44100)
+        np.testing.assert_array_equal(a[0], b[0])
+        assert a[1] == b[1] and a[2] == b[2]
+
+
 def test_rescale_mean_and_offsets():
     assert OA.dsa_rescale_and_clip(0.0) == 0
     assert OA.dsa_rescale_and_clip(1e9) == 32766 and OA.dsa_rescale_and_clip(-1e9) == -32766
