@@ -48,7 +48,8 @@ enum {
   LD_FILT_VIDEO05 = 3,  /* FVideo05 (LPF * deemph * firwin65)      core.py:648-652 */
   LD_FILT_BURST = 4,    /* FVideoBurst (LPF * deemph * firwin81)   core.py:654-658 */
   LD_FILT_PILOT = 5,    /* FVideoPilot, PAL only                   core.py:660-669 */
-  LD_NFILT = 6
+  LD_FILT_EFM = 6,      /* Fefm (Hermitian part), digital audio    core.py:420-465 */
+  LD_NFILT = 7
 };
 
 /* Input sample formats: int16 (10-bit codes << 6, utils.py:388), float32,
@@ -124,6 +125,9 @@ typedef struct ld_demod_outputs {
   float* audio_right;
   int64_t video_stride;
   int64_t audio_stride;
+  /* EFM front-end (core.py:847-851): int16(clip(irFFT(X * Fefm))), cut and
+   * strided like the video outputs; needs LD_FILT_EFM; NULL: skip */
+  int16_t* efm;
 } ld_demod_outputs;
 
 /* Demodulate `nblocks` blocks; block b reads input[first + b*step .. + blocklen).

@@ -12,7 +12,7 @@ import threading
 LIB_PATH = pathlib.Path(__file__).resolve().with_name("libldgpu.so")
 
 LD_OK, LD_EINVAL, LD_ERANGE, LD_EHIP, LD_ESTATE = 0, -1, -2, -3, -4
-LD_FILT_RFHPF, LD_FILT_RFVIDEO, LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT = range(6)
+LD_FILT_RFHPF, LD_FILT_RFVIDEO, LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT, LD_FILT_EFM = range(7)
 LD_IN_INT16, LD_IN_FLOAT32, LD_IN_PACKED10 = 0, 1, 2
 ABI_VERSION = 1
 
@@ -46,7 +46,8 @@ class LdGpuError(RuntimeError):
 class DemodOutputs(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "rfhpf", "demod", "demod_raw", "demod_05", "demod_burst", "demod_pilot",
-        "audio_left", "audio_right")] + [("video_stride", ctypes.c_int64), ("audio_stride", ctypes.c_int64)]
+        "audio_left", "audio_right")] + [("video_stride", ctypes.c_int64), ("audio_stride", ctypes.c_int64),
+                                         ("efm", ctypes.c_void_p)]
 
 
 class TbcField(ctypes.Structure):

@@ -68,6 +68,8 @@ class CaptureDecoder:
         dev = rf.device
         names = list(rf.video_fields) + ["rfhpf"]
         self.buffers = {n: torch.empty(self.total, dtype=torch.float32, device=dev) for n in names}
+        if rf.decode_digital_audio:
+            self.buffers["efm"] = torch.empty(self.total, dtype=torch.int16, device=dev)
         self.audio = bool(audio)
         if self.audio:
             na = rf.filterbank.audio["left"]["nbins"]

@@ -181,6 +181,8 @@ struct DemodArgs {
   int64_t astride;
   float fs, clip_lo, clip_hi, dc;
   int notch_lo, notch_hi;  // spur notch window, empty: off
+  const float2* h_efm;     // EFM equaliser (Hermitian half spectrum, c2r scale)
+  int16_t* efm;            // EFM output (cut and strided as the video outputs)
 };
 
 constexpr int kNotchMax = 508;  // bins in the notch window
@@ -412,11 +414,38 @@ __device__ __forceinline__ void load_half(const float2* table, float2 (&h)[kP], 
 //   2, 3     X * RFVideo -> analytic even / odd (inverse)
 //   4        inst. frequency, clip -> D         (rFFT)
 //   5..8     D * post filter -> video outputs   (c2r)
-enum { kPhIn = 0, kPhHpf = 1, kPhEven = 2, kPhOdd = 3, kPhDemod = 4, kPhPost = 5 };
+enum { kPhIn = 0, kPhHpf = 1, kPhEven = 2, kPhOdd = 3, kPhDemod = 4, kPhPost = 5, kPhEfm = -1 };
 
-// kPacked: the input is the packed 10-bit format (a separate instantiation,
-// so that the int16 / float32 kernel's register allocation is not touched).
-template <bool kPacked>
+// The EFM output (core.py:847-851) from a transform staged as c2r_store
+// reads it: np.int16(np.clip(y, -32768, 32767)) (truncation toward zero)
+// for i < len; pairs on even offsets go out as one 32-bit store.
+__device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int tid, Lds& s) {
+  const rsrc_t r = make_rsrc(outp, (uint32_t)len * 2u);
+  const LdsPair own(s.re, s.im, pidx(tid));
+  auto q = [](float y) -> uint32_t {
+    return (uint32_t)(uint16_t)(int16_t)(int)fminf(fmaxf(y, -32768.0f), 32767.0f);
+  };
+  const bool pairs = ((cut | len) & 1) == 0;
+  static_for<0, kP>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const float2 v = own.template ld<POff<kT * m>::value>();
+    const int n = 2 * (tid + kT * m);
+    const uint32_t i0 = (uint32_t)((n - cut) & (kN - 1));
+    if (pairs) {
+      __builtin_amdgcn_raw_buffer_store_b32(q(v.x) | (q(-v.y) << 16), r, (int)(i0 * 2u), 0, 0);
+    } else {
+      const uint32_t i1 = (uint32_t)((n + 1 - cut) & (kN - 1));
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(v.x), r, (int)(i0 * 2u), 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(-v.y), r, (int)(i1 * 2u), 0, 0);
+    }
+  });
+}
+
+// kPacked: the input is the packed 10-bit format; kEfm: the EFM output is
+// produced too, as one more c2r phase of the input spectrum between the RF
+// high-pass and the analytic signal (separate instantiations, so that the
+// plain kernel's register allocation is not touched).
+template <bool kPacked, bool kEfm>
 __device__ __forceinline__ void demod_block(const DemodArgs& a) {
   __shared__ Lds s;
   __shared__ float2 aslice[2][kNA / 2 + 1];
@@ -425,8 +454,8 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
   __shared__ int notch_flag[kNotchMax + 4];
   __shared__ double notch_red[128];
   __shared__ uint32_t notch_mask[kT];  // bit m: bin tid + T m zeroed by the spur notch
-  // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters
-  __shared__ float2 bin_m[2 + 4];
+  // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters, 6 EFM
+  __shared__ float2 bin_m[2 + 4 + 1];
   const int tid = threadIdx.x;
   const int b = blockIdx.x;
   const int64_t s0 = a.first + a.step * b;
@@ -439,6 +468,9 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
     if (tid < 2 + a.npost) {
       const float2* t = tid == 0 ? a.hpf.h : (tid == 1 ? a.h_rfv : a.post[tid - 2].h);
       bin_m[tid] = t ? bload2(make_rsrc(t, kHalfBytes), 0u, kM * 8u) : make_float2(0.0f, 0.0f);
+    }
+    if constexpr (kEfm) {
+      if (tid == kT - 1) bin_m[6] = bload2(make_rsrc(a.h_efm, kHalfBytes), 0u, kM * 8u);
     }
   }
   __syncthreads();
@@ -484,7 +516,9 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
     __syncthreads();    // the first transform reuses the exchange area
   }
 
-  for (int ph = 0; ph < last; ++ph) {
+  for (int it = 0; it < last + (kEfm ? 1 : 0); ++it) {
+    int ph = it;
+    if constexpr (kEfm) ph = it == kPhEven ? kPhEfm : (it > kPhEven ? it - 1 : it);
     // the split twiddles derived from wt are loop-invariant; keep the
     // compiler from hoisting (and spilling) all 32 of them out of the loop
     float2 wt = wt0;
@@ -518,6 +552,9 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
       // RF high-pass branch (core.py:769-772)
       if (!has_hpf) continue;
       c2r_pre(x, xM, bin_m[0], v, s, wt, tidv);
+    } else if (kEfm && ph == kPhEfm) {
+      // EFM equaliser on the input spectrum (core.py:847)
+      c2r_pre(x, xM, bin_m[6], v, s, wt, tidv);
     } else if (ph == kPhEven || ph == kPhOdd) {
       // analytic signal (core.py:800-805), conjugated: even samples from
       // Z[k] (+ Z[M] folded into bin 0), odd samples from Z[k] W_N^-k
@@ -623,7 +660,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
       // the next phase's filter: rfhpf's or RFVideo's after the input
       // transform, the first post filter's after the demod transform
       if (ph == kPhIn)
-        load_half(has_hpf ? a.hpf.h : a.h_rfv, v, tidv);
+        load_half(has_hpf ? a.hpf.h : (kEfm ? a.h_efm : a.h_rfv), v, tidv);
       else if (a.npost > 0)
         load_half(a.post[0].h, v, tidv);
     } else if (ph == kPhEven) {
@@ -636,8 +673,12 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
       });
     } else if (ph == kPhHpf) {
       stage_own(v, s, tidv);
-      load_half(a.h_rfv, v, tidv);
+      load_half(kEfm ? a.h_efm : a.h_rfv, v, tidv);
       c2r_store(a.hpf, (int64_t)b * a.vstride, tidv, s);
+    } else if (kEfm && ph == kPhEfm) {
+      stage_own(v, s, tidv);
+      load_half(a.h_rfv, v, tidv);
+      efm_store(a.efm + (int64_t)b * a.vstride, a.vcut, a.vlen, tidv, s);
     } else if (ph >= kPhPost) {
       stage_own(v, s, tidv);
       if (ph + 1 < last) load_half(a.post[ph + 1 - kPhPost].h, v, tidv);
@@ -655,8 +696,10 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
   }
 }
 
-__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false>(a); }
-__global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true>(a); }
+__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false, false>(a); }
+__global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true, false>(a); }
+__global__ __launch_bounds__(kT) void k_demod_block_efm(DemodArgs a) { demod_block<false, true>(a); }
+__global__ __launch_bounds__(kT) void k_demod_block_packed_efm(DemodArgs a) { demod_block<true, true>(a); }
 
 // ------------------------------------------------------- audio stage 2
 // One blocklen segment of a field read's stage-1 audio: y = the filtered
@@ -1162,12 +1205,18 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
     // covers that round-up)
     a.in_total = (uint32_t)((bytes + 3) / 4 * 4);
   }
+  a.efm = out->efm;
+  a.h_efm = h->filt[LD_FILT_EFM];
+  if (a.efm && !a.h_efm) {
+    set_error("ld_demod_blocks: EFM output requested without the EFM filter (LD_FILT_EFM)");
+    return LD_ESTATE;
+  }
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
   if (dtype == LD_IN_PACKED10)
-    hipLaunchKernelGGL(k_demod_block_packed, dim3(nblocks), dim3(kT), 0, st, a);
+    hipLaunchKernelGGL(a.efm ? k_demod_block_packed_efm : k_demod_block_packed, dim3(nblocks), dim3(kT), 0, st, a);
   else
-    hipLaunchKernelGGL(k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
+    hipLaunchKernelGGL(a.efm ? k_demod_block_efm : k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

@@ -487,9 +487,11 @@ class Field:
 
     def downscale(self, lineinfo=None, linesout=None, outwidth=None, channel="demod", audio=0,
                   final=False, lastfieldwritten=None):
-        """core.py:2626-2722: returns (dsout, dsaudio, None); dsout is the
+        """core.py:2626-2722: returns (dsout, dsaudio, efmout); dsout is the
         float32 Hz field, or the uint16 codes when final=True; dsaudio the
-        int16 audio pairs at rate `audio` (None when audio is 0)."""
+        int16 audio pairs at rate `audio` (None when audio is 0); efmout the
+        EFM samples from line 1 to line linecount + 1 when the RFDecode
+        decodes digital audio (decode['efm'], as demod_blocks returns it)."""
         dsaudio = None
         if audio and self.rf.decode_analog_audio:
             dsaudio = self.downscale_audio(audio, lineinfo, lastfieldwritten)
@@ -509,4 +511,10 @@ class Field:
         dsout = u16 if final else f32
         if final:
             self.dspicture = dsout
-        return dsout, dsaudio, None
+        efmout = None
+        if self.rf.decode_digital_audio and "efm" in self.data and getattr(self, "linecount", None):
+            # the field's active span of the EFM output (core.py:2704-2707)
+            llh = ll.cpu().numpy()
+            efmout = self.data["efm"][int(llh[1]):int(llh[self.linecount + 1])]
+        self.efmout = efmout
+        return dsout, dsaudio, efmout

@@ -63,15 +63,45 @@ def audio2_filter(a1_freq, n):
     return freq_response(*sps.butter(order, wn), n) * freq_response(*emphasis(5.3e-6, 75e-6, a1_freq), n)
 
 
-def audio2_half(h):
+def hermitian_half(h):
     """The half spectrum (bins 0..n/2) that a real inverse transform applies
     for np.fft.ifft(X * h).real with X a real signal's spectrum: the
-    Hermitian part (h[k] + conj(h[n-k])) / 2.  The stage-2 low-pass in b/a
-    form is conjugate-symmetric only to ~1e-4, so this is not just the first
-    half of h."""
+    Hermitian part (h[k] + conj(h[n-k])) / 2.  For a one-sided table (the
+    EFM equaliser) that is half of it; the stage-2 audio low-pass in b/a
+    form is conjugate-symmetric only to ~1e-4."""
     n = len(h)
     k = np.arange(n // 2 + 1)
     return 0.5 * (h[k] + np.conj(h[(-k) % n]))
+
+
+audio2_half = hermitian_half
+
+
+def supergauss(x, freq, order=1, centerfreq=0):
+    """Super-Gaussian window (utils.py:870-877)."""
+    return np.exp(-2 * np.power((2 * (x - centerfreq) * (math.log(2.0) / 2.0) ** (1 / (2 * order))) / freq,
+                                2 * order))
+
+
+def efm_filter(freq_hz, n):
+    """The EFM equaliser Fefm (core.py:420-465): cubic interpolation of the
+    11 amplitude / phase knots from 0 to 1.9 MHz onto the bins below 1.9 MHz
+    (one-sided), times 8 and a 60th-order super-Gaussian 20 kHz - 1.6 MHz
+    band-pass that the reference builds for 32768 bins."""
+    import scipy.interpolate as spi
+    knots = np.linspace(0.0e6, 1.9e6, num=11)
+    amp = np.array([0.0, 0.215, 0.41, 0.73, 0.98, 1.03, 0.99, 0.81, 0.59, 0.42, 0.0])
+    phase = [p * 1.25 for p in (0.0, -0.92, -1.03, -1.11, -1.2, -1.2, -1.2, -1.2, -1.05, -0.95, -0.8)]
+    per_bin = freq_hz / n
+    nz = int(knots[-1] / per_bin) + 1
+    f = np.arange(nz) * per_bin
+    a = spi.interp1d(knots, amp, kind="cubic")(f)
+    p = spi.interp1d(knots, phase, kind="cubic")(f)
+    coeffs = np.zeros(n, dtype=complex)
+    coeffs[:nz] = a * (np.cos(p) + (complex(0, -1) * np.sin(p)))
+    bl = 32768
+    sg = supergauss(np.linspace(0, 20000000, bl // 2 + 1), 1600000 - 20000, 60, (1600000 + 20000) / 2.0)[:-1]
+    return coeffs * 8 * np.concatenate([sg, np.flip(sg)])
 
 
 def gd_equaliser(system, lpf, fs, lpf_freq):

@@ -18,7 +18,7 @@ import numpy as np
 
 from . import _lib
 from .field import upload_sinc_lut
-from .filters import FilterBank, audio2_half
+from .filters import FilterBank, audio2_half, efm_filter, hermitian_half
 from .hostdsp import calczc, genwave
 from .params import BLOCKSIZE, system_tables
 
@@ -65,8 +65,6 @@ class RFDecode:
         torch = _torch()
         self.torch = torch
         self.device = _device(torch, device)
-        if decode_digital_audio:
-            raise NotImplementedError("EFM output (decode_digital_audio) is not part of the GPU path")
         # core.py:328-329
         self.NTSC_ColorNotchFilter = bool(extra_options.get("NTSC_ColorNotchFilter", False))
         self.PAL_V4300D_NotchFilter = bool(extra_options.get("PAL_V4300D_NotchFilter", False))
@@ -182,6 +180,10 @@ class RFDecode:
         self._upload(_lib.LD_FILT_BURST, F["FVideoBurst"], c2r, F["FVideoBurst"][0], bank.offset_burst)
         if self.system == "PAL":
             self._upload(_lib.LD_FILT_PILOT, F["FVideoPilot"], c2r, F["FVideoPilot"][0], 0)
+        if self.decode_digital_audio:
+            # core.py:416-418; the GPU applies the part ifft(X * Fefm).real sees
+            F["Fefm"] = efm_filter(self.freq_hz, self.blocklen)
+            self._upload(_lib.LD_FILT_EFM, hermitian_half(F["Fefm"]), c2r)
         self._mtf_uploaded = None
         self._upload_rfvideo(0)
         if self.decode_analog_audio:
@@ -316,18 +318,19 @@ class RFDecode:
         if audio:
             # cut [blockcut//fdiv : -blockcut_end//fdiv] (core.py:872-875)
             alen = (nb_a - self.blockcut // fdiv - (-(-32 // fdiv))) if cut else nb_a
-        names = list(self.video_fields) + ["rfhpf"]
+        names = list(self.video_fields) + ["rfhpf"] + (["efm"] if self.decode_digital_audio else [])
         if channels is not None:
             names = [n for n in names if n in channels]
         res = {}
         o = _lib.DemodOutputs()
         for n in names:
+            odt = torch.int16 if n == "efm" else torch.float32
             if out is not None and n in out:
                 t = out[n]
-                if t.numel() < nblocks * (S if n == "rfhpf" else vlen):
-                    raise ValueError(f"output buffer {n} too small")
+                if t.numel() < nblocks * (S if n == "rfhpf" else vlen) or t.dtype != odt:
+                    raise ValueError(f"output buffer {n} too small or of the wrong type")
             else:
-                t = torch.empty(nblocks * (S if n == "rfhpf" else vlen), dtype=torch.float32, device=self.device)
+                t = torch.empty(nblocks * (S if n == "rfhpf" else vlen), dtype=odt, device=self.device)
             res[n] = t
             setattr(o, n, t.data_ptr())
         if audio:
@@ -350,6 +353,8 @@ class RFDecode:
         rv = {"video": Channels((n, res[n]) for n in self.video_fields if n in res)}
         if "rfhpf" in res:
             rv["rfhpf"] = res["rfhpf"]
+        if "efm" in res:
+            rv["efm"] = res["efm"]
         if audio:
             rv["audio"] = Channels((n, res[n]) for n in ("audio_left", "audio_right"))
         rv["setupcount"] = self.setupcount
