@@ -157,6 +157,28 @@ def run_gpu(args, torch, rf, cap, nfields, steps, warmup, single=None, packed=Fa
     return out
 
 
+def efm_demod_ms(torch, cap, steps, warmup):
+    """Per-pass demodulation time of the 64-field capture with the EFM
+    output (decode_digital_audio, core.py:847-851, on by default in the
+    reference's command line) next to the same pass without it."""
+    from lddecode_amd.rfdecode import RFDecode
+    rf = RFDecode(system="NTSC", decode_analog_audio=44100, decode_digital_audio=True, device=0)
+    x = torch.from_numpy(cap.data).to(rf.device)
+    nblocks = (len(cap.data) - rf.blocklen) // rf.block_stride + 1
+    rv = rf.demod_blocks(x, 0, nblocks, cut=True, audio=True)
+    out = dict(rv["video"], rfhpf=rv["rfhpf"], efm=rv["efm"], **rv["audio"])
+    stream = torch.cuda.current_stream(rf.device)
+    for _ in range(warmup):
+        rf.demod_blocks(x, 0, nblocks, cut=True, audio=True, out=out)
+    e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    e0.record(stream)
+    for _ in range(steps):
+        rf.demod_blocks(x, 0, nblocks, cut=True, audio=True, out=out)
+    e1.record(stream)
+    torch.cuda.synchronize()
+    return e0.elapsed_time(e1) / steps
+
+
 def cpu_baseline(cap, walk, nsample, workers, min_seconds, max_seconds):
     """The same step on the host with the FP64 numpy/scipy oracle: the
     blocks of the first `nsample` located fields' reads are demodulated on a
@@ -374,6 +396,8 @@ def main():
     packed = run_gpu(args, torch, rf, cap, nf, args.steps, args.warmup, packed=True)
     print(f"# packed 10-bit input: {packed['pipe_ms']:.3f} ms/step overlapped", file=sys.stderr, flush=True)
     print(f"# plain: {plain['step_ms']:.3f} ms/step", file=sys.stderr, flush=True)
+    efm_ms = efm_demod_ms(torch, cap, args.steps, args.warmup)
+    print(f"# demod with the EFM output: {efm_ms:.3f} ms/pass", file=sys.stderr, flush=True)
 
     value = batched["usable"] / (batched["pipe_ms"] / 1e3) / 1e6
     plain_value = plain["usable"] / (plain["step_ms"] / 1e3) / 1e6
@@ -433,6 +457,9 @@ def main():
                       "audio_stage2_44k1": round(batched["audio_ms"], 4),
                       "facts_verify": round(batched["facts_ms"], 4)},
         "host_walk_ms": round(batched["walk_host_ms"], 4),
+        "efm": {"demod_ms_with_efm": round(efm_ms, 4), "demod_ms": round(batched["demod_ms"], 4),
+                "note": "k_demod_block_efm: the same pass also writing the int16 EFM output (one more transform "
+                        "per block); not in ms_per_step"},
         "dropouts": {"gpu_ms": round(batched["dropouts_ms"], 4),
                      "host_lists_ms": round(batched["dropout_lists_host_ms"], 4),
                      "error_samples": batched["dropout_errors"], "entries": batched["dropout_entries"],
