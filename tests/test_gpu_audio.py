@@ -123,3 +123,29 @@ def test_field_downscale_audio(gpu_rf):
     pcm = fd.downscale_audio(44100, timeoffset=offs[k])
     np.testing.assert_array_equal(pcm, res[k][0])
     assert fd.audio_next_offset == res[k][1]
+
+
+def test_pipelined_audio_equals_sequential(gpu_rf):
+    """PipelinedDecoder(audio_rate=44100) runs the audio stage in each
+    pass's stage B: every pass of the same capture gives the sequential
+    decoder's int16 pairs, next offsets and failure flags."""
+    import torch
+    rf = gpu_rf("NTSC", audio=True)
+    fl = 667_334
+    cap = synth.make_capture("NTSC", 5 * fl, seed=7, snr_db=40.0, jitter_ns=50, audio=True)
+    x = torch.from_numpy(cap.data).cuda()
+    seq = capture.CaptureDecoder(rf, x, locate=True, audio=True, max_fields=8, audio_rate=44100)
+    seq.step()
+    want = seq.audio_results()
+    assert len(want) >= 3
+    pipe = capture.PipelinedDecoder(rf, x, locate=True, audio=True, max_fields=8, audio_rate=44100)
+    outs = [pipe.step() for _ in range(3)]
+    outs = [o for o in outs if o is not None] + pipe.join()
+    torch.cuda.synchronize()
+    assert len(outs) == 3
+    for d in outs:
+        got = d.audio_results()
+        assert len(got) == len(want)
+        for (a, an, af), (b, bn, bf) in zip(got, want):
+            np.testing.assert_array_equal(a, b)
+            assert an == bn and af == bf
