@@ -471,13 +471,32 @@ __device__ __forceinline__ int rank_in(const unsigned long long* keys, int n, un
   return r;
 }
 
+// The r-th smallest of the wavefront's keys (kk[j] of every lane, unique):
+// the largest t with fewer than r + 1 keys below it, built bit by bit from
+// the top; the counts are ballots, so every step is a few scalar ops.
+template <int J>
+__device__ __forceinline__ unsigned long long wave_select(const unsigned long long (&kk)[J], int r) {
+  unsigned long long ans = 0;
+  for (int b = 63; b >= 0; --b) {
+    const unsigned long long t = ans | (1ull << b);
+    int c = 0;
+#pragma unroll
+    for (int j = 0; j < J; ++j) c += __popcll(__ballot(kk[j] < t));
+    if (c <= r) ans = t;
+  }
+  return ans;
+}
+
+constexpr int kChainJ = (kBurstLines + 63) / 64;   // keys per lane of the selecting wavefront
+
 // One workgroup walks the fields in order; line l = thread l.  Per field:
 // both burst passes from the previous field's median, then the median of
-// the line adjustments by rank (two barriers; LDS double-buffered by field
-// parity).  The next field's line data load while the median forms.
+// the line adjustments, selected by the first wavefront from the keys in
+// LDS (two barriers; LDS double-buffered by field parity).  The next
+// field's line data load while the median forms.
 __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
-  static_assert(kBurstLines % 2 == 0, "rank_in reads pairs");
   __shared__ __align__(16) unsigned long long keys[2][kRefThreads];
+  __shared__ double adjs[2][kRefThreads];
   __shared__ int wcnt[2][kRefThreads / 64], wrise[2][kRefThreads / 64], wnan[2][kRefThreads / 64];
   __shared__ double mid[2][2];
   const int l = threadIdx.x, w = l >> 6, lane = l & 63;
@@ -546,6 +565,7 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
     }
     const unsigned long long key = v ? rank_key(adj, l) : ~0ull;
     keys[p][l] = key;
+    adjs[p][l] = adj;
     const unsigned long long vb = __ballot(v);
     const unsigned long long rb = __ballot(v && rise && (l % 2) == 0);
     const unsigned long long nb = __ballot(v && !(adj == adj));
@@ -563,10 +583,32 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
       rs += wrise[p][i];
       nn += wnan[p][i];
     }
-    if (v) {
-      const int r = rank_in(keys[p], kBurstLines, key);
-      if (r == (m - 1) / 2) mid[p][0] = adj;
-      if (r == m / 2) mid[p][1] = adj;
+    if (w == 0 && m > 0 && nn == 0) {
+      // ranks (m - 1) / 2 and m / 2 of the valid keys (invalid lines hold
+      // the largest key and rank last)
+      unsigned long long kk[kChainJ];
+#pragma unroll
+      for (int j = 0; j < kChainJ; ++j) {
+        const int i = lane + 64 * j;
+        kk[j] = i < kBurstLines ? keys[p][i] : ~0ull;
+      }
+      const unsigned long long k0 = wave_select(kk, (m - 1) / 2);
+      unsigned long long k1 = k0;
+      if ((m % 2) == 0) {
+        unsigned long long lo = ~0ull;
+#pragma unroll
+        for (int j = 0; j < kChainJ; ++j)
+          if (kk[j] > k0 && kk[j] < lo) lo = kk[j];
+        for (int o = 32; o > 0; o >>= 1) {
+          const unsigned long long q = __shfl_xor(lo, o);
+          lo = q < lo ? q : lo;
+        }
+        k1 = lo;
+      }
+      if (lane == 0) {
+        mid[p][0] = adjs[p][k0 & 0x1ff];
+        mid[p][1] = adjs[p][k1 & 0x1ff];
+      }
     }
     __syncthreads();
     // numpy's median (NaN if any value is NaN or none exists), times 2
