@@ -135,12 +135,15 @@ class AudioOutput:
                 ll.data_ptr(), ll.numel(), p, self.out.data_ptr(), self.out.numel() // 2, self.failed.data_ptr(),
                 ws.data_ptr(), ws.numel(), ctypes.c_void_p(self._stream(stream))))
         self._plan = plan
+        self.done = torch.cuda.Event()
+        self.done.record(torch.cuda.ExternalStream(self._stream(stream)))
 
     def results(self):
         """Per field: (int16 array of interleaved left/right pairs, the next
         field's time offset as downscale_audio returns it, failed)."""
         if not self._plan:
             return []
+        self.done.synchronize()
         out = self.out.cpu().numpy()
         failed = self.failed[:len(self._plan)].cpu().numpy()
         return [(out[2 * o:2 * (o + n)].copy(), nxt, bool(failed[k])) for k, (o, n, nxt) in enumerate(self._plan)]
