@@ -274,7 +274,7 @@ def profile_step(args):
     nf = args.fields
     cap = synth.make_capture("NTSC", capture_samples(nf), seed=args.seed, snr_db=40.0)
     dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).to(rf.device), audio=True, locate=True,
-                                 max_fields=nf)
+                                 max_fields=nf, audio_rate=44100)
     for _ in range(args.warmup + args.steps):
         dec.step()
     torch.cuda.synchronize()

@@ -33,6 +33,11 @@ struct LineRow {
   long long first_px[kLinesMax + 1];
 };
 
+constexpr int kTbcThreads = 256;       // pixels per workgroup (one chunk of a line)
+constexpr int kTbcMaxWidth = 2048;
+constexpr int kTbcChunksMax = kTbcMaxWidth / kTbcThreads;
+constexpr int kChunkStage = 1024;      // staged source samples per chunk (256 pixels at up to 3.9 samples each)
+
 // Everything the pixel kernel needs for one output line, written by
 // k_tbc_lines so that a line's workgroups start with a single record load.
 struct LineRec {
@@ -45,6 +50,10 @@ struct LineRec {
   long long lineloc_offset;
   double inl, rinl, outscale;
   int jB, nseg, W, field;
+  // per 256-pixel chunk: first staged source sample and staged count (0:
+  // the chunk's taps are read from global memory)
+  long long slo[kTbcChunksMax];
+  int nst[kTbcChunksMax];
 };
 
 // Segment of the k=1 spline holding x: knots at j*inlinelen, right-continuous
@@ -116,6 +125,41 @@ __device__ double weighted_median(const double* key, const int* wgt, int* scan, 
   const double m = (L % 2) ? out[1] : (out[0] + out[1]) / 2.0;
   __syncthreads();
   return m;
+}
+
+// Source position of pixel p of a line (the spline value of pixel_prep).
+__device__ __forceinline__ double line_loc(const double* __restrict__ linelocs, const LineRec& rc, long long p) {
+  const double x = (double)p * rc.outscale;
+  int j = (int)(x * rc.rinl);
+  if ((double)(j + 1) * rc.inl <= x) ++j;
+  if ((double)j * rc.inl > x) --j;
+  j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
+  const int jl = j - rc.jB;
+  double llj, llj1, w;
+  if (jl >= 0 && jl <= 2) {
+    llj = rc.ll[jl];
+    llj1 = rc.ll[jl + 1];
+    w = rc.w[jl];
+  } else {
+    const double* ll = linelocs + rc.lineloc_offset;
+    llj = ll[j];
+    llj1 = ll[j + 1];
+    w = 1.0 / (((double)(j + 1) * rc.inl) - ((double)j * rc.inl));
+  }
+  const double t0 = (double)j * rc.inl, t1 = (double)(j + 1) * rc.inl;
+  return llj * (w * (t1 - x)) + llj1 * (w * (x - t0));
+}
+
+// Staged source span of pixels c0 .. c1 of a line: from the positions of
+// the first and last pixel, with the taps' reach either side.
+__device__ __forceinline__ void chunk_span(const double* __restrict__ linelocs, const LineRec& rc, int c0, int c1,
+                                           long long& slo, int& nst) {
+  const double la = line_loc(linelocs, rc, rc.p0 + c0), lb = line_loc(linelocs, rc, rc.p0 + c1);
+  const double lo = fmin(la, lb), hi = fmax(la, lb);
+  const bool finite = lo == lo && hi == hi && fabs(lo) < 1e15 && fabs(hi) < 1e15;
+  slo = finite ? (long long)floor(lo) - (kSincTaps / 2 + 2) : 0;
+  const long long shi = finite ? (long long)floor(hi) + (kSincTaps / 2 + 4) : 0;
+  nst = (finite && shi - slo <= kChunkStage) ? (int)(shi - slo) : 0;
 }
 
 __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* __restrict__ fields,
@@ -223,6 +267,12 @@ __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* 
     r.nseg = nseg;
     r.W = W;
     r.field = blockIdx.x;
+    for (int c = 0; c < kTbcChunksMax; ++c) {
+      const int c0 = c * kTbcThreads;
+      r.slo[c] = 0;
+      r.nst[c] = 0;
+      if (c0 < W) chunk_span(linelocs, r, c0, (W - c0 < kTbcThreads ? W : c0 + kTbcThreads) - 1, r.slo[c], r.nst[c]);
+    }
     out[line] = r;
   }
 }
@@ -242,9 +292,6 @@ struct TbcPixelArgs {
   int chunks;              // workgroups per output line
 };
 
-constexpr int kTbcThreads = 256;       // pixels per workgroup (one chunk of a line)
-constexpr int kTbcMaxWidth = 2048;
-constexpr int kChunkStage = 2048;      // staged source samples per chunk
 constexpr int kStagePerThread = kChunkStage / kTbcThreads;
 
 typedef __amdgpu_buffer_rsrc_t rsrc_t;
@@ -252,14 +299,21 @@ __device__ __forceinline__ float4 lut_load(rsrc_t r, uint32_t off) {
   return __builtin_bit_cast(float4, __builtin_amdgcn_raw_buffer_load_b128(r, (int)off, 0, 0));
 }
 
-// Per-pixel state between the LUT loads and the tap sum.
+__device__ __forceinline__ float comp(const float4& f, int c) {
+  return c == 0 ? f.x : (c == 1 ? f.y : (c == 2 ? f.z : f.w));
+}
+
+// Per-pixel state between the LUT row addresses and the tap sum.
 struct PixelPrep {
   long long start;  // first tap's sample index (field-relative)
   double level;
   float alpha;
-  bool fold;
-  float4 u[4], v[4];  // LUT groups: ws taps (u) and we taps (v), see pixel_prep
+  int fold;
+  uint32_t ou, ov;  // byte offsets of the ws / we groups, see pixel_prep
+  int step;         // byte step between groups (-16 in a folded row)
 };
+// LUT offset past the table: buffer loads there return zeros
+constexpr uint32_t kLutNone = 0x7fffff00u;
 
 // Source position, level factor and LUT rows of pixel t of the line.
 __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec& rc, rsrc_t lut, int t,
@@ -337,22 +391,73 @@ __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec&
   const bool last = ph0 == kSincPhases - 1;
   const uint32_t ou = fold ? rowB + 48u : rowA;
   const uint32_t ov = fold ? (last ? rowB : rowA) + 48u : rowB;
-  const int step = fold ? -16 : 16;
-#pragma unroll
-  for (int g = 0; g < 4; ++g) {
-    q.u[g] = lut_load(lut, ou + g * step);
-    q.v[g] = lut_load(lut, ov + g * step);
-  }
+  q.step = fold ? -16 : 16;
+  q.ou = ou;
+  q.ov = ov;
   q.fold = fold;
 }
 
-__device__ __forceinline__ float comp(const float4& f, int c) {
-  return c == 0 ? f.x : (c == 1 ? f.y : (c == 2 ? f.z : f.w));
+// Value of lane g of the thread's 4-lane group (DPP quad broadcast).
+template <int G>
+__device__ __forceinline__ int quad_bcast(int v) {
+  return __builtin_amdgcn_mov_dpp(v, G * 0x55, 0xf, 0xf, true);
+}
+
+// The sinc weights of the 4 pixels of each 4-lane group are loaded
+// together: lane l of the group fetches tap group l (taps 4l .. 4l+3) of
+// every pixel of the group, so one load instruction touches one 64-byte row
+// per 4 lanes instead of one per lane.  quad_fetch issues the loads;
+// quad_store forms wt = ws + alpha (we - ws) for those taps (the same fmaf
+// as a per-pixel evaluation) and writes them to LDS at wts_slot.
+// Every lane of the wavefront takes part.
+//
+// wts_slot: float4 slot of tap group c of pixel P.  Pixel blocks are
+// permuted (P ^ (P/4 mod 4)) and the groups inside a block rotated by P, so
+// that both the stores (lane l of quad m: group l of pixel 4m+g) and the
+// loads (lane P: group k of its own pixel) spread over all banks.
+__device__ __forceinline__ int wts_slot(int P, int c) { return 4 * (P ^ ((P >> 2) & 3)) + ((c + P) & 3); }
+struct QuadRows {
+  float4 u[4], v[4];
+  int fold[4];
+  float alpha[4];
+};
+__device__ __forceinline__ void quad_fetch(rsrc_t lut, const PixelPrep& q, QuadRows& r, int tid) {
+  const int l = tid & 3;
+  auto fetch = [&](auto gc) {
+    constexpr int g = decltype(gc)::value;
+    const uint32_t ou = (uint32_t)quad_bcast<g>((int)q.ou), ov = (uint32_t)quad_bcast<g>((int)q.ov);
+    const int step = quad_bcast<g>(q.step);
+    r.fold[g] = quad_bcast<g>(q.fold);
+    r.alpha[g] = __int_as_float(quad_bcast<g>(__float_as_int(q.alpha)));
+    r.u[g] = lut_load(lut, ou + (uint32_t)(l * step));
+    r.v[g] = lut_load(lut, ov + (uint32_t)(l * step));
+  };
+  fetch(std::integral_constant<int, 0>());
+  fetch(std::integral_constant<int, 1>());
+  fetch(std::integral_constant<int, 2>());
+  fetch(std::integral_constant<int, 3>());
+}
+__device__ __forceinline__ void quad_store(const QuadRows& r, float4* wts, int tid) {
+  const int l = tid & 3;
+  const int qb = tid & ~3;
+#pragma unroll
+  for (int g = 0; g < 4; ++g) {
+    float w[4];
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+      const float ws = comp(r.u[g], c), we = comp(r.v[g], c);
+      w[c] = fmaf(r.alpha[g], we - ws, ws);
+    }
+    // a folded row holds the taps of the group in reverse order
+    const float4 o = r.fold[g] ? make_float4(w[3], w[2], w[1], w[0]) : make_float4(w[0], w[1], w[2], w[3]);
+    wts[wts_slot(qb + g, l)] = o;
+  }
 }
 
 // 16-tap sum, level factor and output codes of pixel t.
 __device__ __forceinline__ void pixel_finish(const TbcPixelArgs& a, const LineRec& rc, const float* stage,
-                                             const float* src, long long slo, int nst, int t, const PixelPrep& q) {
+                                             const float* src, long long slo, int nst, int t, const PixelPrep& q,
+                                             const float4* wts, int c0) {
   const long long rel = q.start - slo;
   float sv[kSincTaps];
   if (rel >= 0 && rel + kSincTaps <= nst) {
@@ -366,15 +471,12 @@ __device__ __forceinline__ void pixel_finish(const TbcPixelArgs& a, const LineRe
       sv[e] = (idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
     }
   }
+  float4 wq[4];
+#pragma unroll
+  for (int g = 0; g < 4; ++g) wq[g] = wts[wts_slot(t - c0, g)];
   double acc = 0.0;
 #pragma unroll
-  for (int e = 0; e < kSincTaps; ++e) {
-    const int g = e >> 2, c = e & 3;
-    const float ws = q.fold ? comp(q.u[g], 3 - c) : comp(q.u[g], c);
-    const float we = q.fold ? comp(q.v[g], 3 - c) : comp(q.v[g], c);
-    const float wt = fmaf(q.alpha, we - ws, ws);
-    acc += (double)(sv[e] * wt);
-  }
+  for (int e = 0; e < kSincTaps; ++e) acc += (double)(sv[e] * comp(wq[e >> 2], e & 3));
   const float r = (float)(q.level * acc);
   const long long o = rc.out_offset + t;
   if (a.out_f32) a.out_f32[o] = r;
@@ -383,29 +485,6 @@ __device__ __forceinline__ void pixel_finish(const TbcPixelArgs& a, const LineRe
     vv = vv < 0.0 ? 0.0 : (vv > 65535.0 ? 65535.0 : vv);
     a.out_u16[o] = (uint16_t)vv;
   }
-}
-
-// Source position of pixel p of a line (the spline value of pixel_prep).
-__device__ __forceinline__ double line_loc(const TbcPixelArgs& a, const LineRec& rc, long long p) {
-  const double x = (double)p * rc.outscale;
-  int j = (int)(x * rc.rinl);
-  if ((double)(j + 1) * rc.inl <= x) ++j;
-  if ((double)j * rc.inl > x) --j;
-  j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
-  const int jl = j - rc.jB;
-  double llj, llj1, w;
-  if (jl >= 0 && jl <= 2) {
-    llj = rc.ll[jl];
-    llj1 = rc.ll[jl + 1];
-    w = rc.w[jl];
-  } else {
-    const double* ll = a.linelocs + rc.lineloc_offset;
-    llj = ll[j];
-    llj1 = ll[j + 1];
-    w = 1.0 / (((double)(j + 1) * rc.inl) - ((double)j * rc.inl));
-  }
-  const double t0 = (double)j * rc.inl, t1 = (double)(j + 1) * rc.inl;
-  return llj * (w * (t1 - x)) + llj1 * (w * (x - t0));
 }
 
 // One workgroup per 256-pixel chunk of an output line (blockIdx.x = line *
@@ -417,6 +496,7 @@ __device__ __forceinline__ double line_loc(const TbcPixelArgs& a, const LineRec&
 // them from global memory.
 __global__ __launch_bounds__(kTbcThreads) void k_tbc_pixels(TbcPixelArgs a) {
   __shared__ float stage[kChunkStage];
+  __shared__ float4 wts[kTbcThreads * 4];
   const int line = blockIdx.x / a.chunks, chunk = blockIdx.x - line * a.chunks;
   const LineRec& rc = a.recs[line];
   const int W = rc.W;
@@ -428,17 +508,21 @@ __global__ __launch_bounds__(kTbcThreads) void k_tbc_pixels(TbcPixelArgs a) {
   const int t = c0 + tid;
   const rsrc_t lut = __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(a.lut), (short)0,
                                                        (int)((kSincPhases + 1) * kSincTaps * 4), 0x00020000);
-  // staging span of the chunk (uniform)
-  const double la = line_loc(a, rc, rc.p0 + c0), lb = line_loc(a, rc, rc.p0 + c1);
-  const double lo = fmin(la, lb), hi = fmax(la, lb);
-  const bool finite = lo == lo && hi == hi && fabs(lo) < 1e15 && fabs(hi) < 1e15;
-  const long long slo = finite ? (long long)floor(lo) - (kSincTaps / 2 + 2) : 0;
-  const long long shi = finite ? (long long)floor(hi) + (kSincTaps / 2 + 4) : 0;
-  const int nst = (finite && shi - slo <= kChunkStage) ? (int)(shi - slo) : 0;
-  // LUT loads first: the in-order load counter then lets the barrier wait
-  // cover both the LUT rows and the staging loads issued after them
+  // staging span of the chunk (uniform, from the line record)
+  const long long slo = rc.slo[chunk];
+  const int nst = rc.nst[chunk];
+  // LUT loads first: the in-order load counter then lets the weights be
+  // formed while the staging loads issued after them are in flight
   PixelPrep q;
+  q.start = 0;
+  q.level = 0.0;
+  q.alpha = 0.0f;
+  q.fold = 0;
+  q.ou = q.ov = kLutNone;
+  q.step = 16;
   if (t <= c1) pixel_prep(a, rc, lut, t, q);
+  QuadRows rows;
+  quad_fetch(lut, q, rows, tid);
   float sreg[kStagePerThread];
 #pragma unroll
   for (int k = 0; k < kStagePerThread; ++k) {
@@ -446,13 +530,14 @@ __global__ __launch_bounds__(kTbcThreads) void k_tbc_pixels(TbcPixelArgs a) {
     const long long idx = slo + i;
     sreg[k] = (i < nst && idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
   }
+  quad_store(rows, wts, tid);
 #pragma unroll
   for (int k = 0; k < kStagePerThread; ++k) {
     const int i = tid + k * kTbcThreads;
     if (i < nst) stage[i] = sreg[k];
   }
   __syncthreads();
-  if (t <= c1) pixel_finish(a, rc, stage, src, slo, nst, t, q);
+  if (t <= c1) pixel_finish(a, rc, stage, src, slo, nst, t, q, wts, c0);
 }
 
 // ------------------------------------------------------------- hsync refine
