@@ -7,8 +7,9 @@
 //   PAL   FieldPAL.refine_linelocs_pilot (core.py:3054-3093) twice, then
 //         fix_badlines (:3221-3226)
 // Kernels:
-//   k_burst_scan    one thread per (field, line): burst area statistics and
-//                   the zero-crossing scan, reduced to 16 cycle positions
+//   k_burst_scan    8 lanes per (field, line), 8 lines per wavefront: burst
+//                   area statistics, the zero-crossing scan, the 16 cycle
+//                   positions and the step table over them
 //   k_burst_chain   one workgroup walks the fields in order: each field's
 //                   phase-adjust median seeds the next field (core.py:3366-
 //                   3369); lines in parallel within a field
@@ -262,116 +263,66 @@ __device__ double burst_passes(const double (&base)[kZc], int count, int rmask, 
   return pa;
 }
 
-// Step positions and per-interval results of one line (see BurstLine), by
-// one wavefront: lane k < 16 takes slot k's step position frac(-0.5 -
-// base[k]) and its rank among the 16; lanes i <= 16 then run both passes at
-// the middle of interval i.
-__device__ void burst_table(BurstLine& out, const double (&base)[kZc], int count, int rmask, double* sbp, int lane) {
-  double mine = 0.0;
-#pragma unroll
-  for (int k = 0; k < kZc; ++k)
-    if (k == lane) mine = base[k];
-  double t = -0.5 - mine;
-  t = t - floor(t);
-  if (lane < kZc) sbp[lane] = t;
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  int r = 0;
-#pragma unroll
-  for (int k = 0; k < kZc; ++k) {
-    const double o = sbp[k];
-    r += (o < t) || (o == t && k < lane);
-  }
-  __builtin_amdgcn_wave_barrier();
-  if (lane < kZc) {
-    sbp[kZc + r] = t;
-    out.bp[r] = t;
-  }
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
-  __builtin_amdgcn_wave_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
-  if (lane <= kZc) {
-    const double lo = lane == 0 ? 0.0 : sbp[kZc + lane - 1];
-    const double hi = lane == kZc ? 1.0 : sbp[kZc + lane];
-    int rc;
-    const double pa = burst_passes(base, count, rmask, 0.5 * (lo + hi), rc);
-    out.tab[lane] = pa;
-    out.rc[lane] = (unsigned char)rc;
-  }
-}
+constexpr int kAreaMax = 256;              // burst-area samples a line may have
+constexpr int kScanLines = 8;              // lines per wavefront of k_burst_scan
+constexpr int kLaneGroup = 64 / kScanLines; // lanes per line
+constexpr int kAreaPitch = kAreaMax + 4;   // LDS row of one line's area (rows 16-byte aligned, 4 banks apart)
 
-constexpr int kAreaMax = 256;  // burst-area samples a wavefront holds (4 per lane)
-constexpr int kAreaWords = kAreaMax / 64;
-
-// First set bit at or above position j of a kAreaMax-bit mask, or -1.
-__device__ __forceinline__ int next_bit(const unsigned long long (&m)[kAreaWords], int j) {
-#pragma unroll
-  for (int h = 0; h < kAreaWords; ++h) {
-    if (j < 64 * (h + 1)) {
-      const int k = j - 64 * h;
-      const unsigned long long t = k > 0 ? (m[h] >> k) << k : m[h];
-      if (t) return 64 * h + __builtin_ctzll(t);
-      j = 64 * (h + 1);
-    }
-  }
-  return -1;
-}
-
-// Value i (uniform) of the area held as x[h] = value(lane + 64 h).
-__device__ __forceinline__ float area_at(const float (&x)[kAreaWords], int i) {
-  float v = 0.0f;
-#pragma unroll
-  for (int h = 0; h < kAreaWords; ++h)
-    if ((i >> 6) == h) v = __builtin_bit_cast(float, __builtin_amdgcn_readlane(__builtin_bit_cast(int, x[h]), i & 63));
-  return v;
-}
-
-constexpr int kScanWaves = 4;  // lines per workgroup of k_burst_scan
-
-// One wavefront per (field, line): compute_line_bursts' statistics and the
-// clb_findbursts zero-crossing scan (core.py:2999-3047, utils.py:1339-1368),
-// reduced to the 16 cycle bases and the step table.  The statistics keep
-// the reference's summation orders (all lanes evaluate the sequential sums
-// redundantly from LDS); the scan walks ballot masks.  A burst area longer
-// than kAreaMax samples (input rates above about 130 MSPS) counts as no
-// burst.
-__global__ __launch_bounds__(64 * kScanWaves) void k_burst_scan(RefineArgs a, const float* __restrict__ demod,
-                                                               const float* __restrict__ dburst) {
-  __shared__ __align__(16) float sb[kScanWaves][kAreaMax];
-  __shared__ __align__(16) float sd[kScanWaves][kAreaMax];
-  __shared__ double sbp[kScanWaves][2 * kZc];
-  // the wavefront index is uniform: readfirstlane lets the line's scalars,
-  // loads and loop bounds live in scalar registers
-  const int wv = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6), lane = threadIdx.x & 63;
-  const int l = blockIdx.x * kScanWaves + wv;
+// compute_line_bursts (core.py:2999-3047) and the burst table (see
+// BurstLine), kScanLines lines per wavefront with kLaneGroup lanes per
+// line.  The area statistics and the zero-crossing walk are sequential per
+// line (the float32 sums keep the reference's order), so every lane of a
+// line's group runs them on the line's area staged in LDS; the 17 table
+// intervals are then split over the group's lanes.
+__global__ __launch_bounds__(64) void k_burst_scan(RefineArgs a, const float* __restrict__ demod,
+                                                   const float* __restrict__ dburst) {
+  __shared__ __align__(16) float sb[kScanLines][kAreaPitch];
+  __shared__ __align__(16) float sd[kScanLines][kAreaPitch];
+  __shared__ double sbp[kScanLines][kZc];
+  const int lane = threadIdx.x;
+  const int q = lane % kLaneGroup, ln = lane / kLaneGroup;
+  const int l = blockIdx.x * kScanLines + ln;
   const int f = blockIdx.y;
-  if (l >= kBurstLines) return;
   const ld_line_field fd = a.fields[f];
   const double* ll = a.ll2 + fd.lineloc_offset;
-  BurstLine& out = a.lines[(size_t)f * kBurstLines + l];
-  if (lane == 0) {
+  const bool inrange = l < kBurstLines;
+  BurstLine& out = a.lines[(size_t)f * kBurstLines + (inrange ? l : 0)];
+  if (inrange && q == 0) {
     out.valid = 0;
     out.count = 0;
     out.rising = 0;
   }
-  BurstArea ba;
-  if (!burst_area(a, fd, ll, l, ba)) return;
-  const int n = ba.n;
-  if (n > kAreaMax) return;
-  float* b = sb[wv];
-  float* d = sd[wv];
-  const float* pb = dburst + fd.demod_offset + ba.s0;
-  const float* pd = demod + fd.demod_offset + ba.s0;
+  BurstArea ba{};
+  const bool has = inrange && burst_area(a, fd, ll, l, ba) && ba.n <= kAreaMax;
+  const int n = has ? ba.n : 0;
+  // stage the line's area: lane q of the group loads samples q + 8k
+  float* b = sb[ln];
+  float* d = sd[ln];
+  {
+    const float* pb = dburst + fd.demod_offset + (has ? ba.s0 : 0);
+    const float* pd = demod + fd.demod_offset + (has ? ba.s0 : 0);
+    for (int k0 = q; k0 < n; k0 += 8 * kLaneGroup) {
+      float vb[8], vd[8];
 #pragma unroll
-  for (int h = 0; h < kAreaWords; ++h) {
-    const int i = lane + 64 * h;
-    b[i] = i < n ? pb[i] : 0.0f;
-    d[i] = i < n ? pd[i] : 0.0f;
+      for (int u = 0; u < 8; ++u) {
+        const int i = k0 + u * kLaneGroup;
+        vb[u] = i < n ? pb[i] : 0.0f;
+        vd[u] = i < n ? pd[i] : 0.0f;
+      }
+#pragma unroll
+      for (int u = 0; u < 8; ++u) {
+        const int i = k0 + u * kLaneGroup;
+        if (i < n) {
+          b[i] = vb[u];
+          d[i] = vd[u];
+        }
+      }
+    }
   }
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  if (!has) return;
   // burstarea - nb_mean(burstarea), rounded to float32 (core.py:3022)
   const float mb = (float)nb_mean_f32_lds(b, n);
   auto bf = [&](int i) { return b[i] - mb; };
@@ -382,26 +333,10 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_burst_scan(RefineArgs a, co
   }, n));
   const float md = (float)nb_mean_f32_lds(d, n);
   float amax = 0.0f;
-  float x[kAreaWords];
-#pragma unroll
-  for (int h = 0; h < kAreaWords; ++h) {
-    const int i = lane + 64 * h;
-    if (i < n) amax = fmaxf(amax, fabsf(d[i] - md));
-    x[h] = i < n ? bf(i) : 0.0f;
-  }
-  for (int o = 32; o > 0; o >>= 1) amax = fmaxf(amax, __shfl_xor(amax, o));
+  for (int i = 0; i < n; ++i) amax = fmaxf(amax, fabsf(d[i] - md));
   if ((double)amax > 30 * a.hz_ire) return;
-  // masks over the area: |x| > threshold, x < 0, rising and falling crossings of 0
-  unsigned long long above[kAreaWords], neg[kAreaWords], rise[kAreaWords], fall[kAreaWords];
-#pragma unroll
-  for (int h = 0; h < kAreaWords; ++h) {
-    const int i = lane + 64 * h;
-    const float xp = (i >= 1 && i < n) ? bf(i - 1) : 0.0f;
-    above[h] = __ballot(i < n && fabsf(x[h]) > thr);
-    neg[h] = __ballot(i < n && x[h] < 0.0f);
-    rise[h] = __ballot(i >= 1 && i < n && xp < 0.0f && x[h] >= 0.0f);
-    fall[h] = __ballot(i >= 1 && i < n && xp > 0.0f && x[h] <= 0.0f);
-  }
+  // clb_findbursts' walk: from each sample above the threshold, the next
+  // crossing of 0 in its direction within 16 samples
   float zcs[kZc];
 #pragma unroll
   for (int k = 0; k < kZc; ++k) zcs[k] = 0.0f;
@@ -409,34 +344,47 @@ __global__ __launch_bounds__(64 * kScanWaves) void k_burst_scan(RefineArgs a, co
   int j = 0;
   const int end = n - 1;
   while (j < end && count < kZc) {
-    j = next_bit(above, j);
-    if (j < 0 || j >= end) break;
-    const bool up = (neg[j >> 6] >> (j & 63)) & 1;
-    // calczc_do: first crossing i in (j, min(j + 17, n))
-    const int i = up ? next_bit(rise, j + 1) : next_bit(fall, j + 1);
-    if (i < 0 || i >= (j + 17 < n ? j + 17 : n)) break;
-    const double pa = (double)area_at(x, i - 1), pb2 = (double)area_at(x, i);
-    const double y = (pb2 - pa != 0.0) ? -pa / (-pa + pb2) : 0.0;
-    const double zc = (double)(i - 1) + y;
-    if (up) rising |= 1 << count;
+    while (j < end && !(fabsf(bf(j)) > thr)) ++j;
+    if (j >= end) break;
+    double zc;
+    if (!calczc0(bf, n, j, zc)) break;
+    if (bf(j) < 0.0f) rising |= 1 << count;
 #pragma unroll
     for (int k = 0; k < kZc; ++k)
       if (k == count) zcs[k] = (float)zc;  // zcs is a float32 array in the reference
     ++count;
     j = (int)zc + 1;
   }
-  double base[kZc];
+  double base[kZc], t[kZc];
 #pragma unroll
-  for (int k = 0; k < kZc; ++k) base[k] = (((double)ba.bstart + (double)zcs[k]) - ba.s_rem) / ba.zcburstdiv;
-  if (lane < kZc) {
-    double z = 0.0;
-#pragma unroll
-    for (int k = 0; k < kZc; ++k)
-      if (k == lane) z = base[k];
-    out.base[lane] = z;
+  for (int k = 0; k < kZc; ++k) {
+    base[k] = (((double)ba.bstart + (double)zcs[k]) - ba.s_rem) / ba.zcburstdiv;
+    // slot k's step position frac(-0.5 - base[k])
+    const double v = -0.5 - base[k];
+    t[k] = v - floor(v);
   }
-  burst_table(out, base, count, rising, sbp[wv], lane);
-  if (lane == 0) {
+#pragma unroll
+  for (int k = 0; k < kZc; ++k)
+    if ((k % kLaneGroup) == q) out.base[k] = base[k];
+  sort16(t);
+  if (q == 0) {
+#pragma unroll
+    for (int k = 0; k < kZc; ++k) sbp[ln][k] = t[k];
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  // both passes at the middle of each interval between step positions
+  for (int iv = q; iv <= kZc; iv += kLaneGroup) {
+    const double lo = iv == 0 ? 0.0 : sbp[ln][iv - 1];
+    const double hi = iv == kZc ? 1.0 : sbp[ln][iv];
+    int rc;
+    const double pa = burst_passes(base, count, rising, 0.5 * (lo + hi), rc);
+    out.tab[iv] = pa;
+    out.rc[iv] = (unsigned char)rc;
+    if (iv < kZc) out.bp[iv] = hi;
+  }
+  if (q == 0) {
     double mb2 = base[0];
 #pragma unroll
     for (int k = 1; k < kZc; ++k) mb2 = fmin(mb2, base[k]);
@@ -937,8 +885,8 @@ int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_bu
   // a pageable source is staged before hipMemcpyAsync returns
   LD_HIP(hipMemcpyAsync((void*)a.fields, fields, sizeof(ld_line_field) * (size_t)nfields, hipMemcpyHostToDevice,
                         st));
-  hipLaunchKernelGGL(k_burst_scan, dim3((kBurstLines + kScanWaves - 1) / kScanWaves, nfields), dim3(64 * kScanWaves),
-                     0, st, a, demod, demod_burst);
+  hipLaunchKernelGGL(k_burst_scan, dim3((kBurstLines + kScanLines - 1) / kScanLines, nfields), dim3(64), 0, st, a,
+                     demod, demod_burst);
   LD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_burst_chain, dim3(1), dim3(kRefThreads), 0, st, a);
   LD_HIP(hipGetLastError());
