@@ -128,6 +128,16 @@ typedef struct ld_demod_outputs {
   /* EFM front-end (core.py:847-851): int16(clip(irFFT(X * Fefm))), cut and
    * strided like the video outputs; needs LD_FILT_EFM; NULL: skip */
   int16_t* efm;
+  /* findpulses' comparison fused into the demod_05 output: word w holds
+   * demod_05[32 w + i] <= below_high (compared in double, as numba compares
+   * the float32 buffer with its float64 threshold) for i < 32, paired: bit
+   * j < 16 is sample 32 w + 2 j, bit 16 + j sample 32 w + 2 j + 1.  Block b
+   * fills words (b * video_stride) / 32 onwards; bits past its output are
+   * 0.  Needs demod_05, video_stride % 32 == 0 and demod_05's output offset
+   * (blockcut + its filter shift) % 32 == 0; NULL: skip.  Input of
+   * ld_find_pulses_bits. */
+  uint32_t* demod_05_below;
+  double below_high;
 } ld_demod_outputs;
 
 /* Demodulate `nblocks` blocks; block b reads input[first + b*step .. + blocklen).
@@ -306,6 +316,15 @@ size_t ld_pulse_workspace_bytes(const ld_handle* h, int64_t n);
  * sample, end < its last and length <= 5000. */
 int ld_find_pulses(ld_handle* h, const float* x, int64_t n, double high, int64_t* edges, int64_t cap,
                    int64_t* count, void* workspace, size_t workspace_bytes, void* stream);
+
+/* ld_find_pulses from the demod's in-pulse bits (ld_demod_outputs.
+ * demod_05_below's paired DEVICE words, at least 2 * ceil(n / 64) of them,
+ * 8-byte aligned) instead of the samples: the same edge list for the same
+ * buffer and threshold, reading n / 8 bytes.  Same findpulses_numba_raw
+ * (utils.py:1101-1141). */
+size_t ld_pulse_bits_workspace_bytes(const ld_handle* h, int64_t n);
+int ld_find_pulses_bits(ld_handle* h, const uint32_t* bits, int64_t n, int64_t* edges, int64_t cap,
+                        int64_t* count, void* workspace, size_t workspace_bytes, void* stream);
 
 /* getpulses' vertical-sync levels (core.py:2274-2286): for every pulse k
  * of ld_find_pulses' list whose length exceeds min_len, the mean of

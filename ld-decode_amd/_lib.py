@@ -22,7 +22,8 @@ EXPORTS = (
     "ld_set_demod_params", "ld_demod_workspace_bytes", "ld_demod_blocks", "ld_set_sinc_lut",
     "ld_tbc_workspace_bytes", "ld_tbc_fields", "ld_hsync_workspace_bytes", "ld_refine_hsync", "ld_set_notch",
     "ld_refine_workspace_bytes", "ld_refine_burst_ntsc", "ld_refine_pilot_pal",
-    "ld_pulse_workspace_bytes", "ld_find_pulses", "ld_pulse_means", "ld_range_means",
+    "ld_pulse_workspace_bytes", "ld_find_pulses", "ld_pulse_bits_workspace_bytes", "ld_find_pulses_bits",
+    "ld_pulse_means", "ld_range_means",
     "ld_field_facts_workspace_bytes", "ld_field_facts_measure", "ld_locate_field", "ld_locate_fields",
     "ld_skip_score", "ld_locate_recheck", "ld_unpack_10bit",
     "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
@@ -47,7 +48,8 @@ class DemodOutputs(ctypes.Structure):
     _fields_ = [(n, ctypes.c_void_p) for n in (
         "rfhpf", "demod", "demod_raw", "demod_05", "demod_burst", "demod_pilot",
         "audio_left", "audio_right")] + [("video_stride", ctypes.c_int64), ("audio_stride", ctypes.c_int64),
-                                         ("efm", ctypes.c_void_p)]
+                                         ("efm", ctypes.c_void_p), ("demod_05_below", ctypes.c_void_p),
+                                         ("below_high", ctypes.c_double)]
 
 
 class TbcField(ctypes.Structure):
@@ -175,6 +177,8 @@ def _declare(lib):
                                         c_vp, c_vp, c_vp, c_sz, c_vp]),
         "ld_pulse_workspace_bytes": (c_sz, [c_vp, c_i64]),
         "ld_find_pulses": (c_int, [c_vp, c_vp, c_i64, c_dbl, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
+        "ld_pulse_bits_workspace_bytes": (c_sz, [c_vp, c_i64]),
+        "ld_find_pulses_bits": (c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
         "ld_pulse_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
         "ld_range_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp, c_vp]),
         "ld_field_facts_workspace_bytes": (c_sz, [c_vp, c_int]),

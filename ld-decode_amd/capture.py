@@ -70,6 +70,10 @@ class CaptureDecoder:
         self.buffers = {n: torch.empty(self.total, dtype=torch.float32, device=dev) for n in names}
         if rf.decode_digital_audio:
             self.buffers["efm"] = torch.empty(self.total, dtype=torch.int16, device=dev)
+        if locate:
+            # demod_05's in-pulse bits, written by the demod for the pulse search
+            self.buffers["demod_05_below"] = torch.empty(2 * -(-self.total // 64), dtype=torch.int32, device=dev)
+        self._below_high = None
         self.audio = bool(audio)
         if self.audio:
             na = rf.filterbank.audio["left"]["nbins"]
@@ -172,12 +176,17 @@ class CaptureDecoder:
 
     # ------------------------------------------------------------ stages
     def demod(self, mtf_level=0):
+        below = self.buffers.get("demod_05_below")
+        self._below_high = locatemod.pulse_threshold(self.rf) if below is not None else None
         return self.rf.demod_blocks(self.data, 0, self.nblocks, mtf_level=mtf_level, cut=True, audio=self.audio,
-                                    out=self.buffers)
+                                    out=self.buffers, below=below, below_high=self._below_high)
 
     def pulses(self):
-        """Enqueue the sync-pulse pass over demod_05 and its copy to the host."""
-        self.walker.pulses(self.buffers["demod_05"])
+        """Enqueue the sync-pulse pass over demod_05 and its copy to the host
+        (from the demod's in-pulse bits while their threshold holds)."""
+        below = self.buffers.get("demod_05_below")
+        bits = (below, self._below_high) if below is not None and self._below_high is not None else None
+        self.walker.pulses(self.buffers["demod_05"], bits=bits)
 
     def locate(self, facts=None, prev=None):
         """Host: walk the fields (waits for the pulse pass), then copy the

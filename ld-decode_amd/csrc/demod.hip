@@ -183,6 +183,9 @@ struct DemodArgs {
   int notch_lo, notch_hi;  // spur notch window, empty: off
   const float2* h_efm;     // EFM equaliser (Hermitian half spectrum, c2r scale)
   int16_t* efm;            // EFM output (cut and strided as the video outputs)
+  uint32_t* below;         // in-pulse bits of one post output (video_stride / 32 words per block)
+  float below_highf;       // largest float not above the threshold
+  int below_post;          // index of that output in post[], -1: off
 };
 
 constexpr int kNotchMax = 508;  // bins in the notch window
@@ -265,10 +268,24 @@ __device__ __forceinline__ void stage_own(const float2 (&v)[kP], Lds& s, int tid
   });
 }
 
+// Bits of a paired in-pulse word (even samples in bits 0..15, odd in
+// 16..31) that hold samples 0 .. left-1 of the word, left < 32.
+__device__ __forceinline__ uint32_t below_mask_of(uint32_t left) {
+  const uint32_t ne = (left + 1) >> 1, no = left >> 1;  // even / odd samples kept
+  return ((1u << ne) - 1u) | (((1u << no) - 1u) << 16);
+}
+
 // Write y (from a forward FFT of conjugated input: y[2m] = v.x, y[2m+1] =
 // -v.y, staged in LDS) as out[i] = y[(i + cut + shift) mod N] + dc for
 // i < len; stores with i >= len fall outside the descriptor and are dropped.
-__device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid, Lds& s) {
+//
+// words != nullptr (offset a multiple of 32): also the in-pulse bits of the
+// output, findpulses' y <= high as y <= highf (highf: the largest float not
+// above high).  A wavefront's elements m cover 128 consecutive y; per 32 of
+// them (one output word) bits 0..15 are the even samples' and 16..31 the
+// odd samples' flags, from two ballots; lanes 0..3 store the four words.
+__device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid, Lds& s,
+                                          uint32_t* words = nullptr, float highf = 0.0f) {
   const rsrc_t r = make_rsrc(o.out + base, (uint32_t)o.len * 4u);
   const LdsPair own(s.re, s.im, pidx(tid));
   const int off = o.cut + o.shift;
@@ -276,12 +293,23 @@ __device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid
   if (((off | o.len) & 1) == 0) {
     // even offset and length: y[2m], y[2m+1] land on an aligned pair that
     // is wholly inside or wholly outside [0, len): one 8-byte store
+    const int q = tid & 3;
     static_for<0, kP>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
       const float2 v = own.template ld<POff<kT * m>::value>();
       int n = 2 * (tid + kT * m);
       uint32_t i0 = (uint32_t)((n - off) & (kN - 1));
-      bstore2(r, make_float2(v.x + dc, dc - v.y), i0 * 4u, 0u);
+      const float y0 = v.x + dc, y1 = dc - v.y;
+      bstore2(r, make_float2(y0, y1), i0 * 4u, 0u);
+      if (words != nullptr) {
+        const unsigned long long e0 = __ballot(y0 <= highf), e1 = __ballot(y1 <= highf);
+        const uint32_t w = ((uint32_t)(e0 >> (16 * q)) & 0xffffu) | ((uint32_t)(e1 >> (16 * q)) << 16);
+        const uint32_t j0 = (uint32_t)((2 * (kT * m + (tid & ~63)) + 32 * q - off) & (kN - 1));
+        if ((tid & 63) < 4 && j0 < (uint32_t)o.len) {
+          const uint32_t left = (uint32_t)o.len - j0;
+          words[j0 >> 5] = left >= 32 ? w : (w & below_mask_of(left));
+        }
+      }
     });
     return;
   }
@@ -682,7 +710,8 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
     } else if (ph >= kPhPost) {
       stage_own(v, s, tidv);
       if (ph + 1 < last) load_half(a.post[ph + 1 - kPhPost].h, v, tidv);
-      c2r_store(a.post[ph - kPhPost], (int64_t)b * a.vstride, tidv, s);
+      c2r_store(a.post[ph - kPhPost], (int64_t)b * a.vstride, tidv, s,
+                ph - kPhPost == a.below_post ? a.below + (((int64_t)b * a.vstride) >> 5) : nullptr, a.below_highf);
     }
   }
 
@@ -1160,8 +1189,25 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.hpf.len = S;
   const int ids[4] = {LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT};
   float* outs[4] = {out->demod, out->demod_05, out->demod_burst, out->demod_pilot};
+  a.below_post = -1;
+  if (out->demod_05_below) {
+    if (!out->demod_05 || (nblocks > 1 && out->video_stride % 32 != 0)) {
+      set_error("ld_demod_blocks: demod_05_below needs demod_05 and a video stride that is a multiple of 32");
+      return LD_EINVAL;
+    }
+    if ((vcut + h->filt_shift[LD_FILT_VIDEO05]) % 32 != 0 || vlen % 2 != 0) {
+      set_error("ld_demod_blocks: demod_05_below needs demod_05's output offset to be a multiple of 32");
+      return LD_EINVAL;
+    }
+    a.below = out->demod_05_below;
+    // y <= highf exactly when (double)y <= below_high
+    float f = (float)out->below_high;
+    if ((double)f > out->below_high) f = nextafterf(f, -INFINITY);
+    a.below_highf = f;
+  }
   for (int i = 0; i < 4; ++i) {
     if (!outs[i]) continue;
+    if (i == 1 && out->demod_05_below) a.below_post = a.npost;
     C2rOut& o = a.post[a.npost++];
     o.h = h->filt[ids[i]];
     o.out = outs[i];

@@ -2,8 +2,8 @@
 // findpulses_numba_raw, utils.py:1101-1141):
 //   k_edge_count / k_edge_scan / k_edge_write   findpulses over a whole
 //       stitched buffer as an order-preserving stream compaction of the
-//       threshold crossings: per-tile counts, one exclusive scan, then each
-//       tile writes its edges at its offset
+//       threshold crossings: per-tile counts (and per-thread edge bits),
+//       one exclusive scan, then each tile writes its edges at its offset
 //   k_pulse_means   getpulses' vertical-sync levels: numpy's float32 mean of
 //       the body of every long pulse, one wavefront per pulse, the samples
 //       staged in LDS
@@ -64,14 +64,19 @@ __device__ __forceinline__ unsigned edge_bits(unsigned m, unsigned prev, long lo
   return e & 0xffffu;
 }
 
+// Per tile: the edge count, and every thread's edge bits for k_edge_write
+// (which then reads 2 bytes per 16 samples instead of the samples).
 __global__ __launch_bounds__(kEdgeThreads) void k_edge_count(const float* __restrict__ x, long long n, double high,
-                                                             long long* __restrict__ counts) {
+                                                             long long* __restrict__ counts,
+                                                             uint16_t* __restrict__ masks) {
   __shared__ int part[kEdgeThreads / 64];
   const int t = threadIdx.x;
   const long long base = (long long)blockIdx.x * kEdgeTile + (long long)t * kEdgePer;
   unsigned prev;
   const unsigned m = below_mask(x, n, base, high, prev);
-  int c = __popc(edge_bits(m, prev, n, base));
+  const unsigned e = edge_bits(m, prev, n, base);
+  masks[(long long)blockIdx.x * kEdgeThreads + t] = (uint16_t)e;
+  int c = __popc(e);
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((t & 63) == 0) part[t >> 6] = c;
   __syncthreads();
@@ -109,15 +114,13 @@ __global__ __launch_bounds__(kScanThreads) void k_edge_scan(long long* __restric
   if (t == kScanThreads - 1) *total = part[t];
 }
 
-__global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const float* __restrict__ x, long long n, double high,
+__global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const uint16_t* __restrict__ masks,
                                                              const long long* __restrict__ offsets,
                                                              long long* __restrict__ edges, long long cap) {
   __shared__ int part[kEdgeThreads];
   const int t = threadIdx.x;
   const long long base = (long long)blockIdx.x * kEdgeTile + (long long)t * kEdgePer;
-  unsigned prev;
-  const unsigned m = below_mask(x, n, base, high, prev);
-  unsigned e = edge_bits(m, prev, n, base);
+  unsigned e = masks[(long long)blockIdx.x * kEdgeThreads + t];
   const int c = __popc(e);
   part[t] = c;
   __syncthreads();
@@ -130,6 +133,74 @@ __global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const float* __rest
   long long k = offsets[blockIdx.x] + part[t] - c;
   while (e) {
     const int i = __ffs(e) - 1;
+    e &= e - 1;
+    if (k < cap) edges[k] = base + i;
+    ++k;
+  }
+}
+
+// The same compaction from in-pulse bits, 64 samples per thread: a tile
+// covers kBitTile samples.  Word w holds samples 32 w .. 32 w + 31 paired:
+// bit j < 16 is sample 32 w + 2 j, bit 16 + j sample 32 w + 2 j + 1.
+constexpr int kBitTile = kEdgeThreads * 64;
+
+__device__ __forceinline__ uint32_t spread16(uint32_t x) {
+  x = (x | (x << 8)) & 0x00ff00ffu;
+  x = (x | (x << 4)) & 0x0f0f0f0fu;
+  x = (x | (x << 2)) & 0x33333333u;
+  return (x | (x << 1)) & 0x55555555u;
+}
+// sample-ordered bits of a paired word
+__device__ __forceinline__ uint32_t unpair(uint32_t w) { return spread16(w & 0xffffu) | (spread16(w >> 16) << 1); }
+
+__device__ __forceinline__ unsigned long long bit_edges(const uint32_t* __restrict__ bits, long long n,
+                                                        long long base) {
+  const long long w = base >> 5;
+  const uint2 q = *reinterpret_cast<const uint2*>(bits + w);
+  const unsigned long long m = (unsigned long long)unpair(q.x) | ((unsigned long long)unpair(q.y) << 32);
+  // sample 32 w - 1 is the previous word's last odd sample: its bit 31
+  const unsigned long long prev = base > 0 ? (bits[w - 1] >> 31) : 0ull;
+  unsigned long long e = m ^ ((m << 1) | prev);
+  const long long left = n - base;
+  if (left < 64) e &= left > 0 ? ((1ull << left) - 1ull) : 0ull;
+  return e;
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_bits_count(const uint32_t* __restrict__ bits, long long n,
+                                                             long long* __restrict__ counts) {
+  __shared__ int part[kEdgeThreads / 64];
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * kBitTile + (long long)t * 64;
+  int c = base < n ? __popcll(bit_edges(bits, n, base)) : 0;
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
+  if ((t & 63) == 0) part[t >> 6] = c;
+  __syncthreads();
+  if (t == 0) {
+    long long s = 0;
+    for (int w = 0; w < kEdgeThreads / 64; ++w) s += part[w];
+    counts[blockIdx.x] = s;
+  }
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_bits_write(const uint32_t* __restrict__ bits, long long n,
+                                                             const long long* __restrict__ offsets,
+                                                             long long* __restrict__ edges, long long cap) {
+  __shared__ int part[kEdgeThreads];
+  const int t = threadIdx.x;
+  const long long base = (long long)blockIdx.x * kBitTile + (long long)t * 64;
+  unsigned long long e = base < n ? bit_edges(bits, n, base) : 0ull;
+  const int c = __popcll(e);
+  part[t] = c;
+  __syncthreads();
+  for (int o = 1; o < kEdgeThreads; o <<= 1) {
+    const int v = t >= o ? part[t - o] : 0;
+    __syncthreads();
+    part[t] += v;
+    __syncthreads();
+  }
+  long long k = offsets[blockIdx.x] + part[t] - c;
+  while (e) {
+    const int i = __ffsll((long long)e) - 1;
     e &= e - 1;
     if (k < cap) edges[k] = base + i;
     ++k;
@@ -321,7 +392,7 @@ using namespace ldgpu;
 
 extern "C" size_t ld_pulse_workspace_bytes(const ld_handle*, int64_t n) {
   const int64_t tiles = n > 0 ? (n + kEdgeTile - 1) / kEdgeTile : 1;
-  return (size_t)(tiles + 1) * sizeof(long long);
+  return (size_t)(tiles + 1) * sizeof(long long) + (size_t)tiles * kEdgeThreads * sizeof(uint16_t);
 }
 
 extern "C" int ld_find_pulses(ld_handle* h, const float* x, int64_t n, double high, int64_t* edges, int64_t cap,
@@ -346,11 +417,51 @@ extern "C" int ld_find_pulses(ld_handle* h, const float* x, int64_t n, double hi
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
   long long* counts = (long long*)workspace;
-  hipLaunchKernelGGL(k_edge_count, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, x, (long long)n, high, counts);
+  uint16_t* masks = (uint16_t*)(counts + tiles + 1);
+  hipLaunchKernelGGL(k_edge_count, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, x, (long long)n, high, counts,
+                     masks);
   LD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_edge_scan, dim3(1), dim3(kScanThreads), 0, st, counts, tiles, (long long*)count);
   LD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_edge_write, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, x, (long long)n, high,
+  hipLaunchKernelGGL(k_edge_write, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, (const uint16_t*)masks,
+                     (const long long*)counts, (long long*)edges, (long long)cap);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+extern "C" size_t ld_pulse_bits_workspace_bytes(const ld_handle*, int64_t n) {
+  const int64_t tiles = n > 0 ? (n + kBitTile - 1) / kBitTile : 1;
+  return (size_t)(tiles + 1) * sizeof(long long);
+}
+
+extern "C" int ld_find_pulses_bits(ld_handle* h, const uint32_t* bits, int64_t n, int64_t* edges, int64_t cap,
+                                   int64_t* count, void* workspace, size_t ws_bytes, void* stream) {
+  if (!h || !bits || n <= 0 || !count || cap < 0 || (cap > 0 && !edges) || !workspace) {
+    set_error("ld_find_pulses_bits: bad arguments");
+    return LD_EINVAL;
+  }
+  if (((uintptr_t)bits & 7) != 0) {
+    set_error("ld_find_pulses_bits: bits must be 8-byte aligned");
+    return LD_EINVAL;
+  }
+  if (ws_bytes < ld_pulse_bits_workspace_bytes(h, n)) {
+    set_error("ld_find_pulses_bits: workspace %zu bytes < %zu needed", ws_bytes,
+              ld_pulse_bits_workspace_bytes(h, n));
+    return LD_EINVAL;
+  }
+  const long long tiles = (n + kBitTile - 1) / kBitTile;
+  if (tiles > 0x7fffffffLL) {
+    set_error("ld_find_pulses_bits: buffer too long");
+    return LD_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  long long* counts = (long long*)workspace;
+  hipLaunchKernelGGL(k_bits_count, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, bits, (long long)n, counts);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_edge_scan, dim3(1), dim3(kScanThreads), 0, st, counts, tiles, (long long*)count);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bits_write, dim3((unsigned)tiles), dim3(kEdgeThreads), 0, st, bits, (long long)n,
                      (const long long*)counts, (long long*)edges, (long long)cap);
   LD_HIP(hipGetLastError());
   return LD_OK;

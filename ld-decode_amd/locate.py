@@ -86,17 +86,24 @@ class PulseFinder:
         self.h_means = torch.empty(self.cap // 2, dtype=torch.float32, pin_memory=True)
         self.done = None
 
-    def launch(self, demod05, n=None, stream=None, high=None):
+    def launch(self, demod05, n=None, stream=None, high=None, bits=None):
         """Enqueue the pulse search, the sync levels and their copies to the
-        host on `stream` (default: current)."""
+        host on `stream` (default: current).  bits: (int32 words, threshold)
+        of the demod's in-pulse bits over demod05; used when that threshold
+        is this search's."""
         torch, rf = self.torch, self.rf
         n = int(n if n is not None else demod05.numel())
         assert n <= self.n and demod05.dtype == torch.float32 and demod05.is_contiguous()
         st = stream if stream is not None else torch.cuda.current_stream(rf.device)
         sp = ctypes.c_void_p(st.cuda_stream)
         hi = pulse_threshold(rf) if high is None else high
-        _lib.check(rf.lib.ld_find_pulses(rf.handle, demod05.data_ptr(), n, float(hi), self.edges.data_ptr(),
-                                         self.cap, self.count.data_ptr(), self.ws.data_ptr(), self.ws.numel(), sp))
+        if bits is not None and float(bits[1]) == float(hi):
+            _lib.check(rf.lib.ld_find_pulses_bits(rf.handle, bits[0].data_ptr(), n, self.edges.data_ptr(), self.cap,
+                                                  self.count.data_ptr(), self.ws.data_ptr(), self.ws.numel(), sp))
+        else:
+            _lib.check(rf.lib.ld_find_pulses(rf.handle, demod05.data_ptr(), n, float(hi), self.edges.data_ptr(),
+                                             self.cap, self.count.data_ptr(), self.ws.data_ptr(), self.ws.numel(),
+                                             sp))
         _lib.check(rf.lib.ld_pulse_means(rf.handle, demod05.data_ptr(), n, self.edges.data_ptr(),
                                          self.count.data_ptr(), self.cap // 2,
                                          10.0 * ((rf.freq / rf.linelen) * rf.linelen),    # usectoinpx(10)
@@ -308,8 +315,8 @@ class FieldWalker:
         self.prov_delta = 0.0
         self.bufs = WalkBuffers(rf, max_fields + 16)
 
-    def pulses(self, demod05, stream=None):
-        self.finder.launch(demod05, self.total, stream)
+    def pulses(self, demod05, stream=None, bits=None):
+        self.finder.launch(demod05, self.total, stream, bits=bits)
 
     def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None):
         """Returns (locs, linelocs1, linebad, prev_out, next start,

@@ -293,11 +293,15 @@ class RFDecode:
         return out
 
     def demod_blocks(self, data, first=0, nblocks=None, mtf_level=0, cut=True, step=None,
-                     audio=None, channels=None, out=None, stream=None, _raw_level=False):
+                     audio=None, channels=None, out=None, stream=None, _raw_level=False, below=None,
+                     below_high=None):
         """Demodulate blocks b = 0..nblocks-1 reading data[first + b*step : + blocklen].
 
         With cut=True and the default step (blocklen - 1056) the outputs are the
         stitched DemodCache arrays: element i is raw sample first + 1024 + i.
+        below: int32 device words receiving demod_05's in-pulse bits against
+        below_high (the input of ld_find_pulses_bits), 2 * ceil(n / 64)
+        words for n stitched samples.
         Returns {'video': Channels, 'rfhpf': tensor, 'audio': Channels}."""
         torch = self.torch
         x, dt, nsamp = self._as_device_input(data)
@@ -345,6 +349,11 @@ class RFDecode:
             raise ValueError("uncut output with several blocks: call per block")
         o.video_stride = vlen
         o.audio_stride = alen
+        if below is not None and "demod_05" in res:
+            if below.dtype != torch.int32 or below.numel() < 2 * -(-(nblocks * vlen) // 64):
+                raise ValueError("in-pulse bit buffer too small or not int32")
+            o.demod_05_below = below.data_ptr()
+            o.below_high = float(below_high)
         ws = self._workspace(nblocks)
         st = stream if stream is not None else self._stream()
         _lib.check(self.lib.ld_demod_blocks(self.handle, x.data_ptr(), dt, nsamp, first, step, nblocks,

@@ -89,6 +89,75 @@ def test_pulse_edges_adversarial(gpu_rf):
             np.testing.assert_array_equal(e, _edges_np(x, high))
 
 
+def _pack_bits(below):
+    """Paired in-pulse words, 2 * ceil(n / 64) int32 of them: bit j < 16 of
+    word w = below[32 w + 2 j], bit 16 + j = below[32 w + 2 j + 1]."""
+    n = len(below)
+    nw = 2 * -(-n // 64)
+    b = np.zeros(nw * 32, np.uint8)
+    b[:n] = below
+    b = b.reshape(-1, 32)
+    b = np.concatenate([b[:, 0::2], b[:, 1::2]], axis=1)     # bit order within the word
+    return np.packbits(b[:, ::-1], axis=1).view(">u4").astype(np.uint32).view(np.int32).ravel()
+
+
+def _pulses_bits_gpu(rf, words, n):
+    import torch
+    wt = torch.from_numpy(np.ascontiguousarray(words)).cuda()
+    cap = n + 2
+    edges = torch.empty(cap, dtype=torch.int64, device="cuda")
+    count = torch.zeros(1, dtype=torch.int64, device="cuda")
+    ws = torch.empty(rf.lib.ld_pulse_bits_workspace_bytes(rf.handle, n), dtype=torch.uint8, device="cuda")
+    _lib.check(rf.lib.ld_find_pulses_bits(rf.handle, wt.data_ptr(), n, edges.data_ptr(), cap, count.data_ptr(),
+                                          ws.data_ptr(), ws.numel(), None))
+    c = int(count.cpu()[0])
+    return edges[:c].cpu().numpy()
+
+
+def test_pulse_bits_edges(gpu_rf):
+    """ld_find_pulses_bits against the same edges from the samples: word and
+    tile boundaries, a buffer starting inside a pulse, a run reaching the
+    end, lengths around 32 / 64 / a tile; bits past n are ignored."""
+    rf = gpu_rf("NTSC", audio=False)
+    rng = np.random.default_rng(11)
+    for n in (1, 5, 31, 32, 33, 63, 64, 65, 16383, 16384, 16385, 100_003):
+        for density in (0.05, 0.5):
+            below = rng.random(n) < density
+            below[0] = True
+            if n > 40:
+                below[31:33] = (False, True)      # an edge on a word boundary
+                below[-5:] = True                 # a run reaching the end
+            words = _pack_bits(below)
+            junk = np.zeros(len(words) * 32, bool)
+            junk[n:] = rng.random(len(junk) - n) < 0.5
+            words |= _pack_bits(junk)[:len(words)]
+            x = np.where(below, -1.0, 1.0).astype(np.float32)
+            np.testing.assert_array_equal(_pulses_bits_gpu(rf, words, n), _edges_np(x, 0.0), err_msg=str(n))
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_demod_in_pulse_bits(gpu_rf, system):
+    """The demod's fused in-pulse bits equal demod_05 <= threshold (in
+    double) for every stitched sample, and the bit search gives the sample
+    search's edges."""
+    import torch
+    rf = gpu_rf(system, audio=False)
+    fl = 667_334 if system == "NTSC" else 800_000
+    cap = synth.make_capture(system, 3 * fl, seed=4, snr_db=40.0, jitter_ns=50, audio=False)
+    dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).cuda(), locate=True, max_fields=4)
+    dec.demod()
+    torch.cuda.synchronize()
+    d05 = dec.buffers["demod_05"].cpu().numpy()
+    hi = locate.pulse_threshold(rf)
+    words = dec.buffers["demod_05_below"].cpu().numpy()
+    nw = -(-len(d05) // 32)                   # words the demod writes (the rest is padding)
+    np.testing.assert_array_equal(words[:nw], _pack_bits(d05.astype(np.float64) <= hi)[:nw])
+    e_bits = _pulses_bits_gpu(rf, words, len(d05))
+    np.testing.assert_array_equal(e_bits, _edges_np(d05, hi))
+    e_float, _ = _pulses_gpu(rf, d05, hi)
+    np.testing.assert_array_equal(e_bits, e_float)
+
+
 def test_range_means_exact(gpu_rf):
     import torch
     rf = gpu_rf("NTSC", audio=False)
