@@ -70,8 +70,9 @@ class CaptureDecoder:
         self.buffers = {n: torch.empty(self.total, dtype=torch.float32, device=dev) for n in names}
         if rf.decode_digital_audio:
             self.buffers["efm"] = torch.empty(self.total, dtype=torch.int16, device=dev)
-        if locate:
+        if locate and S % 32 == 0 and (rf.blockcut + int(rf.Filters["F05_offset"])) % 32 == 0:
             # demod_05's in-pulse bits, written by the demod for the pulse search
+            # (whole 32-sample words per block; otherwise the search reads the samples)
             self.buffers["demod_05_below"] = torch.empty(2 * -(-self.total // 64), dtype=torch.int32, device=dev)
         self._below_high = None
         self.audio = bool(audio)
