@@ -146,7 +146,10 @@ __device__ __forceinline__ double get_linelen(const double* ll, int line, int li
   return length <= 0 ? linelen : length;
 }
 
-// compare-exchange network sort of 16 values (ascending)
+// Bitonic network sort of 16 values (ascending).  Each compare-exchange is
+// a min and a max (two FP64 instructions); for values without NaN or -0.0,
+// which is what both callers sort, the result is the same sequence as a
+// compare-and-swap network's.
 __device__ __forceinline__ void sort16(double (&v)[kZc]) {
 #pragma unroll
   for (int k = 2; k <= kZc; k <<= 1) {
@@ -157,10 +160,9 @@ __device__ __forceinline__ void sort16(double (&v)[kZc]) {
         const int l = i ^ j;
         if (l > i) {
           const bool up = (i & k) == 0;
-          const double a = v[i], b = v[l];
-          const bool sw = up ? (a > b) : (a < b);
-          v[i] = sw ? b : a;
-          v[l] = sw ? a : b;
+          const double lo = fmin(v[i], v[l]), hi = fmax(v[i], v[l]);
+          v[i] = up ? lo : hi;
+          v[l] = up ? hi : lo;
         }
       }
     }
