@@ -119,6 +119,27 @@ def test_tbc_asymmetric_lut(gpu_rf, oracle_rf):
         fieldmod.upload_sinc_lut(rf)
 
 
+def test_tbc_distorted_lines(gpu_rf, oracle_rf):
+    """Line locations that stretch some lines past a chunk's LDS stage (the
+    taps of those chunks come from global memory) and one line that runs
+    backwards: float32 output equals the oracle's bit for bit in both
+    coordinate modes."""
+    import torch
+    rf, orf = gpu_rf("NTSC", audio=False), oracle_rf("NTSC", audio=False)
+    rng = np.random.default_rng(33)
+    demod = (8.1e6 + rng.normal(0, 3e5, 800_000)).astype(np.float32)
+    ll = 5000 + np.arange(273) * 2542.3 + rng.uniform(-2, 2, 273)
+    ll[100:] += 3000          # line 99: ~6.1 samples per pixel (1560 per 256-pixel chunk)
+    ll[180:] += 15000         # line 179: ~7 times the usual length
+    ll[150] = ll[149] - 50.0  # line 149 runs backwards, line 150 twice as long
+    for coord_f32 in (True, False):
+        ds_o, _ = _oracle_field(orf, demod, ll, 0, coord_f32)
+        f = fieldmod.Field(rf, {"video": {"demod": torch.from_numpy(demod).cuda()}}, linelocs=ll,
+                           coord_f32=coord_f32)
+        ds = _np(f.downscale(final=False)[0])
+        np.testing.assert_array_equal(ds, ds_o.astype(np.float32))
+
+
 def test_batched_fields_equal_single(gpu_rf):
     """Three fields in one tbc_fields launch equal three single-field calls."""
     import torch
