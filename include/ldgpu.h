@@ -18,7 +18,15 @@
  *     pointers.  `stream` is a hipStream_t (NULL = default stream).
  *   - Handles are not internally locked: use one handle per host thread (the
  *     reference gives each DemodCache worker thread its own RFDecode,
- *     core.py:1165).  Calls only enqueue work on `stream`.
+ *     core.py:1165).  Calls that take a `stream` only enqueue work on it.
+ *   - The table setters (ld_set_filter, ld_set_audio, ld_set_audio2,
+ *     ld_set_sinc_lut) are synchronous host calls that never write a table
+ *     a queued launch may read: each upload goes to a fresh device buffer,
+ *     launches queued earlier (on any stream) keep the table they were
+ *     given, and launches queued after the call use the new one.  Replaced
+ *     tables are freed after a device synchronisation once 32 have piled up,
+ *     and by ld_destroy.  The scalar setters only change what later launches
+ *     are given.
  */
 #ifndef LDGPU_H
 #define LDGPU_H

@@ -842,6 +842,31 @@ __global__ __launch_bounds__(kT) void k_audio2_block(Audio2Args a) {
   }
 }
 
+constexpr size_t kMaxRetired = 32;  // retired tables kept before a synchronising free
+
+int replace_table(ld_handle* h, void** slot, const void* host, size_t bytes) {
+  LD_HIP(hipSetDevice(h->device));
+  void* fresh = nullptr;
+  LD_HIP(hipMalloc(&fresh, bytes));
+  hipError_t e = hipMemcpy(fresh, host, bytes, hipMemcpyHostToDevice);
+  if (e != hipSuccess) {
+    (void)hipFree(fresh);
+    return hip_fail(e, "table upload");
+  }
+  if (*slot != nullptr) h->retired.push_back(*slot);
+  *slot = fresh;
+  if (h->retired.size() > kMaxRetired) free_retired(h);
+  return LD_OK;
+}
+
+void free_retired(ld_handle* h) {
+  if (h->retired.empty()) return;
+  (void)hipSetDevice(h->device);
+  (void)hipDeviceSynchronize();
+  for (void* p : h->retired) (void)hipFree(p);
+  h->retired.clear();
+}
+
 }  // namespace ldgpu
 
 using namespace ldgpu;
@@ -900,6 +925,7 @@ int ld_create(int device, int blocklen, ld_handle** out) {
 
 void ld_destroy(ld_handle* h) {
   if (!h) return;
+  free_retired(h);
   hipSetDevice(h->device);
   hipFree(h->twiddle);
   hipFree(h->sinc_lut);
@@ -909,13 +935,10 @@ void ld_destroy(ld_handle* h) {
   delete h;
 }
 
-static int upload_complex(float2** dst, const double* reim, int n, double scale, int device) {
+static int upload_complex(ld_handle* h, float2** dst, const double* reim, int n, double scale) {
   std::vector<float2> tmp(n);
   for (int i = 0; i < n; ++i) tmp[i] = make_float2((float)(reim[2 * i] * scale), (float)(reim[2 * i + 1] * scale));
-  LD_HIP(hipSetDevice(device));
-  if (*dst == nullptr) LD_HIP(hipMalloc(dst, sizeof(float2) * n));
-  LD_HIP(hipMemcpy(*dst, tmp.data(), sizeof(float2) * n, hipMemcpyHostToDevice));
-  return LD_OK;
+  return replace_table(h, reinterpret_cast<void**>(dst), tmp.data(), sizeof(float2) * n);
 }
 
 int ld_set_filter(ld_handle* h, int id, const double* reim, int nbins, double scale, double dc_gain,
@@ -932,7 +955,7 @@ int ld_set_filter(ld_handle* h, int id, const double* reim, int nbins, double sc
     set_error("ld_set_filter: shift %d out of range", shift);
     return LD_EINVAL;
   }
-  int rc = upload_complex(&h->filt[id], reim, nbins, scale, h->device);
+  int rc = upload_complex(h, &h->filt[id], reim, nbins, scale);
   if (rc) return rc;
   h->filt_dc[id] = dc_gain;
   h->filt_shift[id] = shift;
@@ -948,7 +971,7 @@ int ld_set_audio2(ld_handle* h, int ch, const double* reim, int nbins, double ce
     set_error("ld_set_audio2: nbins %d != blocklen/2+1 = %d", nbins, h->blocklen / 2 + 1);
     return LD_EINVAL;
   }
-  int rc = upload_complex(&h->audio2_filt[ch], reim, nbins, 1.0 / (h->blocklen / 2), h->device);
+  int rc = upload_complex(h, &h->audio2_filt[ch], reim, nbins, 1.0 / (h->blocklen / 2));
   if (rc) return rc;
   h->audio2_center[ch] = center_freq;
   return LD_OK;
@@ -1078,7 +1101,7 @@ int ld_set_audio(ld_handle* h, int ch, const double* reim, int nbins, int lowbin
     set_error("ld_set_audio: lowbin %d outside the half spectrum", lowbin);
     return LD_EINVAL;
   }
-  int rc = upload_complex(&h->audio_filt[ch], reim, nbins, 1.0 / nbins, h->device);
+  int rc = upload_complex(h, &h->audio_filt[ch], reim, nbins, 1.0 / nbins);
   if (rc) return rc;
   h->audio_nbins = nbins;
   h->audio_lowbin[ch] = lowbin;

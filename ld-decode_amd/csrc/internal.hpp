@@ -4,6 +4,8 @@
 #include <stdarg.h>
 #include <stdio.h>
 
+#include <vector>
+
 #include "../../include/ldgpu.h"
 
 struct ld_handle {
@@ -29,6 +31,9 @@ struct ld_handle {
   double clip_lo = 1.5e6, clip_hi = 30e6, dc_offset = 0.0;
   int video_rot = 0;
   bool params_set = false;
+  // tables replaced by a later upload; launches queued before the upload
+  // still read them, so they are freed only after a device synchronisation
+  std::vector<void*> retired;
 };
 
 namespace ldgpu {
@@ -41,6 +46,15 @@ int hip_fail(hipError_t e, const char* what);
     hipError_t e_ = (call);                             \
     if (e_ != hipSuccess) return ldgpu::hip_fail(e_, #call); \
   } while (0)
+
+// Upload `bytes` of host data as a device table into *slot.  A table that
+// already exists is never overwritten in place: the data goes to a fresh
+// buffer (synchronous copy, read by no launch yet) and the old one is
+// retired, so kernels queued earlier on any stream keep reading the table
+// they were launched with.
+int replace_table(ld_handle* h, void** slot, const void* host, size_t bytes);
+// Free retired tables (synchronises the device first).
+void free_retired(ld_handle* h);
 
 constexpr int kSincTaps = 16;
 constexpr int kSincPhases = 1 << 16;

@@ -483,8 +483,11 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
       int rc = 0;
       if (count == 0) {
         pa = p0;  // clb_findbursts leaves phase_adjust unchanged
-      } else if (minbase > 0.0 && minbase + p0 + 0.5 > 0.0) {
-        // step-table lookup: p0 = q + t, t in [0, 1)
+      } else if (minbase > 0.0 && minbase + p0 >= 0.0) {
+        // step-table lookup: p0 = q + t, t in [0, 1).  The table holds the
+        // passes in the regime where int32 truncation equals floor; pass 2
+        // starts from p0 + median with median > -0.5, so every rounding
+        // term of both passes stays positive only if minbase + p0 >= 0.
         const double q = floor(p0);
         const double t = p0 - q;
         int idx = 0;

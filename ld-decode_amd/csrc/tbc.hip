@@ -667,10 +667,9 @@ int ld_set_sinc_lut(ld_handle* h, const float* host_lut, int rows, int taps) {
     set_error("ld_set_sinc_lut: expected a %d x %d float32 table", kSincPhases + 1, kSincTaps);
     return LD_EINVAL;
   }
-  LD_HIP(hipSetDevice(h->device));
   size_t bytes = sizeof(float) * (size_t)rows * taps;
-  if (!h->sinc_lut) LD_HIP(hipMalloc(&h->sinc_lut, bytes));
-  LD_HIP(hipMemcpy(h->sinc_lut, host_lut, bytes, hipMemcpyHostToDevice));
+  int rc = replace_table(h, reinterpret_cast<void**>(&h->sinc_lut), host_lut, bytes);
+  if (rc) return rc;
   // A Kaiser-windowed sinc table is mirror-symmetric: the weights of phase
   // p/65536 at distances (7-t)+p/65536 equal those of phase 1-p/65536 at the
   // negated distances.  When every row holds that bitwise (and the last row

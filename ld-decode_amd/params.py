@@ -39,8 +39,9 @@ SYS_NTSC = {
     "vsyncPulseUS": 27.1,
     "outputZero": 1024,
     "fieldPhases": 4,
-    # 227.5 colour cycles per line (core.py:119)
-    "line_period": 227.5 / _NTSC_FSC,
+    # 227.5 colour cycles per line, formed as the reference forms it
+    # (core.py:119): 1 / (fsc / 227.5) is one ulp below 227.5 / fsc
+    "line_period": 1 / (_NTSC_FSC / 227.5),
 }
 SYS_NTSC["activeVideoUS"] = (9.45, SYS_NTSC["line_period"] - 1.0)
 SYS_NTSC["FPS"] = 1000000 / (525 * SYS_NTSC["line_period"])

@@ -248,6 +248,45 @@ def test_pipelined_located_equals_sequential(gpu_rf):
         np.testing.assert_array_equal(d.out_u16[:n].cpu().numpy(), want[:n].cpu().numpy())
 
 
+def test_pipelined_alternating_mtf_matches_sequential(gpu_rf):
+    """Consecutive pipelined passes at alternating MTF levels (the reference
+    changes the level per field, core.py:3841-3852): each pass's RFVideo
+    table is uploaded while the previous passes may still run, and every
+    returned set must equal a sequential decode at its own level, checked
+    before the next step() call reuses a set."""
+    import torch
+    rf = gpu_rf("NTSC", audio=False)
+    cap, _ = _demod("NTSC")
+    x = torch.from_numpy(cap.data).cuda()
+    levels = (0.0, 1.0)
+    want, plan = {}, {}
+    for lv in levels:
+        seq = capture.CaptureDecoder(rf, x, locate=True, max_fields=8)
+        want[lv] = seq.step(mtf_level=lv).clone()
+        torch.cuda.synchronize()
+        plan[lv] = [f["demod_offset"] for f in seq.fields]
+    n = len(plan[0.0]) * seq.linesout * seq.W
+    assert not torch.equal(want[0.0][:n], want[1.0][:n]), "the MTF level must change the output"
+    pipe = capture.PipelinedDecoder(rf, x, locate=True, max_fields=8)
+    checked = 0
+    for k in range(7):
+        d = pipe.step(mtf_level=levels[k % 2])
+        if d is None:
+            continue
+        torch.cuda.synchronize()
+        lv = levels[(k - 2) % 2]          # step k returns pass k - 2
+        assert [f["demod_offset"] for f in d.fields] == plan[lv]
+        np.testing.assert_array_equal(d.out_u16[:n].cpu().numpy(), want[lv][:n].cpu().numpy())
+        checked += 1
+    rest = pipe.join()
+    torch.cuda.synchronize()
+    for j, d in enumerate(rest):
+        lv = levels[(7 - 2 + j) % 2]
+        np.testing.assert_array_equal(d.out_u16[:n].cpu().numpy(), want[lv][:n].cpu().numpy())
+        checked += 1
+    assert checked == 7
+
+
 @pytest.mark.parametrize("system", ["NTSC", "PAL"])
 def test_field_compute_linelocs(gpu_rf, system):
     """Field.compute_linelocs on one field buffer (the reference's per-field

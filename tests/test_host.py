@@ -22,10 +22,17 @@ ROOT = __import__("pathlib").Path(__file__).resolve().parents[1]
 def test_param_tables_match_oracle(system):
     sp, dp = params.system_tables(system)
     osp, odp = sys_params(system), filter_params(system)
+    # bit-equal: the tables feed the audio tick grid and the field lengths
     for k, v in osp.items():
-        assert np.allclose(np.asarray(sp[k], float), np.asarray(v, float), rtol=1e-15, atol=0), k
+        np.testing.assert_array_equal(np.asarray(sp[k], float), np.asarray(v, float), err_msg=k)
     for k, v in odp.items():
         assert dp[k] == v, k
+
+
+def test_ntsc_line_period_is_the_references_expression():
+    # core.py:119 forms 1 / (fsc / 227.5); 227.5 / fsc differs in the last bit
+    sp, _ = params.system_tables("NTSC")
+    assert sp["line_period"] == 1 / ((315.0 / 88.0) / 227.5) == 63.55555555555555
 
 
 @pytest.mark.parametrize("system", ["NTSC", "PAL"])
