@@ -171,6 +171,23 @@ int ld_demod_blocks(ld_handle* h, const void* input, int input_dtype, int64_t in
                     const ld_demod_outputs* out, void* workspace, size_t workspace_bytes,
                     void* stream);
 
+/* No counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under HIP graph capture, tight serving loops, harnesses that
+ * forbid hipMalloc): ld_demod_blocks with its workspace taken from a static
+ * device array of ld_demod_noalloc_max_bytes() bytes (32 MiB) that every GPU
+ * loading the library holds until the process ends; nothing is allocated or
+ * freed.  The caller still passes the input and the outputs.  nblocks is
+ * limited to what fits the array (ld_demod_workspace_bytes(h, nblocks) <=
+ * 32 MiB: 256 blocks of 32768); larger requests return LD_EINVAL.  Nothing in
+ * the array needs to start at zero (every block writes its workspace before
+ * reading it).  All calls on one GPU share the array: calls that overlap on
+ * one GPU (concurrent streams, host threads) race and must be serialised by
+ * the caller. */
+size_t ld_demod_noalloc_max_bytes(void);
+int ld_demod_blocks_noalloc(ld_handle* h, const void* input, int input_dtype, int64_t input_len,
+                            int64_t first, int64_t step, int nblocks, int cut,
+                            const ld_demod_outputs* out, void* stream);
+
 /* Upload the 16-tap sinc interpolation table: (65536 + 1) rows x 16 float32
  * (row p = Kaiser(beta=5)-windowed sinc weights at phase p/65536; the extra
  * row repeats the last), as lddecode/sinc_lut.npz (utils.py:73-124).

@@ -28,6 +28,7 @@ EXPORTS = (
     "ld_skip_score", "ld_locate_recheck", "ld_unpack_10bit",
     "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
     "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
+    "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -163,6 +164,9 @@ def _declare(lib):
         "ld_demod_workspace_bytes": (c_sz, [c_vp, c_int]),
         "ld_demod_blocks": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
                                     P(DemodOutputs), c_vp, c_sz, c_vp]),
+        "ld_demod_noalloc_max_bytes": (c_sz, []),
+        "ld_demod_blocks_noalloc": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
+                                            P(DemodOutputs), c_vp]),
         "ld_set_sinc_lut": (c_int, [c_vp, P(ctypes.c_float), c_int, c_int]),
         "ld_tbc_workspace_bytes": (c_sz, [c_vp, c_int]),
         "ld_tbc_fields": (c_int, [c_vp, c_vp, c_i64, P(TbcField), c_int, c_vp, c_i64, c_int, c_dbl, c_dbl,

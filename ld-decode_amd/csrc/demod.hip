@@ -844,6 +844,33 @@ __global__ __launch_bounds__(kT) void k_audio2_block(Audio2Args a) {
 
 constexpr size_t kMaxRetired = 32;  // retired tables kept before a synchronising free
 
+// Scratch of the no-allocation demod entry point: the per-block workspace
+// (even analytic samples) of up to kScratchBytes, one copy per GPU that
+// loads the library, held until the process ends.  Host code uses the
+// address hipGetSymbolAddress returns on each device, never the array's name.
+constexpr size_t kScratchBytes = (size_t)32 << 20;
+constexpr int kMaxDevices = 64;
+static __device__ __align__(256) unsigned char scratch[kScratchBytes];
+static std::mutex g_scratch_mu;
+static void* g_scratch_ptr[kMaxDevices] = {};
+
+// The scratch address on `device` (current), looked up once per device.
+static int scratch_address(int device, void** out) {
+  std::lock_guard<std::mutex> lock(g_scratch_mu);
+  if (device < 0 || device >= kMaxDevices) {
+    set_error("device %d outside [0, %d)", device, kMaxDevices);
+    return LD_EINVAL;
+  }
+  if (g_scratch_ptr[device] == nullptr) {
+    void* ptr = nullptr;
+    hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+    if (e != hipSuccess) return hip_fail(e, "hipGetSymbolAddress(scratch)");
+    g_scratch_ptr[device] = ptr;
+  }
+  *out = g_scratch_ptr[device];
+  return LD_OK;
+}
+
 int replace_table(ld_handle* h, void** slot, const void* host, size_t bytes) {
   LD_HIP(hipSetDevice(h->device));
   void* fresh = nullptr;
@@ -889,8 +916,11 @@ int ld_create(int device, int blocklen, ld_handle** out) {
     return LD_EINVAL;
   }
   LD_HIP(hipSetDevice(device));
+  void* scr = nullptr;
+  if (int rc = scratch_address(device, &scr)) return rc;
   ld_handle* h = new ld_handle();
   h->device = device;
+  h->scratch = scr;
   h->blocklen = blocklen;
   // [0, TwLayout::total): Stockham pass rows (TwLayout), then the TwLds
   // image (pass rows and factor tables), then the split twiddles
@@ -1288,6 +1318,29 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
     hipLaunchKernelGGL(a.efm ? k_demod_block_efm : k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
+}
+
+size_t ld_demod_noalloc_max_bytes(void) { return kScratchBytes; }
+
+int ld_demod_blocks_noalloc(ld_handle* h, const void* input, int dtype, int64_t input_len, int64_t first,
+                            int64_t step, int nblocks, int cut, const ld_demod_outputs* out, void* stream) {
+  if (!h || nblocks <= 0) {
+    set_error("ld_demod_blocks_noalloc: null handle or nblocks <= 0");
+    return LD_EINVAL;
+  }
+  if (h->scratch == nullptr) {
+    set_error("ld_demod_blocks_noalloc: the handle has no scratch address");
+    return LD_ESTATE;
+  }
+  const size_t need = ld_demod_workspace_bytes(h, nblocks);
+  if (need > kScratchBytes) {
+    set_error("ld_demod_blocks_noalloc: %d blocks need %zu workspace bytes, over the %zu-byte scratch "
+              "(at most %zu blocks of %d)", nblocks, need, kScratchBytes,
+              kScratchBytes / ld_demod_workspace_bytes(h, 1), h->blocklen);
+    return LD_EINVAL;
+  }
+  return ld_demod_blocks(h, input, dtype, input_len, first, step, nblocks, cut, out, h->scratch, kScratchBytes,
+                         stream);
 }
 
 }  // extern "C"

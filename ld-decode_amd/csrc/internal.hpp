@@ -34,6 +34,8 @@ struct ld_handle {
   // tables replaced by a later upload; launches queued before the upload
   // still read them, so they are freed only after a device synchronisation
   std::vector<void*> retired;
+  // this device's address of the no-allocation entry point's scratch array
+  void* scratch = nullptr;
 };
 
 namespace ldgpu {
