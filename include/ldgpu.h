@@ -171,6 +171,33 @@ int ld_demod_blocks(ld_handle* h, const void* input, int input_dtype, int64_t in
                     const ld_demod_outputs* out, void* workspace, size_t workspace_bytes,
                     void* stream);
 
+/* One block of a batched launch (ld_demod_batch), in a DEVICE table:
+ * `first` is its first input sample, `sys` its system (0 or 1: which
+ * handle's tables and outputs; < 0: padding, skipped), `slot` where its
+ * outputs go (slot * video_stride / audio_stride in that system's outputs). */
+typedef struct ld_batch_block {
+  int64_t first;
+  int32_t sys;
+  int32_t slot;
+} ld_batch_block;
+
+/* Blocks of many captures of both systems in ONE launch (the reference runs
+ * one LDdecode, with its own RFDecode and DemodCache workers, per capture,
+ * core.py:1049-1421, :3498-3700): sys[0], sys[1] are two handles with the
+ * same blocklen on the same device, each with its filters uploaded (either
+ * may be NULL if no block uses it); outs[k] and nslots[k] are system k's
+ * outputs (as ld_demod_blocks') and their capacity in blocks.  Block i of
+ * the launch demodulates input[blocks[i].first .. + blocklen) with its
+ * system's tables (the same arithmetic as ld_demod_blocks) and writes output
+ * slot blocks[i].slot.  The table is DEVICE memory (so a captured HIP graph
+ * can replay the launch with new contents); entries that would read outside
+ * the input or write outside nslots are skipped on the device.  int16 or
+ * float32 input, no EFM output.  workspace: ld_demod_workspace_bytes(h,
+ * nblocks).  No allocation, copy or memset: safe under stream capture. */
+int ld_demod_batch(ld_handle* const* sys, const void* input, int input_dtype, int64_t input_len,
+                   const ld_batch_block* blocks, int nblocks, const int32_t* nslots, int cut,
+                   const ld_demod_outputs* outs, void* workspace, size_t workspace_bytes, void* stream);
+
 /* No counterpart in the reference.  For callers that cannot allocate device
  * memory (code under HIP graph capture, tight serving loops, harnesses that
  * forbid hipMalloc): ld_demod_blocks with its workspace taken from a static
@@ -270,7 +297,10 @@ typedef struct ld_line_field {
   int32_t nlines;
   int32_t linecount;
   int32_t first_field;
-  int32_t reserved;
+  /* NTSC burst chain the field belongs to (ld_refine_burst_ntsc_chains):
+   * chains are consecutive runs of fields numbered 0, 1, ...; 0 for every
+   * field otherwise (one chain).  Ignored by the other entry points. */
+  int32_t chain;
 } ld_line_field;
 
 /* The RFDecode / SysParams constants the refinements read: input rate in
@@ -306,6 +336,19 @@ int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_bu
                          double prev_phase_adjust, double* linelocs_out, uint8_t* linebad_out,
                          double* phase_adjust_out, int32_t* field_phase_id, void* workspace,
                          size_t workspace_bytes, void* stream);
+
+/* ld_refine_burst_ntsc over several independent chains of consecutive
+ * fields in one call (fields of different captures decoded together): the
+ * fields' `chain` ids are consecutive runs numbered 0 .. nchains-1, and
+ * chain c's first field starts its search from host chain_seeds[c] (each
+ * capture's LDdecode carries its own previous field, core.py:3366-3369).
+ * ld_refine_burst_ntsc is this with one chain seeded by prev_phase_adjust. */
+int ld_refine_burst_ntsc_chains(ld_handle* h, const float* demod, const float* demod_burst, int64_t demod_total,
+                                const ld_line_field* fields, int nfields, const ld_line_params* params,
+                                const double* linelocs2, const uint8_t* linebad_in, int64_t nlinelocs,
+                                const double* chain_seeds, int nchains, double* linelocs_out,
+                                uint8_t* linebad_out, double* phase_adjust_out, int32_t* field_phase_id,
+                                void* workspace, size_t workspace_bytes, void* stream);
 
 /* PAL pilot line alignment: linebad |= compute_deriv_error(linelocs2), two
  * refine_linelocs_pilot passes (core.py:3054-3093, :3221-3224) and

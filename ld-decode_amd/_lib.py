@@ -28,7 +28,8 @@ EXPORTS = (
     "ld_skip_score", "ld_locate_recheck", "ld_unpack_10bit",
     "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
     "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
-    "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc",
+    "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc", "ld_refine_burst_ntsc_chains",
+    "ld_demod_batch",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -51,6 +52,10 @@ class DemodOutputs(ctypes.Structure):
         "audio_left", "audio_right")] + [("video_stride", ctypes.c_int64), ("audio_stride", ctypes.c_int64),
                                          ("efm", ctypes.c_void_p), ("demod_05_below", ctypes.c_void_p),
                                          ("below_high", ctypes.c_double)]
+
+
+class BatchBlock(ctypes.Structure):
+    _fields_ = [("first", ctypes.c_int64), ("sys", ctypes.c_int32), ("slot", ctypes.c_int32)]
 
 
 class TbcField(ctypes.Structure):
@@ -79,7 +84,7 @@ class LineField(ctypes.Structure):
         ("nlines", ctypes.c_int32),
         ("linecount", ctypes.c_int32),
         ("first_field", ctypes.c_int32),
-        ("reserved", ctypes.c_int32),
+        ("chain", ctypes.c_int32),
     ]
 
 
@@ -165,6 +170,8 @@ def _declare(lib):
         "ld_demod_blocks": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
                                     P(DemodOutputs), c_vp, c_sz, c_vp]),
         "ld_demod_noalloc_max_bytes": (c_sz, []),
+        "ld_demod_batch": (c_int, [P(c_vp), c_vp, c_int, c_i64, c_vp, c_int, P(ctypes.c_int32), c_int,
+                                   P(DemodOutputs), c_vp, c_sz, c_vp]),
         "ld_demod_blocks_noalloc": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
                                             P(DemodOutputs), c_vp]),
         "ld_set_sinc_lut": (c_int, [c_vp, P(ctypes.c_float), c_int, c_int]),
@@ -177,6 +184,9 @@ def _declare(lib):
         "ld_refine_workspace_bytes": (c_sz, [c_vp, c_int, c_i64]),
         "ld_refine_burst_ntsc": (c_int, [c_vp, c_vp, c_vp, c_i64, P(LineField), c_int, P(LineParams), c_vp, c_vp,
                                          c_i64, c_dbl, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz, c_vp]),
+        "ld_refine_burst_ntsc_chains": (c_int, [c_vp, c_vp, c_vp, c_i64, P(LineField), c_int, P(LineParams), c_vp,
+                                                c_vp, c_i64, P(c_dbl), c_int, c_vp, c_vp, c_vp, c_vp, c_vp, c_sz,
+                                                c_vp]),
         "ld_refine_pilot_pal": (c_int, [c_vp, c_vp, c_i64, P(LineField), c_int, P(LineParams), c_vp, c_vp, c_i64,
                                         c_vp, c_vp, c_vp, c_sz, c_vp]),
         "ld_pulse_workspace_bytes": (c_sz, [c_vp, c_i64]),

@@ -473,8 +473,11 @@ __device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int t
 // produced too, as one more c2r phase of the input spectrum between the RF
 // high-pass and the analytic signal (separate instantiations, so that the
 // plain kernel's register allocation is not touched).
+//
+// s0: the block's first input sample; bo: its output slot (outputs at bo
+// times the strides); bw: its workspace slot.
 template <bool kPacked, bool kEfm>
-__device__ __forceinline__ void demod_block(const DemodArgs& a) {
+__device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int bo, int bw) {
   __shared__ Lds s;
   __shared__ float2 aslice[2][kNA / 2 + 1];
   __shared__ TwLds twl;
@@ -485,9 +488,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
   // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters, 6 EFM
   __shared__ float2 bin_m[2 + 4 + 1];
   const int tid = threadIdx.x;
-  const int b = blockIdx.x;
-  const int64_t s0 = a.first + a.step * b;
-  const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)b * kM, kM * 8u);
+  const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)bw * kM, kM * 8u);
   const uint32_t vo = tid * 8u;
   {  // twiddle tables into LDS
     const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
@@ -627,7 +628,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
         v[m] = make_float2(h0, h1);
       });
       if (a.demod_raw != nullptr) {
-        const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)b * a.vstride, (uint32_t)a.vlen * 4u);
+        const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)bo * a.vstride, (uint32_t)a.vlen * 4u);
         const bool pairs = ((a.vcut | a.vlen) & 1) == 0;
 #pragma unroll
         for (int m = 0; m < kP; ++m) {  // i < 0 wraps to a huge offset: dropped
@@ -702,16 +703,16 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
     } else if (ph == kPhHpf) {
       stage_own(v, s, tidv);
       load_half(kEfm ? a.h_efm : a.h_rfv, v, tidv);
-      c2r_store(a.hpf, (int64_t)b * a.vstride, tidv, s);
+      c2r_store(a.hpf, (int64_t)bo * a.vstride, tidv, s);
     } else if (kEfm && ph == kPhEfm) {
       stage_own(v, s, tidv);
       load_half(a.h_rfv, v, tidv);
-      efm_store(a.efm + (int64_t)b * a.vstride, a.vcut, a.vlen, tidv, s);
+      efm_store(a.efm + (int64_t)bo * a.vstride, a.vcut, a.vlen, tidv, s);
     } else if (ph >= kPhPost) {
       stage_own(v, s, tidv);
       if (ph + 1 < last) load_half(a.post[ph + 1 - kPhPost].h, v, tidv);
-      c2r_store(a.post[ph - kPhPost], (int64_t)b * a.vstride, tidv, s,
-                ph - kPhPost == a.below_post ? a.below + (((int64_t)b * a.vstride) >> 5) : nullptr, a.below_highf);
+      c2r_store(a.post[ph - kPhPost], (int64_t)bo * a.vstride, tidv, s,
+                ph - kPhPost == a.below_post ? a.below + (((int64_t)bo * a.vstride) >> 5) : nullptr, a.below_highf);
     }
   }
 
@@ -721,14 +722,38 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a) {
     const int c = tid >> 6;
     const TwGlobal tg{make_rsrc(a.tw_rows, TwLayout::total * 8u), TwLayout::k1k_p2, TwLayout::k1k_p3, 16, 4, 2};
     if (a.out_audio[c] != nullptr)
-      audio_wave(a, c, aslice[c], s.re + c * LdsSize<kNA>::floats, s.im + c * LdsSize<kNA>::floats, b, tg);
+      audio_wave(a, c, aslice[c], s.re + c * LdsSize<kNA>::floats, s.im + c * LdsSize<kNA>::floats, bo, tg);
   }
 }
 
-__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false, false>(a); }
-__global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true, false>(a); }
-__global__ __launch_bounds__(kT) void k_demod_block_efm(DemodArgs a) { demod_block<false, true>(a); }
-__global__ __launch_bounds__(kT) void k_demod_block_packed_efm(DemodArgs a) { demod_block<true, true>(a); }
+#define LD_DEMOD_AT(a) (a), (a).first + (a).step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x
+__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false, false>(LD_DEMOD_AT(a)); }
+__global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true, false>(LD_DEMOD_AT(a)); }
+__global__ __launch_bounds__(kT) void k_demod_block_efm(DemodArgs a) { demod_block<false, true>(LD_DEMOD_AT(a)); }
+__global__ __launch_bounds__(kT) void k_demod_block_packed_efm(DemodArgs a) { demod_block<true, true>(LD_DEMOD_AT(a)); }
+#undef LD_DEMOD_AT
+
+// Blocks of many captures and both systems in one launch: launch block i
+// demodulates the block the DEVICE table entry i describes with the
+// parameters and filter tables of its system (entries with sys < 0 are
+// padding and exit at once).
+// An entry whose block would read outside the input or write outside its
+// system's output slots is skipped as well (the table is device data the
+// host cannot check before the launch).
+struct DemodBatchArgs {
+  DemodArgs sys[2];
+  const ld_batch_block* blocks;
+  int64_t input_len;
+  int nslots[2];
+};
+static_assert(sizeof(DemodBatchArgs) <= 4096, "kernel arguments over 4 KiB");
+__global__ __launch_bounds__(kT) void k_demod_batch(DemodBatchArgs a) {
+  const ld_batch_block d = a.blocks[blockIdx.x];
+  if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - kN || d.slot < 0 ||
+      d.slot >= a.nslots[d.sys])
+    return;
+  demod_block<false, false>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
+}
 
 // ------------------------------------------------------- audio stage 2
 // One blocklen segment of a field read's stage-1 audio: y = the filtered
@@ -1180,40 +1205,29 @@ size_t ld_demod_workspace_bytes(const ld_handle* h, int nblocks) {
   return (size_t)nblocks * (size_t)(h->blocklen / 2) * sizeof(float2);
 }
 
-int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_len, int64_t first,
-                    int64_t step, int nblocks, int cut, const ld_demod_outputs* out, void* workspace,
-                    size_t ws_bytes, void* stream) {
-  if (!h || !input || !out || nblocks <= 0 || !workspace) {
-    set_error("ld_demod_blocks: null argument or nblocks <= 0");
-    return LD_EINVAL;
-  }
+// The launch arguments of one handle's demodulation (shared by the plain
+// and the batched entry points); `who` names the caller in errors.
+// nblocks: the number of output slots the strides are checked for.
+static int build_demod_args(ld_handle* h, const void* input, int dtype, int64_t input_len, int nblocks, int cut,
+                            const ld_demod_outputs* out, void* workspace, DemodArgs& a, const char* who) {
   if (dtype != LD_IN_INT16 && dtype != LD_IN_FLOAT32 && dtype != LD_IN_PACKED10) {
-    set_error("ld_demod_blocks: unknown input dtype %d", dtype);
+    set_error("%s: unknown input dtype %d", who, dtype);
     return LD_EINVAL;
   }
   if (!h->params_set || !h->filt[LD_FILT_RFVIDEO] || !h->filt[LD_FILT_RFHPF] || !h->filt[LD_FILT_VIDEO] ||
       !h->filt[LD_FILT_VIDEO05] || !h->filt[LD_FILT_BURST]) {
-    set_error("ld_demod_blocks: demod parameters or video filters not uploaded");
+    set_error("%s: demod parameters or video filters not uploaded", who);
     return LD_ESTATE;
   }
   if (out->demod_pilot && !h->filt[LD_FILT_PILOT]) {
-    set_error("ld_demod_blocks: demod_pilot requested but no pilot filter uploaded");
+    set_error("%s: demod_pilot requested but no pilot filter uploaded", who);
     return LD_ESTATE;
   }
   if ((out->audio_left && !h->audio_filt[0]) || (out->audio_right && !h->audio_filt[1])) {
-    set_error("ld_demod_blocks: audio requested but audio filters not uploaded");
+    set_error("%s: audio requested but audio filters not uploaded", who);
     return LD_ESTATE;
   }
   const int N = h->blocklen;
-  if (first < 0 || step < 0 || first + (int64_t)(nblocks - 1) * step + N > input_len) {
-    set_error("ld_demod_blocks: blocks [%lld, +%d*%lld + %d) exceed input length %lld", (long long)first,
-              nblocks - 1, (long long)step, N, (long long)input_len);
-    return LD_ERANGE;
-  }
-  if (ws_bytes < ld_demod_workspace_bytes(h, nblocks)) {
-    set_error("ld_demod_blocks: workspace %zu bytes < %zu needed", ws_bytes, ld_demod_workspace_bytes(h, nblocks));
-    return LD_EINVAL;
-  }
   const int S = N - h->cut_begin - h->cut_end;
   const int vcut = cut ? h->cut_begin : 0;
   const int vlen = cut ? S : N;
@@ -1223,13 +1237,11 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   const int alen = cut ? NA - acut - (h->cut_end + fdiv - 1) / fdiv : NA;
   if (out->video_stride < 0 || out->audio_stride < 0 ||
       (nblocks > 1 && ((out->video_stride < vlen) || ((out->audio_left || out->audio_right) && out->audio_stride < alen)))) {
-    set_error("ld_demod_blocks: output strides shorter than one block's output");
+    set_error("%s: output strides shorter than one block's output", who);
     return LD_EINVAL;
   }
-  DemodArgs a = {};
+  a = DemodArgs{};
   a.in = input;
-  a.first = first;
-  a.step = step;
   a.tw_rows = h->twiddle;
   a.tw_lds = h->twiddle + TwLayout::total;
   a.tw_split = h->twiddle + TwLayout::total + kTwLdsCount;
@@ -1245,11 +1257,11 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.below_post = -1;
   if (out->demod_05_below) {
     if (!out->demod_05 || (nblocks > 1 && out->video_stride % 32 != 0)) {
-      set_error("ld_demod_blocks: demod_05_below needs demod_05 and a video stride that is a multiple of 32");
+      set_error("%s: demod_05_below needs demod_05 and a video stride that is a multiple of 32", who);
       return LD_EINVAL;
     }
     if ((vcut + h->filt_shift[LD_FILT_VIDEO05]) % 32 != 0 || vlen % 2 != 0) {
-      set_error("ld_demod_blocks: demod_05_below needs demod_05's output offset to be a multiple of 32");
+      set_error("%s: demod_05_below needs demod_05's output offset to be a multiple of 32", who);
       return LD_EINVAL;
     }
     a.below = out->demod_05_below;
@@ -1296,7 +1308,7 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   if (dtype == LD_IN_PACKED10) {
     const int64_t bytes = (input_len + 3) / 4 * 5;
     if (bytes > 0xfffffff0LL) {
-      set_error("ld_demod_blocks: packed input over 4 GiB (pass a shorter span)");
+      set_error("%s: packed input over 4 GiB (pass a shorter span)", who);
       return LD_EINVAL;
     }
     // dword loads cover the pairs' bytes: every dword holding a sample byte
@@ -1307,15 +1319,92 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.efm = out->efm;
   a.h_efm = h->filt[LD_FILT_EFM];
   if (a.efm && !a.h_efm) {
-    set_error("ld_demod_blocks: EFM output requested without the EFM filter (LD_FILT_EFM)");
+    set_error("%s: EFM output requested without the EFM filter (LD_FILT_EFM)", who);
     return LD_ESTATE;
   }
+  return LD_OK;
+}
+
+int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_len, int64_t first,
+                    int64_t step, int nblocks, int cut, const ld_demod_outputs* out, void* workspace,
+                    size_t ws_bytes, void* stream) {
+  if (!h || !input || !out || nblocks <= 0 || !workspace) {
+    set_error("ld_demod_blocks: null argument or nblocks <= 0");
+    return LD_EINVAL;
+  }
+  const int N = h->blocklen;
+  if (first < 0 || step < 0 || first + (int64_t)(nblocks - 1) * step + N > input_len) {
+    set_error("ld_demod_blocks: blocks [%lld, +%d*%lld + %d) exceed input length %lld", (long long)first,
+              nblocks - 1, (long long)step, N, (long long)input_len);
+    return LD_ERANGE;
+  }
+  if (ws_bytes < ld_demod_workspace_bytes(h, nblocks)) {
+    set_error("ld_demod_blocks: workspace %zu bytes < %zu needed", ws_bytes, ld_demod_workspace_bytes(h, nblocks));
+    return LD_EINVAL;
+  }
+  DemodArgs a;
+  if (int rc = build_demod_args(h, input, dtype, input_len, nblocks, cut, out, workspace, a, "ld_demod_blocks"))
+    return rc;
+  a.first = first;
+  a.step = step;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
   if (dtype == LD_IN_PACKED10)
     hipLaunchKernelGGL(a.efm ? k_demod_block_packed_efm : k_demod_block_packed, dim3(nblocks), dim3(kT), 0, st, a);
   else
     hipLaunchKernelGGL(a.efm ? k_demod_block_efm : k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t input_len,
+                   const ld_batch_block* blocks, int nblocks, const int32_t* nslots, int cut,
+                   const ld_demod_outputs* outs, void* workspace, size_t ws_bytes, void* stream) {
+  if (!sys || !input || !blocks || nblocks <= 0 || !nslots || !outs || !workspace) {
+    set_error("ld_demod_batch: null argument or nblocks <= 0");
+    return LD_EINVAL;
+  }
+  if (dtype != LD_IN_INT16 && dtype != LD_IN_FLOAT32) {
+    set_error("ld_demod_batch: input dtype %d (int16 or float32 only)", dtype);
+    return LD_EINVAL;
+  }
+  ld_handle* h0 = sys[0] ? sys[0] : sys[1];
+  if (!h0) {
+    set_error("ld_demod_batch: no system handle");
+    return LD_EINVAL;
+  }
+  DemodBatchArgs b = {};
+  for (int k = 0; k < 2; ++k) {
+    ld_handle* h = sys[k];
+    b.nslots[k] = 0;
+    if (!h) continue;
+    if (h->blocklen != kN || h->device != h0->device) {
+      set_error("ld_demod_batch: system %d handle: blocklen %d or device %d differs", k, h->blocklen, h->device);
+      return LD_EINVAL;
+    }
+    if (outs[k].efm) {
+      set_error("ld_demod_batch: no EFM output in the batched launch");
+      return LD_EINVAL;
+    }
+    if (nslots[k] < 0) {
+      set_error("ld_demod_batch: system %d: %d output slots", k, nslots[k]);
+      return LD_EINVAL;
+    }
+    const int nb = nslots[k] > 1 ? nslots[k] : 2;  // strides are checked as for several blocks
+    if (int rc = build_demod_args(h, input, dtype, input_len, nb, cut, &outs[k], workspace, b.sys[k],
+                                  "ld_demod_batch"))
+      return rc;
+    b.nslots[k] = nslots[k];
+  }
+  if (ws_bytes < ld_demod_workspace_bytes(h0, nblocks)) {
+    set_error("ld_demod_batch: workspace %zu bytes < %zu needed", ws_bytes, ld_demod_workspace_bytes(h0, nblocks));
+    return LD_EINVAL;
+  }
+  b.blocks = blocks;
+  b.input_len = input_len;
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h0->device));
+  hipLaunchKernelGGL(k_demod_batch, dim3(nblocks), dim3(kT), 0, st, b);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

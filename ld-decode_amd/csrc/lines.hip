@@ -10,9 +10,9 @@
 //   k_burst_scan    8 lanes per (field, line), 8 lines per wavefront: burst
 //                   area statistics, the zero-crossing scan, the 16 cycle
 //                   positions and the step table over them
-//   k_burst_chain   one workgroup walks the fields in order: each field's
-//                   phase-adjust median seeds the next field (core.py:3366-
-//                   3369); lines in parallel within a field
+//   k_burst_chain   one workgroup per chain walks its fields in order: each
+//                   field's phase-adjust median seeds the next field
+//                   (core.py:3366-3369); lines in parallel within a field
 //   k_burst_finish  one workgroup per field: sample adjustments, their
 //                   median and outlier fallback, fix_badlines, shift
 //   k_pilot_pass    one workgroup per field: one pilot alignment pass
@@ -210,7 +210,11 @@ struct RefineArgs {
   BurstLine* lines;
   FieldState* state;
   double freq, linelen, spl, fsc_mhz, pilot_mhz, hz_ire, shift, pilot_len;
-  double prev_phase;
+  // NTSC burst chains: chain c is fields [chain_first[c], chain_first[c+1])
+  // and its first field's search starts from chain_seed[c]
+  const int* chain_first;
+  const double* chain_seed;
+  int nchains;
 };
 
 // ---------------------------------------------------------------- NTSC
@@ -439,7 +443,7 @@ __device__ __forceinline__ unsigned long long wave_select(const unsigned long lo
 
 constexpr int kChainJ = (kBurstLines + 63) / 64;   // keys per lane of the selecting wavefront
 
-// One workgroup walks the fields in order; line l = thread l.  Per field:
+// One workgroup per chain walks its fields in order; line l = thread l.  Per field:
 // both burst passes from the previous field's median, then the median of
 // the line adjustments, selected by the first wavefront from the keys in
 // LDS (two barriers; LDS double-buffered by field parity).  The next
@@ -450,7 +454,9 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
   __shared__ int wcnt[2][kRefThreads / 64], wrise[2][kRefThreads / 64], wnan[2][kRefThreads / 64];
   __shared__ double mid[2][2];
   const int l = threadIdx.x, w = l >> 6, lane = l & 63;
-  double prev = a.prev_phase;
+  const int c = blockIdx.x;
+  const int f0 = a.chain_first[c], f1 = a.chain_first[c + 1];
+  double prev = a.chain_seed[c];
   double bp[kZc], tab[kZc + 1], minbase = 0.0;
   unsigned rcw[5];
   int count = 0, rmask = 0, valid = 0;
@@ -471,8 +477,8 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
       for (int k = 0; k < 5; ++k) rcw[k] = rw[k];
     }
   };
-  load(0);
-  for (int f = 0; f < a.nfields; ++f) {
+  load(f0);
+  for (int f = f0; f < f1; ++f) {
     const int p = f & 1;
     double adj = INFINITY;
     int rise = 0;
@@ -527,7 +533,7 @@ __global__ __launch_bounds__(kRefThreads) void k_burst_chain(RefineArgs a) {
       wrise[p][w] = __popcll(rb);
       wnan[p][w] = __popcll(nb);
     }
-    if (f + 1 < a.nfields) load(f + 1);
+    if (f + 1 < f1) load(f + 1);
     __syncthreads();
     int m = 0, rs = 0, nn = 0;
 #pragma unroll
@@ -831,6 +837,7 @@ static int check_line_fields(const ld_line_field* fields, int nfields, int64_t d
 size_t ld_refine_workspace_bytes(const ld_handle* h, int nfields, int64_t nlinelocs) {
   if (!h || nfields <= 0 || nlinelocs < 0) return 0;
   return align256(sizeof(ld_line_field) * (size_t)nfields) + align256(sizeof(FieldState) * (size_t)nfields) +
+         align256(sizeof(int) * ((size_t)nfields + 1)) + align256(sizeof(double) * (size_t)nfields) +
          align256(sizeof(BurstLine) * (size_t)nfields * kBurstLines) + 2 * align256(sizeof(double) * (size_t)nlinelocs);
 }
 
@@ -841,6 +848,10 @@ static RefineArgs refine_args(ld_handle* h, const ld_line_params* p, int nfields
   w += align256(sizeof(ld_line_field) * (size_t)nfields);
   a.state = (FieldState*)w;
   w += align256(sizeof(FieldState) * (size_t)nfields);
+  a.chain_first = (const int*)w;
+  w += align256(sizeof(int) * ((size_t)nfields + 1));
+  a.chain_seed = (const double*)w;
+  w += align256(sizeof(double) * (size_t)nfields);
   a.lines = (BurstLine*)w;
   a.nfields = nfields;
   a.freq = p->freq_mhz;
@@ -855,14 +866,14 @@ static RefineArgs refine_args(ld_handle* h, const ld_line_params* p, int nfields
   return a;
 }
 
-int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_burst, int64_t demod_total,
-                         const ld_line_field* fields, int nfields, const ld_line_params* params,
-                         const double* linelocs2, const uint8_t* linebad_in, int64_t nlinelocs,
-                         double prev_phase_adjust, double* linelocs_out, uint8_t* linebad_out,
-                         double* phase_adjust_out, int32_t* field_phase_id, void* workspace, size_t ws_bytes,
-                         void* stream) {
+int ld_refine_burst_ntsc_chains(ld_handle* h, const float* demod, const float* demod_burst, int64_t demod_total,
+                                const ld_line_field* fields, int nfields, const ld_line_params* params,
+                                const double* linelocs2, const uint8_t* linebad_in, int64_t nlinelocs,
+                                const double* chain_seeds, int nchains, double* linelocs_out, uint8_t* linebad_out,
+                                double* phase_adjust_out, int32_t* field_phase_id, void* workspace, size_t ws_bytes,
+                                void* stream) {
   if (!h || !demod || !demod_burst || !params || !linelocs2 || !linebad_in || !linelocs_out || !linebad_out ||
-      !phase_adjust_out || !field_phase_id || !workspace) {
+      !phase_adjust_out || !field_phase_id || !workspace || !chain_seeds) {
     set_error("ld_refine_burst_ntsc: null argument");
     return LD_EINVAL;
   }
@@ -870,6 +881,22 @@ int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_bu
   if (rc != LD_OK) return rc;
   if (!(params->linelen > 0) || !(params->fsc_mhz > 0) || !(params->freq_mhz > 0)) {
     set_error("ld_refine_burst_ntsc: bad line parameters");
+    return LD_EINVAL;
+  }
+  // chains: consecutive runs of fields with chain ids 0, 1, .., nchains-1
+  std::vector<int> first(1, 0);
+  for (int f = 0; f < nfields; ++f) {
+    const int c = fields[f].chain, prevc = f ? fields[f - 1].chain : 0;
+    if (c != prevc && c != prevc + 1) {
+      set_error("ld_refine_burst_ntsc: field %d: chain %d after chain %d (chains must be consecutive runs "
+                "numbered from 0)", f, c, prevc);
+      return LD_EINVAL;
+    }
+    if (c == prevc + 1) first.push_back(f);
+  }
+  first.push_back(nfields);
+  if ((int)first.size() - 1 != nchains || fields[0].chain != 0) {
+    set_error("ld_refine_burst_ntsc: the fields hold %d chains, %d seeds given", (int)first.size() - 1, nchains);
     return LD_EINVAL;
   }
   if (ws_bytes < ld_refine_workspace_bytes(h, nfields, nlinelocs)) {
@@ -886,18 +913,32 @@ int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_bu
   a.bad_out = linebad_out;
   a.phase_out = phase_adjust_out;
   a.fpid_out = field_phase_id;
-  a.prev_phase = prev_phase_adjust;
-  // a pageable source is staged before hipMemcpyAsync returns
+  a.nchains = nchains;
+  // pageable sources are staged before hipMemcpyAsync returns
   LD_HIP(hipMemcpyAsync((void*)a.fields, fields, sizeof(ld_line_field) * (size_t)nfields, hipMemcpyHostToDevice,
+                        st));
+  LD_HIP(hipMemcpyAsync((void*)a.chain_first, first.data(), sizeof(int) * first.size(), hipMemcpyHostToDevice, st));
+  LD_HIP(hipMemcpyAsync((void*)a.chain_seed, chain_seeds, sizeof(double) * (size_t)nchains, hipMemcpyHostToDevice,
                         st));
   hipLaunchKernelGGL(k_burst_scan, dim3((kBurstLines + kScanLines - 1) / kScanLines, nfields), dim3(64), 0, st, a,
                      demod, demod_burst);
   LD_HIP(hipGetLastError());
-  hipLaunchKernelGGL(k_burst_chain, dim3(1), dim3(kRefThreads), 0, st, a);
+  hipLaunchKernelGGL(k_burst_chain, dim3(nchains), dim3(kRefThreads), 0, st, a);
   LD_HIP(hipGetLastError());
   hipLaunchKernelGGL(k_burst_finish, dim3(nfields), dim3(kRefThreads), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
+}
+
+int ld_refine_burst_ntsc(ld_handle* h, const float* demod, const float* demod_burst, int64_t demod_total,
+                         const ld_line_field* fields, int nfields, const ld_line_params* params,
+                         const double* linelocs2, const uint8_t* linebad_in, int64_t nlinelocs,
+                         double prev_phase_adjust, double* linelocs_out, uint8_t* linebad_out,
+                         double* phase_adjust_out, int32_t* field_phase_id, void* workspace, size_t ws_bytes,
+                         void* stream) {
+  return ld_refine_burst_ntsc_chains(h, demod, demod_burst, demod_total, fields, nfields, params, linelocs2,
+                                     linebad_in, nlinelocs, &prev_phase_adjust, 1, linelocs_out, linebad_out,
+                                     phase_adjust_out, field_phase_id, workspace, ws_bytes, stream);
 }
 
 int ld_refine_pilot_pal(ld_handle* h, const float* demod_pilot, int64_t demod_total, const ld_line_field* fields,

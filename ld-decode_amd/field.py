@@ -120,8 +120,12 @@ def _build_tbc(fields):
 
 
 def _build_lines(fields):
-    return _struct_array(_lib.LineField, fields,
-                         ("demod_offset", "demod_len", "lineloc_offset", "nlines", "linecount", "first_field"))
+    a = _struct_array(_lib.LineField, fields,
+                      ("demod_offset", "demod_len", "lineloc_offset", "nlines", "linecount", "first_field"))
+    if fields and "chain" in fields[0]:
+        for i, f in enumerate(fields):
+            a[i].chain = int(f["chain"])
+    return a
 
 
 def _field_array(fields):
@@ -210,7 +214,7 @@ def _line_fields(fields):
 
 
 def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust=0.0, stream=None, out=None,
-                        workspace=None):
+                        workspace=None, chain_seeds=None):
     """Burst (NTSC) or pilot (PAL) line alignment + fix_badlines (+ NTSC
     subcarrier shift) for a batch of fields (FieldNTSC.process
     core.py:3472-3495, FieldPAL.process core.py:3212-3237).
@@ -223,6 +227,9 @@ def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust
     linelocs2 / linebad: the hsync refine's outputs for the whole batch.
     out: optional preallocated (linelocs, linebad, phase, fpid) device
     tensors; workspace: optional device buffer of ld_refine_workspace_bytes.
+    chain_seeds (NTSC): the fields form independent burst chains (their
+    "chain" keys, consecutive runs numbered from 0, e.g. one per capture),
+    chain c starting from chain_seeds[c] instead of prev_phase_adjust.
     Returns (linelocs float64, linebad uint8, phase_adjust_median per field or
     None, fieldPhaseID per field or None) as device tensors."""
     torch = rf.torch
@@ -245,10 +252,17 @@ def refine_lines_fields(rf, video, fields, linelocs2, linebad, prev_phase_adjust
     st = rf._stream() if stream is None else stream
     if rf.system == "NTSC":
         demod, burst = video["demod"], video["demod_burst"]
-        _lib.check(rf.lib.ld_refine_burst_ntsc(
-            rf.handle, demod.data_ptr(), burst.data_ptr(), demod.numel(), lf, len(fields), ctypes.byref(lp),
-            ll2.data_ptr(), bad.data_ptr(), n, float(prev_phase_adjust), ll.data_ptr(), bad_out.data_ptr(),
-            phase.data_ptr(), fpid.data_ptr(), ws.data_ptr(), ws.numel(), st))
+        if chain_seeds is None:
+            _lib.check(rf.lib.ld_refine_burst_ntsc(
+                rf.handle, demod.data_ptr(), burst.data_ptr(), demod.numel(), lf, len(fields), ctypes.byref(lp),
+                ll2.data_ptr(), bad.data_ptr(), n, float(prev_phase_adjust), ll.data_ptr(), bad_out.data_ptr(),
+                phase.data_ptr(), fpid.data_ptr(), ws.data_ptr(), ws.numel(), st))
+        else:
+            seeds = (ctypes.c_double * len(chain_seeds))(*[float(v) for v in chain_seeds])
+            _lib.check(rf.lib.ld_refine_burst_ntsc_chains(
+                rf.handle, demod.data_ptr(), burst.data_ptr(), demod.numel(), lf, len(fields), ctypes.byref(lp),
+                ll2.data_ptr(), bad.data_ptr(), n, seeds, len(chain_seeds), ll.data_ptr(), bad_out.data_ptr(),
+                phase.data_ptr(), fpid.data_ptr(), ws.data_ptr(), ws.numel(), st))
         return ll, bad_out, phase, fpid
     pilot = video["demod_pilot"]
     _lib.check(rf.lib.ld_refine_pilot_pal(

@@ -86,11 +86,12 @@ class PulseFinder:
         self.h_means = torch.empty(self.cap // 2, dtype=torch.float32, pin_memory=True)
         self.done = None
 
-    def launch(self, demod05, n=None, stream=None, high=None, bits=None):
+    def launch(self, demod05, n=None, stream=None, high=None, bits=None, record=True):
         """Enqueue the pulse search, the sync levels and their copies to the
         host on `stream` (default: current).  bits: (int32 words, threshold)
         of the demod's in-pulse bits over demod05; used when that threshold
-        is this search's."""
+        is this search's.  record=False: no completion event (the caller
+        tracks the stream, e.g. under graph capture)."""
         torch, rf = self.torch, self.rf
         n = int(n if n is not None else demod05.numel())
         assert n <= self.n and demod05.dtype == torch.float32 and demod05.is_contiguous()
@@ -112,8 +113,9 @@ class PulseFinder:
             self.h_count.copy_(self.count, non_blocking=True)
             self.h_edges.copy_(self.edges, non_blocking=True)
             self.h_means.copy_(self.means, non_blocking=True)
-            self.done = torch.cuda.Event()
-            self.done.record(st)
+            if record:
+                self.done = torch.cuda.Event()
+                self.done.record(st)
         self._args = (demod05, n, st, hi)
 
     def result(self):
@@ -239,12 +241,12 @@ def _f32_median(v):
     return np.float32(np.median(np.asarray(v, dtype=np.float32)))
 
 
-def relevel_pulses(rf, demod05, edges, means, loc, stream=None):
+def relevel_pulses(rf, demod05, edges, means, loc, stream=None, origin=0):
     """getpulses' second pass for a field whose vsync level is off
     (core.py:2292-2322): black levels from the equalising pulses around the
     vertical sync (GPU means), the threshold halfway between black and sync,
     and findpulses again over the field buffer (GPU).  Returns the field's
-    (starts, lens)."""
+    (starts, lens).  origin: the walk coordinate of demod05[0]."""
     torch = rf.torch
     ws, n = int(loc.startloc), int(loc.buflen)
     s, e = edges[0:len(edges) - 1:2], edges[1::2]
@@ -256,8 +258,8 @@ def relevel_pulses(rf, demod05, edges, means, loc, stream=None):
     synclevel = _f32_median(means[k[vl]])
     r = [i for i in list(range(vl[0] - 5, vl[0])) + list(range(vl[-1] + 1, vl[-1] + 6)) if 0 <= i < len(st)]
     r = [i for i in r if rf.freq * 0.75 <= ln[i] <= rf.freq * 2.5]
-    begin = np.array([int(st[i] + rf.freq * 5) + ws for i in r], dtype=np.int64)
-    end = np.array([int(st[i] + rf.freq * 20) + ws for i in r], dtype=np.int64)
+    begin = np.array([int(st[i] + rf.freq * 5) + ws - origin for i in r], dtype=np.int64)
+    end = np.array([int(st[i] + rf.freq * 20) + ws - origin for i in r], dtype=np.int64)
     dev = rf.device
     cur = stream if stream is not None else torch.cuda.current_stream(dev)
     out = torch.empty(max(len(r), 1), dtype=torch.float32, device=dev)
@@ -270,7 +272,7 @@ def relevel_pulses(rf, demod05, edges, means, loc, stream=None):
     blacklevel = _f32_median(out[:len(r)].cpu().numpy()) if len(r) else np.float32(np.nan)
     high = np.float32((blacklevel + synclevel) / np.float32(2))
     pf = PulseFinder(rf, n)
-    pf.launch(demod05[ws:ws + n], n, cur, float(high))
+    pf.launch(demod05[ws - origin:ws - origin + n], n, cur, float(high))
     e2, _ = pf.result()
     s2, t2 = e2[0:len(e2) - 1:2], e2[1::2]
     s2 = s2[:len(t2)]
@@ -307,23 +309,36 @@ class FieldWalker:
     of them are valid (or the buffer ends) and returns every field visited
     (valid or not) with its linelocs1 / linebad."""
 
-    def __init__(self, rf, total, max_fields=64):
+    def __init__(self, rf, total, max_fields=64, finder=True):
         self.rf = rf
         self.total = int(total)
         self.max_fields = max_fields
-        self.finder = PulseFinder(rf, total)
+        # finder=False: the caller supplies the pulses (walk(edges_means=...));
+        # a search buffer is made only if a first-field ire0 recalibration
+        # needs a new search
+        self._finder = PulseFinder(rf, total) if finder else None
         self.prov_delta = 0.0
         self.bufs = WalkBuffers(rf, max_fields + 16)
+        self.allow_recalibration = True
+
+    @property
+    def finder(self):
+        if self._finder is None:
+            self._finder = PulseFinder(self.rf, self.total)
+        return self._finder
 
     def pulses(self, demod05, stream=None, bits=None):
         self.finder.launch(demod05, self.total, stream, bits=bits)
 
-    def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None):
+    def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None, edges_means=None, origin=0):
         """Returns (locs, linelocs1, linebad, prev_out, next start,
         vpulses); the arrays are views of this walker's buffers, valid until
-        its next walk."""
+        its next walk.  edges_means: the buffer's pulses (edge pairs and
+        levels, numpy, in walk coordinates) when the caller searched them
+        itself; origin: the walk coordinate of demod05[0] (a window of a
+        longer capture)."""
         rf = self.rf
-        edges, means = self.finder.result()
+        edges, means = self.finder.result() if edges_means is None else edges_means
         P = locate_params(rf, self.prov_delta)
         B = self.bufs
         locs = []
@@ -346,6 +361,9 @@ class FieldWalker:
                 written = po.fields_written
                 pv = po if len(got) > 1 else prev
                 if tail.status == _lib.LD_LOC_NEED_IRE0:
+                    if not self.allow_recalibration:
+                        raise RuntimeError("first-field ire0 recalibration needed where the decoder's "
+                                           "parameters are shared (batched captures)")
                     rf.DecoderParams["ire0"] = recalibrated_ire0(rf, demod05, tail)
                     P = locate_params(rf, self.prov_delta)
                     self.finder.launch(demod05, self.total, stream)
@@ -353,7 +371,7 @@ class FieldWalker:
                     start = tail.start
                     prev = pv
                     continue
-                st, ln = relevel_pulses(rf, demod05, edges, means, tail, stream)
+                st, ln = relevel_pulses(rf, demod05, edges, means, tail, stream, origin)
                 loc, l1, b1 = locate_one(rf, st, ln, pv if (pv is not None and pv.valid) else None,
                                          tail.startloc, written, P)
                 loc.start, loc.readloc, loc.startloc, loc.buflen = tail.start, tail.readloc, tail.startloc, tail.buflen

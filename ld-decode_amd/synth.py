@@ -192,3 +192,135 @@ def pack_10bit(samples):
     out[:, 3] = ((c[:, 2] & 0x3F) << 2) | (c[:, 3] >> 8)
     out[:, 4] = c[:, 3] & 0xFF
     return out.reshape(-1)
+
+
+# ---------------------------------------------------------------- batches
+# Fields per template: whole colour sequences spanning an integer number of
+# samples and subcarrier (and PAL pilot) cycles at 40 MSPS, so that templates
+# tile without a phase step: 12 NTSC fields = 6300 half-lines of 1271.1 samples
+# = 8,008,000 samples (716,625 fsc cycles); 8 PAL fields = 6,400,000 samples.
+TEMPLATE_FIELDS = {"NTSC": 12, "PAL": 8}
+
+
+def template_samples(system, fs=FS):
+    sysp, _ = system_tables(system)
+    hpf, _, _, _ = _field_layout(system)
+    n = TEMPLATE_FIELDS[system] * hpf / 2 * sysp["line_period"] * 1e-6 * fs
+    if abs(n - round(n)) > 1e-6:
+        raise ValueError(f"{system} template is not a whole number of samples at {fs} Hz")
+    return int(round(n))
+
+
+def batch_plan(seed=4, ncaptures=64, min_s=1.0, max_s=30.0, pal_fraction=0.5, fs=FS):
+    """The batched workload: capture lengths log-uniform in [min_s, max_s]
+    seconds and systems NTSC / PAL with probability 1 - pal_fraction /
+    pal_fraction, from `seed`.  Returns a list of (system, samples)."""
+    rng = np.random.default_rng(seed)
+    secs = np.exp(rng.uniform(np.log(min_s), np.log(max_s), ncaptures))
+    pal = rng.random(ncaptures) < pal_fraction
+    return [("PAL" if p else "NTSC", int(round(s * fs))) for s, p in zip(secs, pal)]
+
+
+def field_template(system, seed, jitter_ns=50.0, fs=FS):
+    """One tileable template: the pre-emphasised instantaneous frequency
+    (float64 Hz) of TEMPLATE_FIELDS[system] fields starting at the first
+    equalising pulse of an even-numbered field, with per-line jitter drawn
+    from `seed`.  The pre-emphasis is applied circularly over the template,
+    which is what it is when the template repeats."""
+    rng = np.random.default_rng(seed)
+    sysp, decp = system_tables(system)
+    n = template_samples(system, fs)
+    pilot = 10.0 if system == "PAL" else 0.0
+    ire, _, _ = ire_raster(system, sysp, n, fs, jitter_ns * 1e-9 * fs, rng, 0, pilot)
+    hz = sysp["ire0"] + sysp["hz_ire"] * ire
+    del ire
+    lpf_b, lpf_a = sps.butter(decp["video_lpf_order"], decp["video_lpf_freq"] / (fs / 2), "low")
+    t1, t2 = decp["video_deemp"]
+    emp_b, emp_a = emphasis(t2, t1, fs)
+    w = np.arange(n // 2 + 1) * (2 * np.pi / n)
+    H = sps.freqz(lpf_b, lpf_a, worN=w)[1] * sps.freqz(emp_b, emp_a, worN=w)[1]
+    mean = hz.mean()
+    out = sfft.irfft(sfft.rfft(hz - mean) * H, n) + mean * H[0].real
+    return out
+
+
+class Batch:
+    """Captures of a batch back to back in one int16 tensor: capture k is
+    data[offsets[k] : offsets[k] + lengths[k]], of system systems[k]."""
+
+    def __init__(self, data, systems, offsets, lengths, fs, snr_db):
+        self.data = data
+        self.systems = list(systems)
+        self.offsets = [int(o) for o in offsets]
+        self.lengths = [int(n) for n in lengths]
+        self.fs = fs
+        self.snr_db = snr_db
+
+    def capture(self, k):
+        return self.data[self.offsets[k]: self.offsets[k] + self.lengths[k]]
+
+
+def make_batch(plan, device, seed=4, snr_db=40.0, ntemplates=2, jitter_ns=50.0, fs=FS, chunk=1 << 25,
+               templates=None):
+    """Generate every capture of `plan` (batch_plan) on `device` with torch:
+    each capture tiles its system's templates (field_template; a seeded
+    random template per repeat, from a seeded random phase of the first),
+    then the FM carrier is phase-accumulated in float64 over the whole
+    capture, the two analogue-audio FM carriers (as make_capture) and AWGN
+    are added and the result is quantised to 10-bit codes stored as
+    (code - 512) << 6.  Work proceeds in chunks of `chunk` samples."""
+    import torch
+    rng = np.random.default_rng(seed)
+    if templates is None:
+        templates = {}
+        for s in sorted({p[0] for p in plan}):
+            templates[s] = [field_template(s, seed * 1000 + 17 * t + (1 if s == "PAL" else 0), jitter_ns, fs)
+                            for t in range(ntemplates)]
+    tdev = {s: [torch.from_numpy(t).to(device) for t in ts] for s, ts in templates.items()}
+    total = sum(n for _, n in plan)
+    data = torch.empty(total, dtype=torch.int16, device=device)
+    offsets, o = [], 0
+    for _, n in plan:
+        offsets.append(o)
+        o += n
+    sigma = None if snr_db is None else float(np.sqrt(0.5 / 10 ** (snr_db / 10)))
+    peak = 1.2 + (6 * sigma if sigma else 0.0)
+    gen = torch.Generator(device=device)
+    gen.manual_seed(int(seed))
+    two_pi_fs = 2 * np.pi / fs
+    for k, (system, n) in enumerate(plan):
+        ts = tdev[system]
+        T = ts[0].numel()
+        reps = (n + T - 1) // T + 2
+        order = rng.integers(0, len(ts), size=reps)
+        phase0 = int(rng.integers(0, T))          # where in the first template the capture begins
+        sysp, _ = system_tables(system)
+        dev_hz = 100e3 if system == "NTSC" else 50e3
+        carriers = [(sysp["audio_lfreq"], 1000.0, dev_hz), (sysp["audio_rfreq"], 3000.0, dev_hz)]
+        carry = 0.0
+        for c0 in range(0, n, chunk):
+            m = min(chunk, n - c0)
+            hz = torch.empty(m, dtype=torch.float64, device=device)
+            p, filled = phase0 + c0, 0
+            while filled < m:
+                q, r = divmod(p, T)
+                take = min(T - r, m - filled)
+                hz[filled:filled + take] = ts[order[q]][r:r + take]
+                filled += take
+                p += take
+            inc = hz.mul_(two_pi_fs)
+            ph = torch.cumsum(inc, 0)
+            ph.sub_(inc).add_(carry)                     # exclusive sum: phase[0] = carry
+            carry = float(ph[-1] + inc[-1]) % (2 * np.pi)
+            del inc, hz
+            sig = torch.sin(ph)
+            t = torch.arange(c0, c0 + m, dtype=torch.float64, device=device).div_(fs)
+            for fc, fa, dv in carriers:
+                sig.add_(torch.sin(t * (2 * np.pi * fc) - (dv / fa) * torch.cos(t * (2 * np.pi * fa))), alpha=0.1)
+            del t, ph
+            if sigma:
+                sig.add_(torch.randn(m, dtype=torch.float64, device=device, generator=gen), alpha=sigma)
+            code = torch.round(sig.mul_(511.0 / peak).add_(512.0)).clamp_(0, 1023).to(torch.int32)
+            data[offsets[k] + c0: offsets[k] + c0 + m] = ((code - 512) << 6).to(torch.int16)
+            del sig, code
+    return Batch(data, [p[0] for p in plan], offsets, [p[1] for p in plan], fs, snr_db)
