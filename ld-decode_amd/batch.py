@@ -36,6 +36,11 @@ from . import locate as locatemod
 
 SYSTEMS = ("NTSC", "PAL")
 
+# algorithmic bytes per usable RF sample of the demodulation: the int16
+# input, the video channels (4 NTSC, 5 PAL) and rfhpf as float32, the two
+# audio stage-1 channels at 1/32 of the rate
+DEMOD_BYTES = {"NTSC": 2 + 4 * 4 + 4 + 2 * 4 / 32, "PAL": 2 + 5 * 4 + 4 + 2 * 4 / 32}
+
 
 class Lane:
     """Decode state of one capture between waves."""
@@ -193,11 +198,27 @@ class BatchDecoder:
         self.use_graph = bool(graph)
         self.stream_a = torch.cuda.Stream(device=self.device)
         self.stream_b = torch.cuda.Stream(device=self.device)
+        # demod_timing: a list receiving (start event, end event, algorithmic
+        # bytes) of every ld_demod_batch launch; stage A then runs without
+        # the graph so that the events bracket the launch alone
+        self.demod_timing = None
+        self.reset()
+
+    def reset(self):
+        """Start the batch again from every capture's beginning (buffers,
+        graphs and streams are kept)."""
+        for ln in self.lanes:
+            ln.start, ln.prev, ln.prov_delta, ln.phase_seed = 0.0, None, 0.0, 0.0
+            ln.fields_done, ln.res, ln.valid = 0, None, []
+            ln.done = ln.length < self.N
+        for st in self.sets:
+            st["free"] = None
         self.waves = 0
         self.fields_total = 0
         self.relocated = 0
         self.redone_waves = 0
         self.blocks_total = 0
+        self.blocks_by_system = {s: 0 for s in self.rfs}
 
     # ------------------------------------------------------------ planning
     def _plan(self, k):
@@ -246,7 +267,10 @@ class BatchDecoder:
             return False
         st["nslots"] = (ctypes.c_int32 * 2)(*[st["sys"][s].cap_blocks if s in st["sys"] else 0 for s in SYSTEMS])
         st["tbl_pin"].numpy()[:] = rows.view(np.uint8)
+        st["used"] = dict(used)
         self.blocks_total += i
+        for sname, nbk in used.items():
+            self.blocks_by_system[sname] += nbk
         return True
 
     # ------------------------------------------------------------ stage A
@@ -258,9 +282,16 @@ class BatchDecoder:
         for s, sset in st["sys"].items():
             outs[SYSTEMS.index(s)] = sset.outputs
         rf0 = next(iter(self.rfs.values()))
+        timing = self.demod_timing is not None
+        if timing:
+            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+            e0.record(cur)
         _lib.check(rf0.lib.ld_demod_batch(self.handles, self.data.data_ptr(), _lib.LD_IN_INT16, self.data.numel(),
                                           st["tbl_dev"].data_ptr(), self.table_cap, st["nslots"], 1, outs,
                                           self.ws.data_ptr(), self.ws.numel(), ctypes.c_void_p(cur.cuda_stream)))
+        if timing:
+            e1.record(cur)
+            self.demod_timing.append((e0, e1, sum(n * self.S * DEMOD_BYTES[s] for s, n in st["used"].items())))
         for s, sset in st["sys"].items():
             sset.finder.launch(sset.buffers["demod_05"], sset.total, cur,
                                bits=(sset.buffers["demod_05_below"], sset.outputs.below_high), record=False)
@@ -271,7 +302,7 @@ class BatchDecoder:
         with torch.cuda.stream(self.stream_a):
             if st["free"] is not None:
                 self.stream_a.wait_event(st["free"])
-            if self.use_graph:
+            if self.use_graph and self.demod_timing is None:
                 if st["graph"] is None:
                     # nslots and the table's address are fixed per set: one
                     # capture per set, replayed every wave
