@@ -73,41 +73,81 @@ def workload(torch, rfs, seed):
 
 
 # ------------------------------------------------------------ C4 batch
+def batch_decoders(args, rfs, b):
+    """The batch split into `args.groups` decoders of about equal samples
+    (LPT), each with its own streams and buffer sets, run taking turns on
+    the host."""
+    from lddecode_amd.batch import BatchDecoder, split_captures
+    decs = []
+    for grp in split_captures(b.lengths, args.groups):
+        decs.append(BatchDecoder(rfs, b.data, [b.systems[k] for k in grp], [b.offsets[k] for k in grp],
+                                 [b.lengths[k] for k in grp], wave_blocks=args.wave_blocks // args.groups,
+                                 graph=True, audio=True))
+    return decs
+
+
 def run_c4(args, torch, rfs, b):
-    from lddecode_amd.batch import BatchDecoder
-    dec = BatchDecoder(rfs, b.data, b.systems, b.offsets, b.lengths, wave_blocks=args.wave_blocks, graph=True,
-                       audio=True)
+    from lddecode_amd.batch import run_interleaved
+    decs = batch_decoders(args, rfs, b)
+
+    def one_pass():
+        for d in decs:
+            d.reset()
+        return run_interleaved(decs)
+
     for i in range(args.warmup):
-        dec.reset()
         t = time.perf_counter()
-        dec.run()
+        nf = one_pass()
         torch.cuda.synchronize()
-        log(f"warm-up pass {i}: {dec.fields_total} fields, {dec.waves} waves, {time.perf_counter() - t:.2f} s")
+        log(f"warm-up pass {i}: {nf} fields, {sum(d.waves for d in decs)} waves, {time.perf_counter() - t:.2f} s")
+    for d in decs:
+        d.demod_timing = []       # events around every wave's demod graph replay
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for i in range(args.steps):
-        dec.reset()
-        dec.run()
-        log(f"timed pass {i}: {dec.fields_total} fields in {dec.waves} waves")
+        nf = one_pass()
+        log(f"timed pass {i}: {nf} fields")
     torch.cuda.synchronize()
     wall = time.perf_counter() - t0
-    out = dict(ms_per_step=wall / args.steps * 1e3, fields=dec.fields_total, waves=dec.waves,
-               blocks=dec.blocks_total, blocks_by_system=dict(dec.blocks_by_system), relocated=dec.relocated,
-               redone_waves=dec.redone_waves, samples=dec.samples_total)
-    # one more pass with events around every ld_demod_batch launch (stage A
-    # without the graph): the dominant kernel's duration per launch
-    dec.demod_timing = []
-    dec.reset()
-    t = time.perf_counter()
-    dec.run()
-    torch.cuda.synchronize()
-    out["eager_pass_s"] = time.perf_counter() - t
-    ms = [e0.elapsed_time(e1) for e0, e1, _ in dec.demod_timing]
-    by = [n for _, _, n in dec.demod_timing]
+    tim = [x for d in decs for x in d.demod_timing]
+    ms = [e0.elapsed_time(e1) for e0, e1, _ in tim]
+    by = [n for _, _, n in tim]
+    for d in decs:
+        d.demod_timing = None
+    bys = {}
+    for d in decs:
+        for s, n in d.blocks_by_system.items():
+            bys[s] = bys.get(s, 0) + n
+    out = dict(ms_per_step=wall / args.steps * 1e3, fields=nf, waves=sum(d.waves for d in decs),
+               blocks=sum(d.blocks_total for d in decs), blocks_by_system=bys,
+               relocated=sum(d.relocated for d in decs), redone_waves=sum(d.redone_waves for d in decs),
+               samples=sum(d.samples_total for d in decs))
     out.update(demod_launches=len(ms), demod_ms_avg=float(np.mean(ms)), demod_bytes_avg=float(np.mean(by)),
-               demod_ms_total=float(np.sum(ms)))
-    dec.demod_timing = None
+               demod_ms_total=float(np.sum(ms)) / args.steps)
+    # the same launches, each alone on the GPU (synchronised around it): the
+    # kernel's own rate without the other decoder's concurrent stage B
+    iso = iso_demod(torch, decs[0])
+    out.update(iso)
     return out
+
+
+def iso_demod(torch, dec, reps=5):
+    """The first wave's demod launch of a pass (a full-size wave), run alone
+    on the GPU `reps` times (nothing else queued beside it)."""
+    rows = []
+    for _ in range(reps):
+        dec.reset()
+        torch.cuda.synchronize()
+        dec.demod_timing = []
+        it = dec.waves_iter()
+        next(it)                      # plans wave 0 and queues its stage A only
+        torch.cuda.synchronize()
+        rows += [(e0.elapsed_time(e1), n) for e0, e1, n in dec.demod_timing]
+        it.close()
+    dec.demod_timing = None
+    dec.reset()
+    return dict(iso_ms_avg=float(np.mean([r[0] for r in rows])), iso_bytes_avg=float(np.mean([r[1] for r in rows])),
+                iso_launches=len(rows))
 
 
 # ------------------------------------------------------------ plain paths
@@ -299,16 +339,16 @@ def profile_step(args):
     the HBM byte counters."""
     __graft_entry__.load_package()
     import torch
-    from lddecode_amd.batch import BatchDecoder
+    from lddecode_amd.batch import run_interleaved
     rfs = decoders(torch)
     b = workload(torch, rfs, args.seed)
-    dec = BatchDecoder(rfs, b.data, b.systems, b.offsets, b.lengths, wave_blocks=args.wave_blocks, graph=True,
-                       audio=True)
+    decs = batch_decoders(args, rfs, b)
     for i in range(args.steps):
-        dec.reset()
-        dec.run()
+        for d in decs:
+            d.reset()
+        nf = run_interleaved(decs)
         torch.cuda.synchronize()
-        log(f"profile pass {i}: {dec.fields_total} fields, {dec.waves} waves")
+        log(f"profile pass {i}: {nf} fields")
     x = torch.ones(COPY_ELEMS, dtype=torch.float32, device=rfs["NTSC"].device)
     y = torch.empty_like(x)
     torch.neg(x, out=y)
@@ -363,16 +403,17 @@ def measure_traffic(args):
         kernels[k] = {"dispatches": len(fetch[k]), "fetch_bytes": round(f), "write_bytes": round(w),
                       "traffic_bytes": round(f + w)}
     res = {"source_hash": kernel_source_hash(), "seed": args.seed, "wave_blocks": args.wave_blocks,
+           "groups": args.groups,
            "copy_bytes_each_way": COPY_ELEMS * 4, "copy_fetch_kib": fc, "copy_write_kib": wc,
            "fetch_scale": fscale, "write_scale": wscale,
-           "command": f"python bench.py --measure-traffic --wave-blocks {args.wave_blocks}", "kernels": kernels}
+           "command": f"python bench.py --measure-traffic --wave-blocks {args.wave_blocks} --groups {args.groups}", "kernels": kernels}
     dest = pathlib.Path(args.traffic_out) if args.traffic_out else TRAFFIC_FILE
     dest.parent.mkdir(parents=True, exist_ok=True)
     dest.write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res, indent=1))
 
 
-def load_traffic(kernel, wave_blocks, seed):
+def load_traffic(kernel, wave_blocks, seed, groups):
     """Measured HBM bytes per launch of `kernel` in the C4 pass, or None when
     the file is missing or was measured on other kernel sources / workload."""
     try:
@@ -381,7 +422,7 @@ def load_traffic(kernel, wave_blocks, seed):
         return None, "no traffic file"
     if res.get("source_hash") != kernel_source_hash():
         return None, "traffic file measured on other kernel sources (stale): not used"
-    if res.get("wave_blocks") != wave_blocks or res.get("seed") != seed:
+    if res.get("wave_blocks") != wave_blocks or res.get("seed") != seed or res.get("groups") != groups:
         return None, "traffic file measured on another workload: not used"
     k = res.get("kernels", {}).get(kernel)
     return (None, "kernel not in traffic file") if k is None else (k["traffic_bytes"], "measured")
@@ -395,6 +436,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=4)
     ap.add_argument("--wave-blocks", type=int, default=8192, help="blocks per wave of the batched decode")
+    ap.add_argument("--groups", type=int, default=2, help="batch decoders taking turns on the host")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="minimum CPU baseline time (0: skip)")
     ap.add_argument("--profile-step", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--measure-traffic", action="store_true",
@@ -442,7 +484,8 @@ def main():
         log(f"packed run failed: {e}")
 
     achieved = c4["demod_bytes_avg"] / (c4["demod_ms_avg"] / 1e3) / 1e9
-    traffic, traffic_note = load_traffic("ldgpu::k_demod_batch", args.wave_blocks, args.seed)
+    iso_gbs = c4["iso_bytes_avg"] / (c4["iso_ms_avg"] / 1e3) / 1e9
+    traffic, traffic_note = load_traffic("ldgpu::k_demod_batch", args.wave_blocks, args.seed, args.groups)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -483,13 +526,14 @@ def main():
             "samples_per_step": c4["samples"],
             "blocklen": 32768,
             "wave_blocks": args.wave_blocks,
+            "decoders": args.groups,
         },
         "x_realtime": round(value / 40.0, 1),
         "batch": {"fields_per_step": c4["fields"], "waves": c4["waves"], "blocks_demodulated": c4["blocks"],
                   "blocks_by_system": c4["blocks_by_system"],
                   "overlap_recompute": round(c4["blocks"] * 31712 / c4["samples"] - 1, 4),
                   "relocated_rounds": c4["relocated"], "redone_waves": c4["redone_waves"],
-                  "demod_launches": c4["demod_launches"], "eager_pass_s": round(c4["eager_pass_s"], 3),
+                  "demod_launches": c4["demod_launches"],
                   "demod_ms_per_pass": round(c4["demod_ms_total"], 2)},
         "plain": {
             "c1_block_ntsc": {"value": round(c1["NTSC"]["msps"], 2), "unit": "Msamples/s",
@@ -514,8 +558,13 @@ def main():
                      "traffic_note": traffic_note,
                      "algorithmic_bytes": round(c4["demod_bytes_avg"]),
                      "kernel_ms": round(c4["demod_ms_avg"], 4),
-                     "kernel": "k_demod_batch, average launch of a C4 pass (22.25 B per usable NTSC sample, "
-                               "26.25 PAL: input, video channels, rfhpf, audio stage 1)"},
+                     "kernel": "k_demod_batch: events around every wave's demod graph replay (block table upload "
+                               "+ the launch) in the timed passes, beside the stage B kernels; algorithmic "
+                               "22.25 B per usable NTSC sample, 26.25 PAL (input, video channels, rfhpf, audio "
+                               "stage 1)",
+                     "isolated": {"achieved": round(iso_gbs, 1), "frac": round(iso_gbs / HBM_PEAK_GBS, 4),
+                                  "kernel_ms": round(c4["iso_ms_avg"], 4), "launches": c4["iso_launches"],
+                                  "note": "the first waves' launches, each alone on the GPU"}},
         "cpu_baseline": cpu,
     }
     print(json.dumps(line), flush=True)

@@ -191,9 +191,10 @@ typedef struct ld_batch_block {
  * system's tables (the same arithmetic as ld_demod_blocks) and writes output
  * slot blocks[i].slot.  The table is DEVICE memory (so a captured HIP graph
  * can replay the launch with new contents); entries that would read outside
- * the input or write outside nslots are skipped on the device.  int16 or
- * float32 input, no EFM output.  workspace: ld_demod_workspace_bytes(h,
- * nblocks).  No allocation, copy or memset: safe under stream capture. */
+ * the input or write outside nslots are skipped on the device (padding
+ * entries: sys = -1).  int16 or float32 input, no EFM output.  workspace:
+ * ld_demod_workspace_bytes(h, nblocks).  No allocation, copy or memset: safe
+ * under stream capture. */
 int ld_demod_batch(ld_handle* const* sys, const void* input, int input_dtype, int64_t input_len,
                    const ld_batch_block* blocks, int nblocks, const int32_t* nslots, int cut,
                    const ld_demod_outputs* outs, void* workspace, size_t workspace_bytes, void* stream);

@@ -138,7 +138,7 @@ class BatchDecoder:
     ``torch.cuda.synchronize()`` before reading them)."""
 
     def __init__(self, rfs, data, systems, offsets, lengths, wave_blocks=8192, min_fields=16, graph=True,
-                 audio=True, coord_f32=True, on_wave=None):
+                 audio=True, coord_f32=True, on_wave=None, streams=None):
         self.rfs = dict(rfs)
         rf0 = next(iter(self.rfs.values()))
         self.torch = torch = rf0.torch
@@ -189,18 +189,22 @@ class BatchDecoder:
             tbl_np = np.ctypeslib.as_array(tbl_host).view(np.uint8)
             tbl_pin = torch.empty(tbl_np.size, dtype=torch.uint8, pin_memory=True)
             tbl_dev = torch.empty(tbl_np.size, dtype=torch.uint8, device=self.device)
-            self.sets.append({"sys": sets, "tbl_pin": tbl_pin, "tbl_dev": tbl_dev, "graph": None,
-                              "free": None, "lanes": [], "wave": -1})
+            self.sets.append({"sys": sets, "tbl_pin": tbl_pin, "tbl_dev": tbl_dev, "graph_demod": None,
+                              "graph_pulses": None, "free": None, "lanes": [], "wave": -1})
         self.ws = torch.empty(max(1, rf0.lib.ld_demod_workspace_bytes(rf0.handle, self.table_cap)),
                               dtype=torch.uint8, device=self.device)
         self.handles = (ctypes.c_void_p * 2)(*[self.rfs[s].handle.value if s in self.rfs else None
                                                for s in SYSTEMS])
         self.use_graph = bool(graph)
-        self.stream_a = torch.cuda.Stream(device=self.device)
-        self.stream_b = torch.cuda.Stream(device=self.device)
+        # streams: (A, B) shared with other decoders taking turns (their
+        # demodulations then run one after the other, as they would fill the
+        # GPU anyway); default: a pair of this decoder's own
+        if streams is None:
+            streams = (torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device))
+        self.stream_a, self.stream_b = streams
         # demod_timing: a list receiving (start event, end event, algorithmic
-        # bytes) of every ld_demod_batch launch; stage A then runs without
-        # the graph so that the events bracket the launch alone
+        # bytes) of every wave's demod graph (the block table upload and the
+        # ld_demod_batch launch) on stream A
         self.demod_timing = None
         self.reset()
 
@@ -274,7 +278,8 @@ class BatchDecoder:
         return True
 
     # ------------------------------------------------------------ stage A
-    def _stage_a_body(self, st):
+    def _demod_body(self, st):
+        """The wave's block table upload and its one ld_demod_batch launch."""
         torch = self.torch
         cur = torch.cuda.current_stream(self.device)
         st["tbl_dev"].copy_(st["tbl_pin"], non_blocking=True)
@@ -282,19 +287,33 @@ class BatchDecoder:
         for s, sset in st["sys"].items():
             outs[SYSTEMS.index(s)] = sset.outputs
         rf0 = next(iter(self.rfs.values()))
-        timing = self.demod_timing is not None
-        if timing:
-            e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-            e0.record(cur)
         _lib.check(rf0.lib.ld_demod_batch(self.handles, self.data.data_ptr(), _lib.LD_IN_INT16, self.data.numel(),
                                           st["tbl_dev"].data_ptr(), self.table_cap, st["nslots"], 1, outs,
                                           self.ws.data_ptr(), self.ws.numel(), ctypes.c_void_p(cur.cuda_stream)))
-        if timing:
-            e1.record(cur)
-            self.demod_timing.append((e0, e1, sum(n * self.S * DEMOD_BYTES[s] for s, n in st["used"].items())))
+
+    def _pulses_body(self, st):
+        """Per system the pulse search over the demod's in-pulse bits, the
+        pulses' levels and their copies to pinned host memory."""
+        cur = self.torch.cuda.current_stream(self.device)
         for s, sset in st["sys"].items():
             sset.finder.launch(sset.buffers["demod_05"], sset.total, cur,
                                bits=(sset.buffers["demod_05_below"], sset.outputs.below_high), record=False)
+
+    def _replay(self, st, name, body):
+        """Run `body` as a HIP graph captured once per buffer set (its
+        pointers and sizes are fixed per set; the table's contents change
+        per wave) and replayed on stream A."""
+        torch = self.torch
+        if not self.use_graph:
+            body(st)
+            return
+        if st.get(name) is None:
+            g = torch.cuda.CUDAGraph()
+            self.stream_a.synchronize()
+            with torch.cuda.graph(g, stream=self.stream_a):
+                body(st)
+            st[name] = g
+        st[name].replay()
 
     def _stage_a(self, k):
         torch = self.torch
@@ -302,18 +321,15 @@ class BatchDecoder:
         with torch.cuda.stream(self.stream_a):
             if st["free"] is not None:
                 self.stream_a.wait_event(st["free"])
-            if self.use_graph and self.demod_timing is None:
-                if st["graph"] is None:
-                    # nslots and the table's address are fixed per set: one
-                    # capture per set, replayed every wave
-                    g = torch.cuda.CUDAGraph()
-                    self.stream_a.synchronize()
-                    with torch.cuda.graph(g, stream=self.stream_a):
-                        self._stage_a_body(st)
-                    st["graph"] = g
-                st["graph"].replay()
-            else:
-                self._stage_a_body(st)
+            timing = self.demod_timing is not None
+            if timing:
+                e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                e0.record(self.stream_a)
+            self._replay(st, "graph_demod", self._demod_body)
+            if timing:
+                e1.record(self.stream_a)
+                self.demod_timing.append((e0, e1, sum(n * self.S * DEMOD_BYTES[s] for s, n in st["used"].items())))
+            self._replay(st, "graph_pulses", self._pulses_body)
             ev = torch.cuda.Event()
             ev.record(self.stream_a)
         st["pulses_done"] = ev
@@ -364,17 +380,21 @@ class BatchDecoder:
             # positions round to multiples of blocklen
             locs, ll1, bad1, prev_out, nxt, vp = w.walk(view, ln.start_at_wave, ln.prev_at_start, fk,
                                                         edges_means=(e, m), origin=b0 * S)
-            ln.res = (list(locs), np.array(ll1), np.array(bad1), prev_out, nxt, np.array(vp))
+            # views of the lane's own walker buffers: valid until its next walk
+            ln.res = (list(locs), ll1, bad1, prev_out, nxt, vp)
             ln.valid = [j for j, c in enumerate(locs) if c.status == _lib.LD_LOC_OK]
             per_sys[ln.system].append(ln)
         return per_sys
 
     def _fields(self, st, s, lanes):
-        """The system's field list and line arrays from its lanes' walks."""
+        """The system's field plan and line arrays from its lanes' walks."""
         sset = st["sys"][s]
         rf = self.rfs[s]
         S, L = self.S, sset.L
-        fields, lls, bads = [], [], []
+        pal = s == "PAL"
+        cols = {k: [] for k in ("demod_offset", "demod_len", "linecount", "first_field", "lineoffset", "chain",
+                                "lane")}
+        lls, bads = [], []
         chain = 0
         row = 0
         for ln in lanes:
@@ -383,23 +403,28 @@ class BatchDecoder:
             ln.row0 = row
             if not ln.valid:
                 continue
+            base = (slot0 - b0) * S
             for j in ln.valid:
                 c = locs[j]
-                first = bool(c.is_first_field)
-                lo = (2 if first else 3) if s == "PAL" else 0
-                fields.append(dict(demod_offset=slot0 * S + int(c.startloc) - b0 * S, demod_len=int(c.buflen), nlines=L,
-                                   linecount=int(c.linecount), first_field=int(first), lineoffset=lo,
-                                   linesout=sset.linesout, outwidth=sset.W, inlinelen=float(rf.linelen),
-                                   lineloc_offset=row * L, out_offset=row * sset.linesout * sset.W, chain=chain,
-                                   lane=ln.idx))
-                row += 1
+                first = int(c.is_first_field)
+                cols["demod_offset"].append(base + int(c.startloc))
+                cols["demod_len"].append(min(int(c.buflen), sset.total - base - int(c.startloc)))
+                cols["linecount"].append(int(c.linecount))
+                cols["first_field"].append(first)
+                cols["lineoffset"].append((2 if first else 3) if pal else 0)
+                cols["chain"].append(chain)
+                cols["lane"].append(ln.idx)
+            row += len(ln.valid)
             lls.append(ll1[ln.valid])
             bads.append(fieldmod.compute_deriv_error_rows(ll1[ln.valid], bad1[ln.valid].astype(bool)))
             chain += 1
         if row > sset.max_fields:
             raise RuntimeError(f"{s}: {row} fields exceed the set's {sset.max_fields}")
-        for f in fields:
-            f["demod_len"] = min(f["demod_len"], sset.total - f["demod_offset"])
+        r = np.arange(row, dtype=np.int64)
+        cols.update(nlines=[L] * row, linesout=[sset.linesout] * row, outwidth=[sset.W] * row,
+                    inlinelen=[float(rf.linelen)] * row, lineloc_offset=(r * L).tolist(),
+                    out_offset=(r * (sset.linesout * sset.W)).tolist())
+        fields = fieldmod.plan_from_columns(cols)
         sset.fields = fields
         sset.nchains = chain
         if row:
@@ -540,17 +565,18 @@ class BatchDecoder:
             ln.prov_delta = float(np.median(d))
 
     # ------------------------------------------------------------ driver
-    def run(self, max_waves=None):
-        """Decode every capture to its end.  Returns the fields decoded."""
+    def waves_iter(self, max_waves=None):
+        """The decode as a generator that yields before each call that waits
+        for the GPU (a wave's pulses before its walk, its facts before its
+        check), so that several decoders can take turns on the host while
+        the GPU works on the others' queued stages (run_interleaved)."""
         k = 0
         if not self._plan(0):
-            return 0
+            return
         self._stage_a(0)
-        pending = None            # wave whose stage B is queued but not verified
         while True:
+            yield
             per_sys = self._walk(k)
-            for ln in self.sets[k % 2]["lanes"]:
-                ln.start_walked = ln.res[4]
             self._queue_stage_b(k, per_sys)
             if self.on_wave is not None:
                 self.on_wave(self._info(k))
@@ -564,23 +590,35 @@ class BatchDecoder:
                 self._stage_a(k + 1)
             for idx, v in saved.items():
                 self.lanes[idx].start = v
+            yield
             moved = self._verify(k)
             self.waves += 1
             if moved and nxt_ok:
                 # a re-walked lane moves: plan and demodulate the next wave again
                 self.redone_waves += 1
-                self.torch.cuda.current_stream(self.device).wait_stream(self.stream_a)
-                self.sets[(k + 1) % 2]["pulses_done"].synchronize()
-                self.blocks_total -= sum(g[1] for g in self.sets[(k + 1) % 2]["geom"].values())
+                nxt = self.sets[(k + 1) % 2]
+                nxt["pulses_done"].synchronize()
+                self.blocks_total -= sum(g[1] for g in nxt["geom"].values())
+                for sname, nbk in nxt["used"].items():
+                    self.blocks_by_system[sname] -= nbk
                 nxt_ok = self._plan(k + 1)
                 if nxt_ok:
                     self._stage_a(k + 1)
             if not nxt_ok:
                 break
             k += 1
+
+    def join(self):
+        """Make torch's current stream wait for everything queued."""
         cur = self.torch.cuda.current_stream(self.device)
         cur.wait_stream(self.stream_a)
         cur.wait_stream(self.stream_b)
+
+    def run(self, max_waves=None):
+        """Decode every capture to its end.  Returns the fields decoded."""
+        for _ in self.waves_iter(max_waves):
+            pass
+        self.join()
         return self.fields_total
 
     def _info(self, k):
@@ -595,3 +633,34 @@ class BatchDecoder:
     @property
     def samples_total(self):
         return sum(ln.length for ln in self.lanes)
+
+
+def run_interleaved(decoders):
+    """Run several BatchDecoders (each with its own streams and buffers)
+    taking turns on the host: while one waits for or walks its wave, the
+    GPU runs the stages the others have queued.  Returns the fields decoded."""
+    its = [d.waves_iter() for d in decoders]
+    while its:
+        nxt = []
+        for it in its:
+            try:
+                next(it)
+                nxt.append(it)
+            except StopIteration:
+                pass
+        its = nxt
+    for d in decoders:
+        d.join()
+    return sum(d.fields_total for d in decoders)
+
+
+def split_captures(lengths, groups):
+    """Indices of the captures per group, longest first onto the group with
+    the fewest samples (LPT), each group keeping the batch's order."""
+    load = [0] * groups
+    out = [[] for _ in range(groups)]
+    for k in sorted(range(len(lengths)), key=lambda k: -lengths[k]):
+        g = min(range(groups), key=lambda g: load[g])
+        out[g].append(k)
+        load[g] += lengths[k]
+    return [sorted(o) for o in out]

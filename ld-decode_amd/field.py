@@ -69,7 +69,9 @@ def tbc_fields(rf, demod, fields, linelocs, coord_f32=True, smoothing=0.0, thres
         upload_sinc_lut(rf)
     nf = len(fields)
     arr = _field_array(fields)
-    total = max(f["out_offset"] + f["linesout"] * f["outwidth"] for f in fields)
+    total = getattr(fields, "total_out", None)
+    if total is None:
+        total = max(f["out_offset"] + f["linesout"] * f["outwidth"] for f in fields)
     if final and out_u16 is None:
         out_u16 = torch.empty(total, dtype=torch.uint16, device=rf.device)
     if not final and out_f32 is None:
@@ -100,6 +102,32 @@ class FieldPlan(list):
 
 
 _DTYPES = {}
+
+
+def _columns_array(ctype, n, cols):
+    dt = _DTYPES.get(ctype)
+    if dt is None:
+        dt = _DTYPES[ctype] = np.dtype(ctype)
+    a = np.zeros(max(n, 1), dt)
+    for k, _ in ctype._fields_:
+        if k in cols:
+            a[k][:n] = cols[k]
+    return (ctype * n).from_buffer(a)
+
+
+def plan_from_columns(cols):
+    """A FieldPlan from per-key columns (equal-length sequences): the field
+    dicts plus the TBC and line struct arrays built column-wise at once."""
+    keys = list(cols)
+    n = len(cols[keys[0]]) if keys else 0
+    plan = FieldPlan(dict(zip(keys, vals)) for vals in zip(*(cols[k] for k in keys)))
+    arrays = plan.__dict__.setdefault("_arrays", {})
+    arrays[_lib.TbcField] = _columns_array(_lib.TbcField, n, cols)
+    arrays[_lib.LineField] = _columns_array(_lib.LineField, n, cols)
+    if n:
+        plan.total_out = int(np.max(np.asarray(cols["out_offset"]) +
+                                    np.asarray(cols["linesout"]) * np.asarray(cols["outwidth"])))
+    return plan
 
 
 def _struct_array(ctype, fields, keys, default=None):
