@@ -76,13 +76,15 @@ def workload(torch, rfs, seed):
 def batch_decoders(args, rfs, b):
     """The batch split into `args.groups` decoders of about equal samples
     (LPT), each with its own streams and buffer sets, run taking turns on
-    the host."""
+    the host; their demod launches run one after the other (each fills the
+    GPU), so the events around each measure that launch."""
     from lddecode_amd.batch import BatchDecoder, split_captures
+    order = {}
     decs = []
     for grp in split_captures(b.lengths, args.groups):
         decs.append(BatchDecoder(rfs, b.data, [b.systems[k] for k in grp], [b.offsets[k] for k in grp],
                                  [b.lengths[k] for k in grp], wave_blocks=args.wave_blocks // args.groups,
-                                 graph=True, audio=True))
+                                 graph=True, audio=True, demod_order=order))
     return decs
 
 

@@ -138,7 +138,7 @@ class BatchDecoder:
     ``torch.cuda.synchronize()`` before reading them)."""
 
     def __init__(self, rfs, data, systems, offsets, lengths, wave_blocks=8192, min_fields=16, graph=True,
-                 audio=True, coord_f32=True, on_wave=None, streams=None):
+                 audio=True, coord_f32=True, on_wave=None, streams=None, demod_order=None):
         self.rfs = dict(rfs)
         rf0 = next(iter(self.rfs.values()))
         self.torch = torch = rf0.torch
@@ -202,6 +202,10 @@ class BatchDecoder:
         if streams is None:
             streams = (torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device))
         self.stream_a, self.stream_b = streams
+        # demod_order: a dict shared by decoders taking turns; each demod
+        # launch waits for the one queued before it by any of them (one
+        # demodulation fills the GPU; their order then follows the host's)
+        self.demod_order = demod_order
         # demod_timing: a list receiving (start event, end event, algorithmic
         # bytes) of every wave's demod graph (the block table upload and the
         # ld_demod_batch launch) on stream A
@@ -217,6 +221,8 @@ class BatchDecoder:
             ln.done = ln.length < self.N
         for st in self.sets:
             st["free"] = None
+        if self.demod_order is not None:
+            self.demod_order["last"] = None
         self.waves = 0
         self.fields_total = 0
         self.relocated = 0
@@ -321,6 +327,8 @@ class BatchDecoder:
         with torch.cuda.stream(self.stream_a):
             if st["free"] is not None:
                 self.stream_a.wait_event(st["free"])
+            if self.demod_order is not None and self.demod_order.get("last") is not None:
+                self.stream_a.wait_event(self.demod_order["last"])
             timing = self.demod_timing is not None
             if timing:
                 e0, e1 = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
@@ -329,6 +337,10 @@ class BatchDecoder:
             if timing:
                 e1.record(self.stream_a)
                 self.demod_timing.append((e0, e1, sum(n * self.S * DEMOD_BYTES[s] for s, n in st["used"].items())))
+            if self.demod_order is not None:
+                done = torch.cuda.Event()
+                done.record(self.stream_a)
+                self.demod_order["last"] = done
             self._replay(st, "graph_pulses", self._pulses_body)
             ev = torch.cuda.Event()
             ev.record(self.stream_a)
