@@ -171,6 +171,26 @@ int ld_demod_blocks(ld_handle* h, const void* input, int input_dtype, int64_t in
                     const ld_demod_outputs* out, void* workspace, size_t workspace_bytes,
                     void* stream);
 
+/* Transform variants of the demodulator (ld_set_fft_variant).
+ * LD_FFT_FP32: FP32 Stockham passes (radix 16 x 16 x 16 x 4 for the 16384-point
+ * complex transforms of 32768-sample blocks), the default.
+ * LD_FFT_MFMA_BF16X3: the two radix-64 stages of 16384 = 64 x 64 x 4 as real
+ * matrix products on the matrix cores with both operands split into bf16
+ * hi + lo parts (hi*hi + hi*lo + lo*hi, FP32 accumulation); twiddles and the
+ * radix-4 stage stay FP32.  About 2^-17 relative error per stage. */
+enum { LD_FFT_FP32 = 0, LD_FFT_MFMA_BF16X3 = 1 };
+
+/* Select the transform variant of later ld_demod_blocks / ld_demod_batch
+ * launches on this handle (int16 / float32 input without EFM output for
+ * the MFMA variant).  No counterpart in the reference (numpy's FFT). */
+int ld_set_fft_variant(ld_handle* h, int variant);
+
+/* The demodulator's forward complex transform alone (blocklen / 2 points),
+ * for checking a variant's accuracy: in / out are DEVICE float2 arrays of
+ * ntransforms * blocklen / 2 points.  Diagnostic; no counterpart in the
+ * reference (its scipy.fft calls, core.py:761, :805, :809-820). */
+int ld_fft_forward(ld_handle* h, const void* in, void* out, int ntransforms, int variant, void* stream);
+
 /* One block of a batched launch (ld_demod_batch), in a DEVICE table:
  * `first` is its first input sample, `sys` its system (0 or 1: which
  * handle's tables and outputs; < 0: padding, skipped), `slot` where its

@@ -14,6 +14,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().with_name("libldgpu.so")
 LD_OK, LD_EINVAL, LD_ERANGE, LD_EHIP, LD_ESTATE = 0, -1, -2, -3, -4
 LD_FILT_RFHPF, LD_FILT_RFVIDEO, LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT, LD_FILT_EFM = range(7)
 LD_IN_INT16, LD_IN_FLOAT32, LD_IN_PACKED10 = 0, 1, 2
+LD_FFT_FP32, LD_FFT_MFMA_BF16X3 = 0, 1
 ABI_VERSION = 1
 
 # every symbol include/ldgpu.h declares
@@ -29,7 +30,7 @@ EXPORTS = (
     "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
     "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
     "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc", "ld_refine_burst_ntsc_chains",
-    "ld_demod_batch",
+    "ld_demod_batch", "ld_set_fft_variant", "ld_fft_forward",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -170,6 +171,8 @@ def _declare(lib):
         "ld_demod_blocks": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,
                                     P(DemodOutputs), c_vp, c_sz, c_vp]),
         "ld_demod_noalloc_max_bytes": (c_sz, []),
+        "ld_set_fft_variant": (c_int, [c_vp, c_int]),
+        "ld_fft_forward": (c_int, [c_vp, c_vp, c_vp, c_int, c_int, c_vp]),
         "ld_demod_batch": (c_int, [P(c_vp), c_vp, c_int, c_i64, c_vp, c_int, P(ctypes.c_int32), c_int,
                                    P(DemodOutputs), c_vp, c_sz, c_vp]),
         "ld_demod_blocks_noalloc": (c_int, [c_vp, c_vp, c_int, c_i64, c_i64, c_i64, c_int, c_int,

@@ -165,6 +165,8 @@ struct DemodArgs {
   const float2* tw_rows;   // Stockham pass twiddles of the audio transform (TwLayout)
   const float2* tw_lds;    // TwLds image, copied to LDS by every workgroup
   const float2* tw_split;  // split[t] = exp(-2 pi i t / N), t < kT
+  const u32x4_t* mfma_g;   // split-BF16 MFMA variant: DFT-64 fragments (kMfmaTabEntries)
+  const MfmaTw* mfma_tw;   // and its twiddle factor tables, copied to LDS
   const float2* h_rfv;
   C2rOut hpf;              // out == nullptr: skip
   C2rOut post[4];
@@ -476,9 +478,13 @@ __device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int t
 //
 // s0: the block's first input sample; bo: its output slot (outputs at bo
 // times the strides); bw: its workspace slot.
-template <bool kPacked, bool kEfm>
+//
+// kMfma: the transforms' radix-64 stages run as split-BF16 matrix products on
+// the matrix cores (WgFft16kMfma) instead of FP32 radix-16 passes.
+template <bool kPacked, bool kEfm, bool kMfma = false>
 __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int bo, int bw) {
   __shared__ Lds s;
+  __shared__ MfmaTw mtw;
   __shared__ float2 aslice[2][kNA / 2 + 1];
   __shared__ TwLds twl;
   __shared__ float notch_mag[kNotchMax];
@@ -500,6 +506,10 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     }
     if constexpr (kEfm) {
       if (tid == kT - 1) bin_m[6] = bload2(make_rsrc(a.h_efm, kHalfBytes), 0u, kM * 8u);
+    }
+    if constexpr (kMfma) {
+      const rsrc_t mt = make_rsrc(a.mfma_tw, sizeof(MfmaTw));
+      if (tid < (int)(sizeof(MfmaTw) / 8)) reinterpret_cast<float2*>(&mtw)[tid] = bload2(mt, (uint32_t)tid * 8u, 0u);
     }
   }
   __syncthreads();
@@ -651,7 +661,10 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     }
 
     // ---------------------------------------------------------- transform
-    FwdM::run(v, s.re, s.im, twf, tidv);
+    if constexpr (kMfma)
+      WgFft16kMfma::run(v, s.re, s.im, twf, a.mfma_g, &mtw, tidv);
+    else
+      FwdM::run(v, s.re, s.im, twf, tidv);
 
     // ---------------------------------------------------------- post
     if (ph == kPhIn || ph == kPhDemod) {
@@ -731,6 +744,9 @@ __global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<f
 __global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true, false>(LD_DEMOD_AT(a)); }
 __global__ __launch_bounds__(kT) void k_demod_block_efm(DemodArgs a) { demod_block<false, true>(LD_DEMOD_AT(a)); }
 __global__ __launch_bounds__(kT) void k_demod_block_packed_efm(DemodArgs a) { demod_block<true, true>(LD_DEMOD_AT(a)); }
+__global__ __launch_bounds__(kT) void k_demod_block_mfma(DemodArgs a) {
+  demod_block<false, false, true>(LD_DEMOD_AT(a));
+}
 #undef LD_DEMOD_AT
 
 // Blocks of many captures and both systems in one launch: launch block i
@@ -747,12 +763,51 @@ struct DemodBatchArgs {
   int nslots[2];
 };
 static_assert(sizeof(DemodBatchArgs) <= 4096, "kernel arguments over 4 KiB");
-__global__ __launch_bounds__(kT) void k_demod_batch(DemodBatchArgs a) {
+template <bool kMfma>
+__device__ __forceinline__ void demod_batch(const DemodBatchArgs& a) {
   const ld_batch_block d = a.blocks[blockIdx.x];
   if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - kN || d.slot < 0 ||
       d.slot >= a.nslots[d.sys])
     return;
-  demod_block<false, false>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
+  demod_block<false, false, kMfma>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
+}
+__global__ __launch_bounds__(kT) void k_demod_batch(DemodBatchArgs a) { demod_batch<false>(a); }
+__global__ __launch_bounds__(kT) void k_demod_batch_mfma(DemodBatchArgs a) { demod_batch<true>(a); }
+
+// The demodulator's forward M-point transform alone (FP32 or split-BF16
+// MFMA), one workgroup per transform: in[b M .. + M) -> out[b M .. + M).
+template <bool kMfma>
+__device__ __forceinline__ void fft_only(const float2* in, float2* out, const DemodArgs& a) {
+  __shared__ Lds s;
+  __shared__ TwLds twl;
+  __shared__ MfmaTw mtw;
+  const int tid = threadIdx.x;
+  {
+    const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
+    float2* dst = reinterpret_cast<float2*>(&twl);
+    for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
+    if (tid < (int)(sizeof(MfmaTw) / 8))
+      reinterpret_cast<float2*>(&mtw)[tid] = bload2(make_rsrc(a.mfma_tw, sizeof(MfmaTw)), (uint32_t)tid * 8u, 0u);
+  }
+  __syncthreads();
+  const TwSrcLds twf{&twl};
+  const rsrc_t ri = make_rsrc(in + (int64_t)blockIdx.x * kM, kM * 8u);
+  const rsrc_t ro = make_rsrc(out + (int64_t)blockIdx.x * kM, kM * 8u);
+  float2 v[kP];
+#pragma unroll
+  for (int m = 0; m < kP; ++m) v[m] = bload2(ri, tid * 8u, m * kStrideB);
+  if constexpr (kMfma)
+    WgFft16kMfma::run(v, s.re, s.im, twf, a.mfma_g, &mtw, tid);
+  else
+    FwdM::run(v, s.re, s.im, twf, tid);
+#pragma unroll
+  for (int m = 0; m < kP; ++m) bstore2(ro, v[m], tid * 8u, m * kStrideB);
+}
+__global__ __launch_bounds__(kT) void k_fft_fp32(const float2* in, float2* out, DemodArgs a) {
+  fft_only<false>(in, out, a);
+}
+__global__ __launch_bounds__(kT) void k_fft_mfma(const float2* in, float2* out, DemodArgs a) {
+  fft_only<true>(in, out, a);
 }
 
 // ------------------------------------------------------- audio stage 2
@@ -969,6 +1024,48 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, blocklen);
   hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
+  // split-BF16 MFMA variant: the DFT-64 matrix [[F_re, -F_im]; [F_im, F_re]]
+  // (F = exp(-2 pi i r s / 64)) as hi / lo bf16 fragments in the lane layout
+  // of v_mfma_f32_32x32x16_bf16, and the NS = 64 twiddle factor tables
+  std::vector<uint32_t> gt((size_t)kMfmaTabEntries * 4);
+  auto bf16 = [](float x) -> uint32_t {
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u += 0x7fffu + ((u >> 16) & 1u);
+    return u >> 16;
+  };
+  auto bf2f = [](uint32_t b) -> double {
+    const uint32_t u = b << 16;
+    float f;
+    memcpy(&f, &u, 4);
+    return f;
+  };
+  for (int rt = 0; rt < 4; ++rt)
+    for (int ks = 0; ks < 8; ++ks)
+      for (int hl = 0; hl < 2; ++hl)
+        for (int l = 0; l < 64; ++l) {
+          uint32_t half[8];
+          for (int el = 0; el < 8; ++el) {
+            const int row = 32 * rt + (l & 31), col = 16 * ks + 8 * (l >> 5) + el;
+            const int sidx = row & 63, r = col & 63;
+            const double ang = 2.0 * M_PI * (double)((r * sidx) & 63) / 64.0;
+            const double c = cos(ang), sn = sin(ang);
+            const double g = row < 64 ? (col < 64 ? c : sn) : (col < 64 ? -sn : c);
+            const uint32_t hi = bf16((float)g);
+            half[el] = hl == 0 ? hi : bf16((float)(g - bf2f(hi)));
+          }
+          uint32_t* dst = gt.data() + ((size_t)((rt * 8 + ks) * 2 + hl) * 64 + l) * 4;
+          for (int w = 0; w < 4; ++w) dst[w] = half[2 * w] | (half[2 * w + 1] << 16);
+        }
+  MfmaTw mt;
+  for (int q = 0; q < 64; ++q) {
+    mt.coarse[q] = cexp(q, 64);
+    mt.fine[q] = cexp(q, 4096);
+  }
+  if (e == hipSuccess) e = hipMalloc(&h->mfma_g, gt.size() * 4);
+  if (e == hipSuccess) e = hipMemcpy(h->mfma_g, gt.data(), gt.size() * 4, hipMemcpyHostToDevice);
+  if (e == hipSuccess) e = hipMalloc(&h->mfma_tw, sizeof(MfmaTw));
+  if (e == hipSuccess) e = hipMemcpy(h->mfma_tw, &mt, sizeof(MfmaTw), hipMemcpyHostToDevice);
   if (e != hipSuccess) {
     int rc = hip_fail(e, "ld_create twiddle upload");
     ld_destroy(h);
@@ -983,6 +1080,8 @@ void ld_destroy(ld_handle* h) {
   free_retired(h);
   hipSetDevice(h->device);
   hipFree(h->twiddle);
+  hipFree(h->mfma_g);
+  hipFree(h->mfma_tw);
   hipFree(h->sinc_lut);
   for (auto& f : h->filt) hipFree(f);
   for (auto& f : h->audio_filt) hipFree(f);
@@ -1245,6 +1344,8 @@ static int build_demod_args(ld_handle* h, const void* input, int dtype, int64_t 
   a.tw_rows = h->twiddle;
   a.tw_lds = h->twiddle + TwLayout::total;
   a.tw_split = h->twiddle + TwLayout::total + kTwLdsCount;
+  a.mfma_g = (const u32x4_t*)h->mfma_g;
+  a.mfma_tw = (const MfmaTw*)h->mfma_tw;
   a.h_rfv = h->filt[LD_FILT_RFVIDEO];
   a.hpf.h = h->filt[LD_FILT_RFHPF];
   a.hpf.out = out->rfhpf;
@@ -1349,10 +1450,17 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.step = step;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
-  if (dtype == LD_IN_PACKED10)
+  if (h->fft_variant == LD_FFT_MFMA_BF16X3) {
+    if (dtype == LD_IN_PACKED10 || a.efm) {
+      set_error("ld_demod_blocks: the split-BF16 MFMA variant takes int16 / float32 input without EFM output");
+      return LD_EINVAL;
+    }
+    hipLaunchKernelGGL(k_demod_block_mfma, dim3(nblocks), dim3(kT), 0, st, a);
+  } else if (dtype == LD_IN_PACKED10) {
     hipLaunchKernelGGL(a.efm ? k_demod_block_packed_efm : k_demod_block_packed, dim3(nblocks), dim3(kT), 0, st, a);
-  else
+  } else {
     hipLaunchKernelGGL(a.efm ? k_demod_block_efm : k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
+  }
   LD_HIP(hipGetLastError());
   return LD_OK;
 }
@@ -1404,7 +1512,34 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t 
   b.input_len = input_len;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h0->device));
-  hipLaunchKernelGGL(k_demod_batch, dim3(nblocks), dim3(kT), 0, st, b);
+  hipLaunchKernelGGL(h0->fft_variant == LD_FFT_MFMA_BF16X3 ? k_demod_batch_mfma : k_demod_batch, dim3(nblocks),
+                     dim3(kT), 0, st, b);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+int ld_set_fft_variant(ld_handle* h, int variant) {
+  if (!h || (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3)) {
+    set_error("ld_set_fft_variant: bad handle or variant %d", variant);
+    return LD_EINVAL;
+  }
+  h->fft_variant = variant;
+  return LD_OK;
+}
+
+int ld_fft_forward(ld_handle* h, const void* in, void* out, int ntransforms, int variant, void* stream) {
+  if (!h || !in || !out || ntransforms <= 0 || (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3)) {
+    set_error("ld_fft_forward: bad arguments");
+    return LD_EINVAL;
+  }
+  DemodArgs a = {};
+  a.tw_lds = h->twiddle + TwLayout::total;
+  a.mfma_g = (const u32x4_t*)h->mfma_g;
+  a.mfma_tw = (const MfmaTw*)h->mfma_tw;
+  hipStream_t st = (hipStream_t)stream;
+  LD_HIP(hipSetDevice(h->device));
+  hipLaunchKernelGGL(variant == LD_FFT_MFMA_BF16X3 ? k_fft_mfma : k_fft_fp32, dim3(ntransforms), dim3(kT), 0, st,
+                     (const float2*)in, (float2*)out, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

@@ -368,4 +368,144 @@ struct WgFft<1024, DIR, SYNC> {
   }
 };
 
+// ------------------------------------------------ split-BF16 MFMA radix 64
+// A radix-64 Stockham pass as a real matrix product on the matrix cores:
+// for the C = M/64 columns j of a pass (inputs a[j + r C], r < 64, twiddled
+// by W_{64 NS}^{r (j mod NS)} in FP32), the DFT-64 is
+//   [Y_re; Y_im] (128 x C) = [[F_re, -F_im]; [F_im, F_re]] (128 x 128) [U_re; U_im]
+// with F[s][r] = exp(-2 pi i r s / 64).  Both operands are split into
+// bf16 hi + lo parts (x = hi + lo to about 2^-17 relative) and the product
+// is G_hi U_hi + G_hi U_lo + G_lo U_hi, accumulated in FP32 by
+// v_mfma_f32_32x32x16_bf16 (the G_lo U_lo term, about 2^-18 relative, is
+// dropped).  The G fragments come precomputed in the instruction's lane
+// layout from a global table (L1/L2 resident, 64 KiB).
+//
+// Lane layout of v_mfma_f32_32x32x16_bf16 (checked on gfx950): lane l holds
+// A[l % 32][8 (l / 32) + e] and B[8 (l / 32) + e][l % 32] for e < 8, and the
+// result D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32] in accumulator q < 16.
+typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(16))) float f32x16_t;
+typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
+
+// float -> bf16 bits, round to nearest even (finite values)
+__device__ __forceinline__ uint32_t bf16_rne(float x) {
+  uint32_t u = __float_as_uint(x);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return u >> 16;
+}
+// x = hi + lo, both bf16 (bits in the low halves)
+__device__ __forceinline__ void bf16_split(float x, uint32_t& hi, uint32_t& lo) {
+  hi = bf16_rne(x);
+  lo = bf16_rne(x - __uint_as_float(hi << 16));
+}
+
+// G fragment table layout: [row tile 0..3][k step 0..7][hi, lo][lane 0..63]
+// of 8 bf16 (one u32x4 per lane).
+constexpr int kMfmaTabEntries = 4 * 8 * 2 * 64;
+
+// W_{64 NS}^q for q < 64 NS as the product of a coarse and a fine table:
+// q = 64 qh + ql, W^q = W_{NS}^{qh} W_{64 NS}^{ql} (NS = 64 here: both 64).
+struct MfmaTw {
+  float2 coarse[64];  // exp(-2 pi i qh / NS)
+  float2 fine[64];    // exp(-2 pi i ql / (64 NS))
+};
+
+template <int M, int T, int NS>
+__device__ __forceinline__ void mfma64_pass(float* __restrict__ lre, float* __restrict__ lim,
+                                            const u32x4_t* __restrict__ gtab, const MfmaTw* tw, int tid) {
+  constexpr int C = M / 64;           // columns
+  constexpr int NW = T / 64;          // wavefronts
+  constexpr int NCT = C / 32;         // column tiles of 32
+  static_assert(NW % 4 == 0 && NCT % (NW / 4) == 0, "tiles must divide over the wavefronts");
+  constexpr int CTW = NCT / (NW / 4); // column tiles per wavefront
+  static_assert(NS == 1 || NS == 64, "twiddle tables cover NS = 1 and 64");
+  const int w = tid >> 6, l = tid & 63;
+  const int rt = w & 3;               // row tile: output rows 32 rt .. 32 rt + 31
+  const int cg = w >> 2;              // column group
+  const int half = l >> 5, col = l & 31;
+  f32x16_t acc[CTW];
+#pragma unroll
+  for (int c = 0; c < CTW; ++c)
+#pragma unroll
+    for (int q = 0; q < 16; ++q) acc[c][q] = 0.0f;
+  const u32x4_t* ga = gtab + (size_t)rt * (8 * 2 * 64) + l;
+#pragma unroll 1
+  for (int ks = 0; ks < 4; ++ks) {
+    // G fragments of k steps ks (U_re rows) and ks + 4 (U_im rows)
+    const bf16x8_t are_h = __builtin_bit_cast(bf16x8_t, ga[(ks * 2 + 0) * 64]);
+    const bf16x8_t are_l = __builtin_bit_cast(bf16x8_t, ga[(ks * 2 + 1) * 64]);
+    const bf16x8_t aim_h = __builtin_bit_cast(bf16x8_t, ga[((ks + 4) * 2 + 0) * 64]);
+    const bf16x8_t aim_l = __builtin_bit_cast(bf16x8_t, ga[((ks + 4) * 2 + 1) * 64]);
+    const int r0 = 16 * ks + 8 * half;
+#pragma unroll
+    for (int c = 0; c < CTW; ++c) {
+      const int j = 32 * (cg * CTW + c) + col;
+      u32x4_t bre_h, bre_l, bim_h, bim_l;
+#pragma unroll
+      for (int e2 = 0; e2 < 4; ++e2) {
+        uint32_t rh[2], rl[2], ih[2], il[2];
+#pragma unroll
+        for (int t = 0; t < 2; ++t) {
+          const int r = r0 + 2 * e2 + t;
+          const int idx = pidx(j + r * C);
+          float2 u = make_float2(lre[idx], lim[idx]);
+          if constexpr (NS > 1) {
+            const int q = (r * (j & (NS - 1))) & (64 * NS - 1);
+            u = cmul(u, cmul(tw->coarse[q >> 6], tw->fine[q & 63]));
+          }
+          bf16_split(u.x, rh[t], rl[t]);
+          bf16_split(u.y, ih[t], il[t]);
+        }
+        bre_h[e2] = rh[0] | (rh[1] << 16);
+        bre_l[e2] = rl[0] | (rl[1] << 16);
+        bim_h[e2] = ih[0] | (ih[1] << 16);
+        bim_l[e2] = il[0] | (il[1] << 16);
+      }
+      const bf16x8_t brh = __builtin_bit_cast(bf16x8_t, bre_h), brl = __builtin_bit_cast(bf16x8_t, bre_l);
+      const bf16x8_t bih = __builtin_bit_cast(bf16x8_t, bim_h), bil = __builtin_bit_cast(bf16x8_t, bim_l);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(are_h, brh, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(are_h, brl, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(are_l, brh, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aim_h, bih, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aim_h, bil, acc[c], 0, 0, 0);
+      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aim_l, bih, acc[c], 0, 0, 0);
+    }
+  }
+  __syncthreads();  // every input read before any output lands on it
+#pragma unroll
+  for (int c = 0; c < CTW; ++c) {
+    const int j = 32 * (cg * CTW + c) + col;
+    const int base = (j / NS) * NS * 64 + (j & (NS - 1));
+#pragma unroll
+    for (int q = 0; q < 16; ++q) {
+      const int row = 32 * rt + 8 * (q >> 2) + 4 * half + (q & 3);  // 0..127
+      const int sidx = row & 63;
+      float* dst = row < 64 ? lre : lim;
+      dst[pidx(base + sidx * NS)] = acc[c][q];
+    }
+  }
+  __syncthreads();
+}
+
+// 16384 points, 1024 threads, forward: radix 64 x 64 on the matrix cores
+// (split BF16), then radix 4 in FP32.  On entry v[m] = x[tid + T m]; on exit
+// v[m] = X[tid + T m].  TWF serves the last pass (NS = 4096).
+struct WgFft16kMfma {
+  static constexpr int T = 1024;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
+                                             const u32x4_t* gtab, const MfmaTw* tw, int tid = threadIdx.x) {
+    __syncthreads();  // LDS may still be read by a previous stage
+    const LdsPair own(lre, lim, pidx(tid));
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      own.template st<POff<1024 * m>::value>(v[m]);
+    });
+    __syncthreads();
+    mfma64_pass<16384, 1024, 1>(lre, lim, gtab, tw, tid);
+    mfma64_pass<16384, 1024, 64>(lre, lim, gtab, tw, tid);
+    stockham_pass<16384, 1024, 4, 4096, -1, true, false, kSyncBlock>(v, lre, lim, twf, tid);
+  }
+};
+
 }  // namespace ldgpu

@@ -36,6 +36,10 @@ struct ld_handle {
   std::vector<void*> retired;
   // this device's address of the no-allocation entry point's scratch array
   void* scratch = nullptr;
+  // split-BF16 MFMA transform variant: DFT-64 fragment table and twiddles
+  void* mfma_g = nullptr;
+  void* mfma_tw = nullptr;
+  int fft_variant = 0;       // LD_FFT_FP32 or LD_FFT_MFMA_BF16X3
 };
 
 namespace ldgpu {

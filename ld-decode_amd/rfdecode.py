@@ -122,6 +122,13 @@ class RFDecode:
         self.computefilters()
         self.blockcut_end = 32  # F05_offset (core.py:402, :659)
 
+    def set_fft_variant(self, variant):
+        """The transform variant of later demodulations: _lib.LD_FFT_FP32
+        (default) or _lib.LD_FFT_MFMA_BF16X3 (radix-64 stages as split-BF16
+        matrix products on the matrix cores)."""
+        _lib.check(self.lib.ld_set_fft_variant(self.handle, int(variant)))
+        self.fft_variant = int(variant)
+
     def __del__(self):
         h = getattr(self, "handle", None)
         if h is not None and h.value:
