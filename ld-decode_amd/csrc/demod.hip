@@ -810,6 +810,467 @@ __global__ __launch_bounds__(kT) void k_fft_mfma(const float2* in, float2* out, 
   fft_only<true>(in, out, a);
 }
 
+// =================================================== 65536-sample blocks
+// The demodulation of 65536-sample blocks (RFDecode(blocklen=65536)): the
+// 32768-point complex transforms are one radix-2 step in registers around
+// two 16384-point workgroup transforms (the 32768-sample kernel's FP32 or
+// split-BF16 MFMA transform, LDS-resident), so that LDS holds 16384 complex
+// points at a time.  A forward transform of data in natural order (thread
+// t holds element t + T m, m < 32) decimates in frequency and ends in pair
+// order (thread t holds bins 2j, 2j + 1 in v[m], v[m + 16], j = t + T m,
+// m < 16); a forward transform of data in pair order decimates in time and
+// ends in natural order.  Spectra go from pair order through filters back to
+// natural order, so no transform needs a transpose.  The spectra (X, then D)
+// and the even analytic samples wait in a per-block workspace: registers and
+// LDS of one CU cannot hold a 32768-point spectrum next to the working data.
+constexpr int kN64 = 65536;
+constexpr int kM64 = kN64 / 2;
+constexpr int kH64 = kM64 / 2;           // points per half transform (16384)
+constexpr int kNA64 = kN64 / 32;         // audio slice bins (2048)
+// workspace per block (float2 units): X / D spectrum (bins 0..M-1), even
+// analytic samples, the two audio slices
+constexpr int kWs64Spec = 0;
+constexpr int kWs64Even = kM64 + 64;
+constexpr int kWs64Slice = kWs64Even + kM64;
+constexpr int kWs64Floats2 = kWs64Slice + 2 * (kNA64 / 2 + 1) + 62;   // multiple of 64 float2 (512 B)
+static_assert(kWs64Floats2 % 64 == 0, "workspace blocks stay 512-byte aligned");
+
+using InvA64 = WgFft<kNA64, +1, kSyncWave>;
+
+// W_M^(tid + T m) from wt = W_M^tid and a compile-time constant
+template <int M_>
+__device__ __forceinline__ float2 wm64(float2 wt) {
+  return cmul(wt, make_float2(Tw<kM64, kT * M_, -1>::re, Tw<kM64, kT * M_, -1>::im));
+}
+
+template <bool kMfma>
+__device__ __forceinline__ void half_fft(float2* hv, Lds& s, const TwSrcLds& twf, const DemodArgs& a,
+                                         const MfmaTw* mtw, int tid) {
+  float2 (&h)[16] = *reinterpret_cast<float2(*)[16]>(hv);
+  if constexpr (kMfma)
+    WgFft16kMfma::run(h, s.re, s.im, twf, a.mfma_g, mtw, tid);
+  else
+    FwdM::run(h, s.re, s.im, twf, tid);
+}
+
+// natural order in, pair order out (radix-2 decimation in frequency)
+template <bool kMfma>
+__device__ __forceinline__ void fft_dif64(float2 (&v)[32], Lds& s, const TwSrcLds& twf, const DemodArgs& a,
+                                          const MfmaTw* mtw, float2 wt, int tid) {
+  static_for<0, 16>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const float2 x0 = v[m], x1 = v[m + 16];
+    v[m] = cadd(x0, x1);
+    v[m + 16] = cmul(csub(x0, x1), wm64<m>(wt));
+  });
+  half_fft<kMfma>(v, s, twf, a, mtw, tid);
+  half_fft<kMfma>(v + 16, s, twf, a, mtw, tid);
+}
+
+// pair order in, natural order out (radix-2 decimation in time)
+template <bool kMfma>
+__device__ __forceinline__ void fft_dit64(float2 (&v)[32], Lds& s, const TwSrcLds& twf, const DemodArgs& a,
+                                          const MfmaTw* mtw, float2 wt, int tid) {
+  half_fft<kMfma>(v, s, twf, a, mtw, tid);
+  half_fft<kMfma>(v + 16, s, twf, a, mtw, tid);
+  static_for<0, 16>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const float2 e = v[m], t = cmul(v[m + 16], wm64<m>(wt));
+    v[m] = cadd(e, t);
+    v[m + 16] = csub(e, t);
+  });
+}
+
+// W_N^k for the thread's bins k = 2j (odd = 0) or 2j + 1 (odd = 1), j = tid + T m
+template <int M_, int ODD>
+__device__ __forceinline__ float2 wbin64(float2 wt) {
+  const float2 w = wm64<M_>(wt);
+  if constexpr (ODD) return cmul(w, make_float2(Tw<kN64, 1, -1>::re, Tw<kN64, 1, -1>::im));
+  return w;
+}
+
+// After a forward transform of y[n] = x[2n] + i x[2n+1] (pair order in v),
+// the half spectrum X[k] of the real x, in pair order in v, and X[M] in xM.
+// The mirror bins M - k come through LDS, one half at a time.
+__device__ __forceinline__ void r2c_split64(float2 (&v)[32], float& xM, Lds& s, float2 wt, int tid) {
+  const LdsPair own(s.re, s.im, pidx(tid));
+#pragma unroll
+  for (int hf = 0; hf < 2; ++hf) {
+    __syncthreads();
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      own.template st<POff<kT * m>::value>(v[m + 16 * hf]);
+    });
+    if (hf == 0 && tid == 0) {  // Y[M] = Y[0] (periodic), the mirror of bin 0
+      s.re[pidx(kH64)] = v[0].x;
+      s.im[pidx(kH64)] = v[0].y;
+    }
+    __syncthreads();
+    // mirror of bin 2j is A[H - j]; of bin 2j + 1, B[H - 1 - j]
+    const LdsPair mir(s.re, s.im, pidx(kH64 - hf - tid));
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const float2 p = v[m + 16 * hf];
+      const float2 q = mir.template ld<-POff<kT * m>::value>();
+      const float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+      const float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+      if (hf == 0 && m == 0 && tid == 0) xM = p.x - p.y;
+      v[m + 16 * hf] = cadd(E, cmul(hf ? wbin64<m, 1>(wt) : wbin64<m, 0>(wt), O));
+      if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    });
+  }
+  __syncthreads();
+}
+
+// The input of a real inverse transform of Z = X H (half spectrum in pair
+// order in v, Z[M] in zM), conjugated for the forward transform: conj(E + iO).
+__device__ __forceinline__ void c2r_pre64(float2 (&v)[32], float2 zM, Lds& s, float2 wt, int tid) {
+  const LdsPair own(s.re, s.im, pidx(tid));
+#pragma unroll
+  for (int hf = 0; hf < 2; ++hf) {
+    __syncthreads();
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      own.template st<POff<kT * m>::value>(v[m + 16 * hf]);
+    });
+    if (hf == 0 && tid == 0) {  // Z[M], the mirror of bin 0
+      s.re[pidx(kH64)] = zM.x;
+      s.im[pidx(kH64)] = zM.y;
+    }
+    __syncthreads();
+    const LdsPair mir(s.re, s.im, pidx(kH64 - hf - tid));
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const float2 A = v[m + 16 * hf];
+      const float2 B = cconj(mir.template ld<-POff<kT * m>::value>());
+      const float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
+      const float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)),
+                             hf ? wbin64<m, 1>(wt) : wbin64<m, 0>(wt));
+      v[m + 16 * hf] = make_float2(E.x - O.y, -(E.y + O.x));
+      if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    });
+  }
+  // the transform's first step works in registers; its half transforms
+  // synchronise before they write LDS
+}
+
+// Stores of a transform result y (natural order in v; y[2n] = v.x, y[2n+1]
+// = -v.y) as out[i] = y[(i + cut + shift) mod N] + dc, i < len, with the
+// in-pulse words of the output when asked (as c2r_store).
+__device__ __forceinline__ void c2r_store64(const C2rOut& o, int64_t base, const float2 (&v)[32], int tid,
+                                            uint32_t* words = nullptr, float highf = 0.0f) {
+  const rsrc_t r = make_rsrc(o.out + base, (uint32_t)o.len * 4u);
+  const int off = o.cut + o.shift;
+  const float dc = o.dc;
+  const bool pairs = ((off | o.len) & 1) == 0;
+  const int q = tid & 3;
+  static_for<0, 32>([&](auto mc) {
+    constexpr int m = decltype(mc)::value;
+    const int n = 2 * (tid + kT * m);
+    const float y0 = v[m].x + dc, y1 = dc - v[m].y;
+    const uint32_t i0 = (uint32_t)((n - off) & (kN64 - 1));
+    if (pairs) {
+      bstore2(r, make_float2(y0, y1), i0 * 4u, 0u);
+      if (words != nullptr) {
+        const unsigned long long e0 = __ballot(y0 <= highf), e1 = __ballot(y1 <= highf);
+        const uint32_t w = ((uint32_t)(e0 >> (16 * q)) & 0xffffu) | ((uint32_t)(e1 >> (16 * q)) << 16);
+        const uint32_t j0 = (uint32_t)((2 * (kT * m + (tid & ~63)) + 32 * q - off) & (kN64 - 1));
+        if ((tid & 63) < 4 && j0 < (uint32_t)o.len) {
+          const uint32_t left = (uint32_t)o.len - j0;
+          words[j0 >> 5] = left >= 32 ? w : (w & below_mask_of(left));
+        }
+      }
+    } else {
+      const uint32_t i1 = (uint32_t)((n + 1 - off) & (kN64 - 1));
+      bstore1(r, y0, i0 * 4u, 0u);
+      bstore1(r, y1, i1 * 4u, 0u);
+    }
+  });
+}
+
+// v = the workspace spectrum times a half spectrum table, bin by bin, in
+// pair order (bins 2j, 2j + 1 in v[m], v[m + 16], j = tid + T m)
+__device__ __forceinline__ void load_prod64(rsrc_t spec, const float2* table, float2 (&v)[32], int tid) {
+  const rsrc_t H = make_rsrc(table, (kM64 + 1) * 8u);
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const uint32_t o = (uint32_t)(tid + kT * m) * 16u;
+    v[m] = cmul(bload2(spec, o, 0u), bload2(H, o, 0u));
+    v[m + 16] = cmul(bload2(spec, o + 8u, 0u), bload2(H, o + 8u, 0u));
+  }
+}
+
+// Analogue audio stage 1 of a 65536-sample block, one wavefront per
+// channel: the slice (in the workspace) times filt1, 2048-point inverse,
+// instantaneous frequency (as audio_wave).
+__device__ void audio_wave64(const DemodArgs& a, int c, const float2* slice, float* lre, float* lim, int b,
+                             const TwGlobal& twr) {
+  const int lane = threadIdx.x & 63;
+  const rsrc_t F = make_rsrc(a.h_audio[c], kNA64 * 8u);
+  const rsrc_t S = make_rsrc(slice, (kNA64 / 2 + 1) * 8u);
+  float2 v[kNA64 / 64];
+#pragma unroll
+  for (int m = 0; m < kNA64 / 64; ++m) {
+    const int k = lane + 64 * m;
+    const float2 f = bload2(F, lane * 8u, m * 512u);
+    float2 z = make_float2(0.0f, 0.0f);
+    if (k < kNA64 / 2)
+      z = cmul(bload2(S, (uint32_t)k * 8u, 0u), f);
+    else if (k == kNA64 / 2)
+      z = cmul(cconj(bload2(S, (uint32_t)k * 8u, 0u)), f);
+    v[m] = z;
+  }
+  InvA64::run(v, lre, lim, twr);
+#pragma unroll
+  for (int m = 0; m < kNA64 / 64; ++m) {
+    const int n = lane + 64 * m;
+    lre[pidx(n)] = v[m].x;
+    lim[pidx(n)] = v[m].y;
+  }
+  lds_barrier<kSyncWave>();
+  const float rate = a.rate[c], lf = a.low_freq[c];
+  const float hz_per_rad = rate / kTwoPi;
+  const rsrc_t out = make_rsrc(a.out_audio[c] + (int64_t)b * a.astride, (uint32_t)a.alen * 4u);
+#pragma unroll
+  for (int m = 0; m < kNA64 / 64; ++m) {
+    const int n = lane + 64 * m;
+    float hz = 0.0f;
+    if (n > 0) {
+      const float2 p = cmulc(v[m], make_float2(lre[pidx(n - 1)], lim[pidx(n - 1)]));
+      hz = inc_to_hz(atan2f(p.y, p.x), hz_per_rad, rate);
+    }
+    bstore1(out, hz + lf, (uint32_t)(n - a.acut) * 4u, 0u);
+  }
+}
+
+template <bool kMfma>
+__device__ __forceinline__ void demod_block64(const DemodArgs& a, int64_t s0, int bo, int bw) {
+  __shared__ Lds s;
+  __shared__ TwLds twl;
+  __shared__ MfmaTw mtw;
+  __shared__ float notch_mag[kNotchMax];
+  __shared__ int notch_flag[kNotchMax + 4];
+  __shared__ double notch_red[128];
+  __shared__ float2 bin_m[2 + 4];  // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters
+  __shared__ float spec_m;         // X[M], then D[M]
+  const int tid = threadIdx.x;
+  float2* ws = a.ws_even + (int64_t)bw * kWs64Floats2;
+  const rsrc_t wspec = make_rsrc(ws + kWs64Spec, kM64 * 8u);
+  const rsrc_t weven = make_rsrc(ws + kWs64Even, kM64 * 8u);
+  float2* slices = ws + kWs64Slice;
+  {
+    const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
+    float2* dst = reinterpret_cast<float2*>(&twl);
+    for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
+    if (tid < 2 + a.npost) {
+      const float2* t = tid == 0 ? a.hpf.h : (tid == 1 ? a.h_rfv : a.post[tid - 2].h);
+      bin_m[tid] = t ? bload2(make_rsrc(t, (kM64 + 1) * 8u), 0u, kM64 * 8u) : make_float2(0.0f, 0.0f);
+    }
+    if constexpr (kMfma) {
+      const rsrc_t mt = make_rsrc(a.mfma_tw, sizeof(MfmaTw));
+      if (tid >= 64 && tid < 64 + (int)(sizeof(MfmaTw) / 8))
+        reinterpret_cast<float2*>(&mtw)[tid - 64] = bload2(mt, (uint32_t)(tid - 64) * 8u, 0u);
+    }
+  }
+  __syncthreads();
+  const TwSrcLds twf{&twl};
+  const float2 wt = bload2(make_rsrc(a.tw_split, kT * 8u), tid * 8u, 0u);   // W_M^tid
+  const bool has_hpf = a.hpf.out != nullptr;
+  const bool notch = a.notch_hi > a.notch_lo;
+  float2 v[32];
+  float xM = 0.0f;
+  uint32_t zmask = 0u;   // bit m: v[m]'s bin zeroed by the spur notch
+
+  // ------------------------------------------------ X = rFFT(block)
+  {
+    const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(kN64 * a.in_bytes));
+    if (a.in_bytes == 4) {
+#pragma unroll
+      for (int m = 0; m < 32; ++m) v[m] = bload2(in, tid * 8u, m * (kT * 8u));
+    } else if ((s0 & 1) == 0) {
+#pragma unroll
+      for (int m = 0; m < 32; ++m) {
+        const uint32_t w = __builtin_bit_cast(uint32_t, bload1(in, tid * 4u, m * (kT * 4u)));
+        v[m] = make_float2((float)(int16_t)(w & 0xffffu), (float)(int16_t)(w >> 16));
+      }
+    } else {
+#pragma unroll
+      for (int m = 0; m < 32; ++m) {
+        const uint32_t o = tid * 4u + m * (kT * 4u);
+        const int16_t lo = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o, 0, 0);
+        const int16_t hi = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o + 2, 0, 0);
+        v[m] = make_float2((float)lo, (float)hi);
+      }
+    }
+  }
+  fft_dif64<kMfma>(v, s, twf, a, &mtw, wt, tid);
+  r2c_split64(v, xM, s, wt, tid);
+  if (notch) {  // PAL spur notch decision (core.py:774-798)
+    static_for<0, 32>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int k = 2 * (tid + kT * (m & 15)) + (m >> 4);
+      if (k >= a.notch_lo && k < a.notch_hi) notch_mag[k - a.notch_lo] = sqrtf(v[m].x * v[m].x + v[m].y * v[m].y);
+    });
+    __syncthreads();
+    if (tid < 64) notch_decide(notch_mag, notch_flag, notch_red, a.notch_lo, a.notch_hi);
+    __syncthreads();
+    static_for<0, 32>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      if (notch_zero(notch_flag, a.notch_lo, a.notch_hi, 2 * (tid + kT * (m & 15)) + (m >> 4))) zmask |= 1u << m;
+    });
+  }
+  // the spectrum to the workspace (bins 2j, 2j + 1 side by side), X[M], the
+  // audio slices (notched bins zeroed)
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const uint32_t o = (uint32_t)(tid + kT * m) * 16u;
+    bstore2(wspec, v[m], o, 0u);
+    bstore2(wspec, v[m + 16], o + 8u, 0u);
+  }
+  if (tid == 0) spec_m = xM;
+#pragma unroll
+  for (int c = 0; c < 2; ++c) {
+    if (a.out_audio[c] == nullptr) continue;
+    const rsrc_t sl = make_rsrc(slices + c * (kNA64 / 2 + 1), (kNA64 / 2 + 1) * 8u);
+    static_for<0, 32>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      const int i = 2 * (tid + kT * (m & 15)) + (m >> 4) - a.lowbin[c];
+      if (i >= 0 && i <= kNA64 / 2) bstore2(sl, (zmask & (1u << m)) ? make_float2(0.0f, 0.0f) : v[m], (uint32_t)i * 8u, 0u);
+    });
+  }
+  __syncthreads();
+  xM = spec_m;
+
+  // ------------------------------------------------ rfhpf (core.py:769-772)
+  if (has_hpf) {
+    load_prod64(wspec, a.hpf.h, v, tid);
+    c2r_pre64(v, cscale(bin_m[0], xM), s, wt, tid);
+    fft_dit64<kMfma>(v, s, twf, a, &mtw, wt, tid);
+    c2r_store64(a.hpf, (int64_t)bo * a.vstride, v, tid);
+  }
+
+  // ------------------------------------------------ analytic signal (core.py:800-805)
+  const float2 hM = cscale(bin_m[1], xM);
+#pragma unroll
+  for (int odd = 0; odd < 2; ++odd) {
+    load_prod64(wspec, a.h_rfv, v, tid);
+    static_for<0, 32>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      float2 z = v[m];
+      if (zmask & (1u << m)) z = make_float2(0.0f, 0.0f);
+      if (m == 0 && tid == 0) z = odd ? csub(z, hM) : cadd(z, hM);   // bin 0: Z[M] folded in
+      else if (odd) z = cmulc(z, wbin64<(m & 15), (m >> 4)>(wt));    // odd samples: Z[k] W_N^-k
+      v[m] = cconj(z);
+    });
+    fft_dit64<kMfma>(v, s, twf, a, &mtw, wt, tid);
+    if (!odd) {
+#pragma unroll
+      for (int m = 0; m < 32; ++m) bstore2(weven, v[m], tid * 8u, m * (kT * 8u));
+    }
+  }
+
+  // ------------------------------------------------ instantaneous frequency, clip (utils.py:898-926, core.py:809)
+  {
+    // z_odd[n - 1] for every n: the odd samples shifted by one through LDS,
+    // half a block at a time (the sample before the second half's first is
+    // carried over by the thread that holds it)
+    const float hz_per_rad = a.fs / kTwoPi;
+    const float2 last0 = v[15];     // z_odd[H - 1] on thread T - 1
+#pragma unroll
+    for (int hf = 0; hf < 2; ++hf) {
+      __syncthreads();
+      const LdsPair sh(s.re, s.im, pidx(tid + 1));
+      static_for<0, 16>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        sh.template st<POff<kT * m>::value>(v[m + 16 * hf]);
+      });
+      if (hf == 1 && tid == kT - 1) {
+        s.re[pidx(0)] = last0.x;
+        s.im[pidx(0)] = last0.y;
+      }
+      __syncthreads();
+      const LdsPair own(s.re, s.im, pidx(tid));
+      static_for<0, 16>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
+        const int j = tid + kT * (m + 16 * hf);
+        const float2 prev = own.template ld<POff<kT * m>::value>();
+        const float2 e = bload2(weven, tid * 8u, (m + 16 * hf) * (kT * 8u)), od = v[m + 16 * hf];
+        const float2 p = cmulc(e, prev);
+        const float h0 = (j > 0) ? inc_to_hz(-atan2f(p.y, p.x), hz_per_rad, a.fs) : 0.0f;
+        const float2 q = cmulc(od, e);
+        const float h1 = inc_to_hz(-atan2f(q.y, q.x), hz_per_rad, a.fs);
+        v[m + 16 * hf] = make_float2(h0, h1);
+      });
+    }
+    if (a.demod_raw != nullptr) {
+      const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)bo * a.vstride, (uint32_t)a.vlen * 4u);
+      const bool pairs = ((a.vcut | a.vlen) & 1) == 0;
+#pragma unroll
+      for (int m = 0; m < 32; ++m) {
+        const uint32_t i0 = (uint32_t)(2 * (tid + kT * m) - a.vcut);
+        if (pairs) {
+          bstore2(raw, v[m], i0 * 4u, 0u);
+        } else {
+          bstore1(raw, v[m].x, i0 * 4u, 0u);
+          bstore1(raw, v[m].y, i0 * 4u + 4u, 0u);
+        }
+      }
+    }
+#pragma unroll
+    for (int m = 0; m < 32; ++m)
+      v[m] = make_float2(fminf(fmaxf(v[m].x, a.clip_lo), a.clip_hi) - a.dc,
+                         fminf(fmaxf(v[m].y, a.clip_lo), a.clip_hi) - a.dc);
+  }
+  fft_dif64<kMfma>(v, s, twf, a, &mtw, wt, tid);
+  r2c_split64(v, xM, s, wt, tid);     // D and D[M]
+#pragma unroll
+  for (int m = 0; m < 16; ++m) {
+    const uint32_t o = (uint32_t)(tid + kT * m) * 16u;
+    bstore2(wspec, v[m], o, 0u);
+    bstore2(wspec, v[m + 16], o + 8u, 0u);
+  }
+  if (tid == 0) spec_m = xM;
+  __syncthreads();
+  xM = spec_m;
+
+  // ------------------------------------------------ video post filters (core.py:811-820)
+  for (int p = 0; p < a.npost; ++p) {
+    const C2rOut& o = a.post[p];
+    load_prod64(wspec, o.h, v, tid);
+    c2r_pre64(v, cscale(bin_m[2 + p], xM), s, wt, tid);
+    fft_dit64<kMfma>(v, s, twf, a, &mtw, wt, tid);
+    c2r_store64(o, (int64_t)bo * a.vstride, v, tid,
+                p == a.below_post ? a.below + (((int64_t)bo * a.vstride) >> 5) : nullptr, a.below_highf);
+  }
+
+  // ------------------------------------------------ audio stage 1 (core.py:854-875)
+  __syncthreads();
+  if (tid < 128) {
+    const int c = tid >> 6;
+    const TwGlobal tg{make_rsrc(a.tw_rows, TwLayout::total * 8u), TwLayout::k2k_p2, TwLayout::k2k_p3, 32, 5, 1};
+    if (a.out_audio[c] != nullptr)
+      audio_wave64(a, c, slices + c * (kNA64 / 2 + 1), s.re + c * LdsSize<kNA64>::floats,
+                   s.im + c * LdsSize<kNA64>::floats, bo, tg);
+  }
+}
+
+__global__ __launch_bounds__(kT) void k_demod_block64(DemodArgs a) {
+  demod_block64<false>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
+}
+__global__ __launch_bounds__(kT) void k_demod_block64_mfma(DemodArgs a) {
+  demod_block64<true>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
+}
+template <bool kMfma>
+__device__ __forceinline__ void demod_batch64(const DemodBatchArgs& a) {
+  const ld_batch_block d = a.blocks[blockIdx.x];
+  if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - kN64 || d.slot < 0 ||
+      d.slot >= a.nslots[d.sys])
+    return;
+  demod_block64<kMfma>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
+}
+__global__ __launch_bounds__(kT) void k_demod_batch64(DemodBatchArgs a) { demod_batch64<false>(a); }
+__global__ __launch_bounds__(kT) void k_demod_batch64_mfma(DemodBatchArgs a) { demod_batch64<true>(a); }
+
 // ------------------------------------------------------- audio stage 2
 // One blocklen segment of a field read's stage-1 audio: y = the filtered
 // segment (index 0 = its first sample); output sample out_base + i takes
@@ -991,8 +1452,8 @@ int ld_create(int device, int blocklen, ld_handle** out) {
     return LD_EINVAL;
   }
   *out = nullptr;
-  if (blocklen != kN) {
-    set_error("ld_create: blocklen %d not supported (only %d)", blocklen, kN);
+  if (blocklen != kN && blocklen != kN64) {
+    set_error("ld_create: blocklen %d not supported (%d or %d)", blocklen, kN, kN64);
     return LD_EINVAL;
   }
   LD_HIP(hipSetDevice(device));
@@ -1020,8 +1481,12 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   rows(TwLayout::k16k_p3, 16, 1024);
   rows(TwLayout::k1k_p2, 16, 16);
   rows(TwLayout::k1k_p3, 4, 256);
+  rows(TwLayout::k2k_p2, 32, 32);
+  rows(TwLayout::k2k_p3, 2, 1024);
   TwLds::fill(tw.data() + TwLayout::total, cexp);
-  for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, blocklen);
+  // split[t] = exp(-2 pi i t / 32768): W_N^t of the 32768-sample kernel's real
+  // split, W_M^t of the 65536-sample kernel's radix-2 step and its split
+  for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, kN);
   hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
   // split-BF16 MFMA variant: the DFT-64 matrix [[F_re, -F_im]; [F_im, F_re]]
@@ -1199,6 +1664,10 @@ int ld_audio2_fields(ld_handle* h, const float* left, const float* right, int64_
     set_error("ld_audio2_fields: bad arguments");
     return LD_EINVAL;
   }
+  if (h->blocklen != kN) {
+    set_error("ld_audio2_fields: audio stage 2 runs on %d-sample handles", kN);
+    return LD_EINVAL;
+  }
   if (!h->audio2_filt[0] || !h->audio2_filt[1]) {
     set_error("ld_audio2_fields: stage-2 filters not set (ld_set_audio2)");
     return LD_ESTATE;
@@ -1247,8 +1716,8 @@ int ld_set_audio(ld_handle* h, int ch, const double* reim, int nbins, int lowbin
     set_error("ld_set_audio: bad handle, table or channel %d", ch);
     return LD_EINVAL;
   }
-  if (nbins != kNA) {
-    set_error("ld_set_audio: nbins %d not supported (only %d at blocklen %d)", nbins, kNA, kN);
+  if (nbins != h->blocklen / 32) {
+    set_error("ld_set_audio: nbins %d not supported (only %d at blocklen %d)", nbins, h->blocklen / 32, h->blocklen);
     return LD_EINVAL;
   }
   if (lowbin < 1 || lowbin + nbins / 2 >= h->blocklen / 2) {
@@ -1301,6 +1770,7 @@ int ld_set_notch(ld_handle* h, int lo, int hi) {
 size_t ld_demod_workspace_bytes(const ld_handle* h, int nblocks) {
   if (!h || nblocks <= 0) return 0;
   // even analytic samples of every block (M float2)
+  if (h->blocklen == kN64) return (size_t)nblocks * (size_t)kWs64Floats2 * sizeof(float2);
   return (size_t)nblocks * (size_t)(h->blocklen / 2) * sizeof(float2);
 }
 
@@ -1330,7 +1800,7 @@ static int build_demod_args(ld_handle* h, const void* input, int dtype, int64_t 
   const int S = N - h->cut_begin - h->cut_end;
   const int vcut = cut ? h->cut_begin : 0;
   const int vlen = cut ? S : N;
-  const int NA = h->audio_nbins ? h->audio_nbins : kNA;
+  const int NA = h->audio_nbins ? h->audio_nbins : h->blocklen / 32;
   const int fdiv = N / NA;
   const int acut = cut ? h->cut_begin / fdiv : 0;
   const int alen = cut ? NA - acut - (h->cut_end + fdiv - 1) / fdiv : NA;
@@ -1450,7 +1920,14 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
   a.step = step;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
-  if (h->fft_variant == LD_FFT_MFMA_BF16X3) {
+  if (h->blocklen == kN64) {
+    if (dtype == LD_IN_PACKED10 || a.efm) {
+      set_error("ld_demod_blocks: 65536-sample blocks take int16 / float32 input without EFM output");
+      return LD_EINVAL;
+    }
+    hipLaunchKernelGGL(h->fft_variant == LD_FFT_MFMA_BF16X3 ? k_demod_block64_mfma : k_demod_block64, dim3(nblocks),
+                       dim3(kT), 0, st, a);
+  } else if (h->fft_variant == LD_FFT_MFMA_BF16X3) {
     if (dtype == LD_IN_PACKED10 || a.efm) {
       set_error("ld_demod_blocks: the split-BF16 MFMA variant takes int16 / float32 input without EFM output");
       return LD_EINVAL;
@@ -1486,7 +1963,7 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t 
     ld_handle* h = sys[k];
     b.nslots[k] = 0;
     if (!h) continue;
-    if (h->blocklen != kN || h->device != h0->device) {
+    if (h->blocklen != h0->blocklen || h->device != h0->device) {
       set_error("ld_demod_batch: system %d handle: blocklen %d or device %d differs", k, h->blocklen, h->device);
       return LD_EINVAL;
     }
@@ -1512,8 +1989,11 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t 
   b.input_len = input_len;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h0->device));
-  hipLaunchKernelGGL(h0->fft_variant == LD_FFT_MFMA_BF16X3 ? k_demod_batch_mfma : k_demod_batch, dim3(nblocks),
-                     dim3(kT), 0, st, b);
+  const bool mf = h0->fft_variant == LD_FFT_MFMA_BF16X3;
+  if (h0->blocklen == kN64)
+    hipLaunchKernelGGL(mf ? k_demod_batch64_mfma : k_demod_batch64, dim3(nblocks), dim3(kT), 0, st, b);
+  else
+    hipLaunchKernelGGL(mf ? k_demod_batch_mfma : k_demod_batch, dim3(nblocks), dim3(kT), 0, st, b);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

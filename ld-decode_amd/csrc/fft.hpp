@@ -305,7 +305,11 @@ struct TwLayout {
   // 1024 points: passes (R=16, NS=16) and (R=4, NS=256)
   static constexpr int k1k_p2 = k16k_p3 + 1024 * 4;
   static constexpr int k1k_p3 = k1k_p2 + 16 * 4;
-  static constexpr int total = k1k_p3 + 256 * 2;
+  // 2048 points (audio slices of 65536-sample blocks): passes (R=32, NS=32)
+  // and (R=2, NS=1024)
+  static constexpr int k2k_p2 = k1k_p3 + 256 * 2;
+  static constexpr int k2k_p3 = k2k_p2 + 32 * 5;
+  static constexpr int total = k2k_p3 + 1024 * 1;
 };
 
 // Twiddle source reading the global rows through a buffer descriptor.
@@ -352,6 +356,19 @@ struct WgFft16kT1024 {
     stockham_pass<16384, 1024, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
     stockham_pass<16384, 1024, 16, 256, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
     stockham_pass<16384, 1024, 4, 4096, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
+  }
+};
+
+// 2048 points, 64 threads (one wavefront): radix 32 x 32 x 2
+template <int DIR, int SYNC>
+struct WgFft<2048, DIR, SYNC> {
+  static constexpr int T = 64;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[32], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<2048, 64, 32, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<2048, 64, 32, 32, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<2048, 64, 2, 1024, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
   }
 };
 

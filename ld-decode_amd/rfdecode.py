@@ -104,6 +104,9 @@ class RFDecode:
         self.samplesperline = self.freq / self.linelen
         self.hsync_tolerance = 0.4
         self.decode_digital_audio = decode_digital_audio
+        if decode_digital_audio and blocklen != 32768:
+            # the EFM equaliser is built for 32768 bins (core.py:465)
+            raise ValueError("EFM output needs blocklen 32768")
         self.decode_analog_audio = decode_analog_audio
         self.delays = None
 
