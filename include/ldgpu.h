@@ -76,7 +76,7 @@ int ld_abi_version(void);
 const char* ld_last_error(void);
 
 /* Create a handle on HIP device `device` for blocks of `blocklen` samples
- * (RFDecode(blocklen=...), core.py:286; 32768 or 65536).  Uploads the FFT
+ * (RFDecode(blocklen=...), core.py:286; 16384, 32768 or 65536).  Uploads the FFT
  * twiddle table.
  * Replaces RFDecode.__init__'s table set-up, core.py:284-403. */
 int ld_create(int device, int blocklen, ld_handle** out);

@@ -62,10 +62,8 @@ constexpr float kTwoPi = 6.283185307179586f;
 using FwdM = std::conditional_t<kT == 512, WgFft<kM, -1>, WgFft16kT1024<-1>>;
 using InvA = WgFft<kNA, +1, kSyncWave>;
 
-struct Lds {
-  float re[LdsSize<kM>::floats];
-  float im[LdsSize<kM>::floats];
-};
+template <int M>
+struct LdsT;
 
 // Pass twiddles of the M-point transform, kept in LDS for the whole kernel
 // (copied once from the global table).  Row k of a pass after NS points holds
@@ -132,19 +130,93 @@ struct TwLdsT<1024> {  // passes (R 16, NS 16), (R 16, NS 256 = 16 h + l), (R 4,
 using TwLds = TwLdsT<kT>;
 constexpr int kTwLdsCount = TwLds::count;
 
-struct TwSrcLds {
-  const TwLds* t;
+template <class TW>
+struct TwSrcLdsT {
+  const TW* t;
   template <int NS>
   __device__ __forceinline__ float2 get(int k, int i) const {
     return t->template get<NS>(k, i);
   }
 };
+using TwSrcLds = TwSrcLdsT<TwLds>;
+
+// The 8192-point, 512-thread transform's pass twiddles: passes (R 16, NS 16),
+// (R 16, NS 256 = 16 h + l), (R 2, NS 4096 = 64 h + l).
+struct TwLds8k {
+  float2 p16[16 * 4];
+  float2 a256[16 * 4];
+  float2 b256[16 * 4];
+  float2 a4k[64];
+  float2 b4k[64];
+  static constexpr int count = 16 * 4 * 3 + 64 * 2;
+  template <int NS>
+  __device__ __forceinline__ float2 get(int k, int i) const {
+    if constexpr (NS == 16) {
+      return p16[k * 4 + i];
+    } else if constexpr (NS == 256) {
+      return cmul(a256[(k >> 4) * 4 + i], b256[(k & 15) * 4 + i]);
+    } else {
+      static_assert(NS == 4096, "M-point transform passes");
+      return cmul(a4k[k >> 6], b4k[k & 63]);
+    }
+  }
+  static void fill(float2* t, float2 (*cexp)(double, double)) {
+    for (int k = 0; k < 16; ++k)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * k, 256);
+    for (int h = 0; h < 16; ++h)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 16 * h, 4096);
+    for (int l = 0; l < 16; ++l)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * l, 4096);
+    for (int h = 0; h < 64; ++h) *t++ = cexp(64.0 * h, 8192);
+    for (int l = 0; l < 64; ++l) *t++ = cexp((double)l, 8192);
+  }
+};
+static_assert(TwLds8k::count <= TwLds::count, "the handle table holds either image");
+
+template <int M>
+struct LdsT {
+  float re[LdsSize<M>::floats];
+  float im[LdsSize<M>::floats];
+};
+
+// Block-length configurations of the single-level demodulator: N real
+// samples per block, one M = N / 2 point complex transform by T threads of P
+// points, audio slices of NA bins.
+template <int N_>
+struct DemodCfg;
+template <>
+struct DemodCfg<32768> {
+  static constexpr int N = 32768, M = 16384, T = 1024, P = 16, NA = 1024;
+  using Fwd = WgFft16kT1024<-1>;
+  using FwdMfma = WgFft16kMfma;
+  using Tw = TwLdsT<1024>;
+  using Inv = WgFft<1024, +1, kSyncWave>;
+  static constexpr uint32_t HalfBytes = (M + 1) * 8, StrideB = T * 8;
+  static __device__ __forceinline__ TwGlobal audio_tw(rsrc_t r) {
+    return TwGlobal{r, TwLayout::k1k_p2, TwLayout::k1k_p3, 16, 4, 2};
+  }
+};
+template <>
+struct DemodCfg<16384> {
+  static constexpr int N = 16384, M = 8192, T = 512, P = 16, NA = 512;
+  using Fwd = WgFft8kT512<-1>;
+  using FwdMfma = WgFft8kMfma;
+  using Tw = TwLds8k;
+  using Inv = WgFft<512, +1, kSyncWave>;
+  static constexpr uint32_t HalfBytes = (M + 1) * 8, StrideB = T * 8;
+  static __device__ __forceinline__ TwGlobal audio_tw(rsrc_t r) {
+    return TwGlobal{r, TwLayout::k512_p2, TwLayout::k512_p3, 8, 3, 3};
+  }
+};
+using Cfg32 = DemodCfg<32768>;
+using Lds = LdsT<kM>;
+using Cfg16 = DemodCfg<16384>;
 
 // W_N^(tid + T m) from the thread's split[tid] = exp(-2 pi i tid / N) and a
 // compile-time constant exp(-2 pi i m T / N)
-template <int M_>
+template <class C, int M_>
 __device__ __forceinline__ float2 split_tw(float2 wt) {
-  return cmul(wt, make_float2(Tw<kN / kT, M_, -1>::re, Tw<kN / kT, M_, -1>::im));
+  return cmul(wt, make_float2(Tw<C::N / C::T, M_, -1>::re, Tw<C::N / C::T, M_, -1>::im));
 }
 
 // Everything one fused launch needs.  Filters are half spectra (bins 0..M)
@@ -199,25 +271,26 @@ constexpr uint32_t kStrideB = kT * 8;           // bytes between a thread's elem
 // After a forward M-point FFT of y[m] = x[2m] + i x[2m+1] (v[m] = Y[k],
 // k = tid+T*m) form the half spectrum X[k] = E + W_N^k O of the real
 // sequence x in x[m], and X[M] (real) in xM on the thread holding k = 0.
-__device__ __forceinline__ void r2c_split(const float2 (&v)[kP], float2 (&x)[kP], float& xM, Lds& s, float2 wt,
+template <class C>
+__device__ __forceinline__ void r2c_split(const float2 (&v)[C::P], float2 (&x)[C::P], float& xM, LdsT<C::M>& s, float2 wt,
                                           int tid) {
   const LdsPair own(s.re, s.im, pidx(tid)), mir(s.re, s.im, pidx(-tid));
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    own.template st<POff<kT * m>::value>(v[m]);
+    own.template st<POff<C::T * m>::value>(v[m]);
   });
   if (tid == 0) {  // Y[M] = Y[0] (periodic), read by k = 0
-    s.re[pidx(kM)] = v[0].x;
-    s.im[pidx(kM)] = v[0].y;
+    s.re[pidx(C::M)] = v[0].x;
+    s.im[pidx(C::M)] = v[0].y;
   }
   __syncthreads();
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     float2 p = v[m];
-    float2 q = mir.template ld<POff<kM - kT * m>::value>();
+    float2 q = mir.template ld<POff<C::M - C::T * m>::value>();
     float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
     float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
-    x[m] = cadd(E, cmul(split_tw<m>(wt), O));
+    x[m] = cadd(E, cmul(split_tw<C, m>(wt), O));
     if (m == 0 && tid == 0) xM = p.x - p.y;
     // bound how far the scheduler hoists the mirror loads (register pressure)
     if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
@@ -229,27 +302,28 @@ __device__ __forceinline__ void r2c_split(const float2 (&v)[kP], float2 (&x)[kP]
 // scaled by 1/M; its bins tid + T m arrive in v, bin M in hM), conjugated
 // so that a forward FFT produces the conjugate of the inverse:
 // v[m] = conj(E + i O).
-__device__ __forceinline__ void c2r_pre(const float2 (&x)[kP], float xM, float2 hM, float2 (&v)[kP], Lds& s,
+template <class C>
+__device__ __forceinline__ void c2r_pre(const float2 (&x)[C::P], float xM, float2 hM, float2 (&v)[C::P], LdsT<C::M>& s,
                                         float2 wt, int tid) {
   const LdsPair own(s.re, s.im, pidx(tid)), mir(s.re, s.im, pidx(-tid));
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
     float2 z = cmul(x[m], v[m]);
     v[m] = z;
-    own.template st<POff<kT * m>::value>(z);
+    own.template st<POff<C::T * m>::value>(z);
   });
   if (tid == 0) {  // Z[M], read by k = 0
     float2 zM = cscale(hM, xM);
-    s.re[pidx(kM)] = zM.x;
-    s.im[pidx(kM)] = zM.y;
+    s.re[pidx(C::M)] = zM.x;
+    s.im[pidx(C::M)] = zM.y;
   }
   __syncthreads();
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    float2 B = cconj(mir.template ld<POff<kM - kT * m>::value>());
+    float2 B = cconj(mir.template ld<POff<C::M - C::T * m>::value>());
     float2 A = v[m];
     float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
-    float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), split_tw<m>(wt));
+    float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), split_tw<C, m>(wt));
     v[m] = make_float2(E.x - O.y, -(E.y + O.x));
     if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
   });
@@ -262,11 +336,12 @@ __device__ __forceinline__ void c2r_pre(const float2 (&x)[kP], float xM, float2 
 // stores are issued: the in-order memory counter then lets the next loads
 // complete without waiting for the stores, and the next transform runs
 // while the stores drain.
-__device__ __forceinline__ void stage_own(const float2 (&v)[kP], Lds& s, int tid) {
+template <class C>
+__device__ __forceinline__ void stage_own(const float2 (&v)[C::P], LdsT<C::M>& s, int tid) {
   const LdsPair own(s.re, s.im, pidx(tid));
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    own.template st<POff<kT * m>::value>(v[m]);
+    own.template st<POff<C::T * m>::value>(v[m]);
   });
 }
 
@@ -286,7 +361,8 @@ __device__ __forceinline__ uint32_t below_mask_of(uint32_t left) {
 // above high).  A wavefront's elements m cover 128 consecutive y; per 32 of
 // them (one output word) bits 0..15 are the even samples' and 16..31 the
 // odd samples' flags, from two ballots; lanes 0..3 store the four words.
-__device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid, Lds& s,
+template <class C>
+__device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid, LdsT<C::M>& s,
                                           uint32_t* words = nullptr, float highf = 0.0f) {
   const rsrc_t r = make_rsrc(o.out + base, (uint32_t)o.len * 4u);
   const LdsPair own(s.re, s.im, pidx(tid));
@@ -296,17 +372,17 @@ __device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid
     // even offset and length: y[2m], y[2m+1] land on an aligned pair that
     // is wholly inside or wholly outside [0, len): one 8-byte store
     const int q = tid & 3;
-    static_for<0, kP>([&](auto mc) {
+    static_for<0, C::P>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      const float2 v = own.template ld<POff<kT * m>::value>();
-      int n = 2 * (tid + kT * m);
-      uint32_t i0 = (uint32_t)((n - off) & (kN - 1));
+      const float2 v = own.template ld<POff<C::T * m>::value>();
+      int n = 2 * (tid + C::T * m);
+      uint32_t i0 = (uint32_t)((n - off) & (C::N - 1));
       const float y0 = v.x + dc, y1 = dc - v.y;
       bstore2(r, make_float2(y0, y1), i0 * 4u, 0u);
       if (words != nullptr) {
         const unsigned long long e0 = __ballot(y0 <= highf), e1 = __ballot(y1 <= highf);
         const uint32_t w = ((uint32_t)(e0 >> (16 * q)) & 0xffffu) | ((uint32_t)(e1 >> (16 * q)) << 16);
-        const uint32_t j0 = (uint32_t)((2 * (kT * m + (tid & ~63)) + 32 * q - off) & (kN - 1));
+        const uint32_t j0 = (uint32_t)((2 * (C::T * m + (tid & ~63)) + 32 * q - off) & (C::N - 1));
         if ((tid & 63) < 4 && j0 < (uint32_t)o.len) {
           const uint32_t left = (uint32_t)o.len - j0;
           words[j0 >> 5] = left >= 32 ? w : (w & below_mask_of(left));
@@ -315,12 +391,12 @@ __device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid
     });
     return;
   }
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    const float2 v = own.template ld<POff<kT * m>::value>();
-    int n = 2 * (tid + kT * m);
-    uint32_t i0 = (uint32_t)((n - off) & (kN - 1));
-    uint32_t i1 = (uint32_t)((n + 1 - off) & (kN - 1));
+    const float2 v = own.template ld<POff<C::T * m>::value>();
+    int n = 2 * (tid + C::T * m);
+    uint32_t i0 = (uint32_t)((n - off) & (C::N - 1));
+    uint32_t i1 = (uint32_t)((n + 1 - off) & (C::N - 1));
     bstore1(r, v.x + dc, i0 * 4u, 0u);
     bstore1(r, dc - v.y, i1 * 4u, 0u);
   });
@@ -336,25 +412,26 @@ __device__ __forceinline__ float inc_to_hz(float d, float hz_per_rad, float fs) 
 // Analogue audio stage 1 for one channel by one wavefront (utils.py:898-926,
 // core.py:857-866): slice bins, 1024-point inverse FFT, instantaneous
 // frequency at the decimated rate plus the slice's low frequency.
+template <class C>
 __device__ void audio_wave(const DemodArgs& a, int c, const float2* __restrict__ slice, float* lre, float* lim,
                            int b, const TwGlobal& twr) {
   const int lane = threadIdx.x & 63;
-  const rsrc_t F = make_rsrc(a.h_audio[c], kNA * 8u);
-  float2 v[kNA / 64];
+  const rsrc_t F = make_rsrc(a.h_audio[c], C::NA * 8u);
+  float2 v[C::NA / 64];
 #pragma unroll
-  for (int m = 0; m < kNA / 64; ++m) {
+  for (int m = 0; m < C::NA / 64; ++m) {
     int k = lane + 64 * m;
     float2 f = bload2(F, lane * 8u, m * 512u);
     float2 z = make_float2(0.0f, 0.0f);
-    if (k < kNA / 2)
+    if (k < C::NA / 2)
       z = cmul(slice[k], f);
-    else if (k == kNA / 2)  // bin N - lowbin - nbins/2 of the full spectrum
-      z = cmul(cconj(slice[kNA / 2]), f);
+    else if (k == C::NA / 2)  // bin N - lowbin - nbins/2 of the full spectrum
+      z = cmul(cconj(slice[C::NA / 2]), f);
     v[m] = z;
   }
-  InvA::run(v, lre, lim, twr);
+  C::Inv::run(v, lre, lim, twr);
 #pragma unroll
-  for (int m = 0; m < kNA / 64; ++m) {
+  for (int m = 0; m < C::NA / 64; ++m) {
     int n = lane + 64 * m;
     lre[pidx(n)] = v[m].x;
     lim[pidx(n)] = v[m].y;
@@ -364,7 +441,7 @@ __device__ void audio_wave(const DemodArgs& a, int c, const float2* __restrict__
   const float hz_per_rad = rate / kTwoPi;
   const rsrc_t out = make_rsrc(a.out_audio[c] + (int64_t)b * a.astride, (uint32_t)a.alen * 4u);
 #pragma unroll
-  for (int m = 0; m < kNA / 64; ++m) {
+  for (int m = 0; m < C::NA / 64; ++m) {
     int n = lane + 64 * m;
     float hz = 0.0f;
     if (n > 0) {
@@ -425,11 +502,12 @@ __device__ __forceinline__ bool notch_zero(const int* flag, int lo, int hi, int 
 
 // The thread's bins tid + T m of a half-spectrum table (bin M of every
 // table sits in LDS, see bin_m).
-__device__ __forceinline__ void load_half(const float2* table, float2 (&h)[kP], int tid) {
-  const rsrc_t H = make_rsrc(table, kHalfBytes);
+template <class C>
+__device__ __forceinline__ void load_half(const float2* table, float2 (&h)[C::P], int tid) {
+  const rsrc_t H = make_rsrc(table, C::HalfBytes);
   const uint32_t vo = tid * 8u;
 #pragma unroll
-  for (int m = 0; m < kP; ++m) h[m] = bload2(H, vo, m * kStrideB);
+  for (int m = 0; m < C::P; ++m) h[m] = bload2(H, vo, m * C::StrideB);
 }
 
 // ---------------------------------------------------------------- kernel
@@ -449,22 +527,23 @@ enum { kPhIn = 0, kPhHpf = 1, kPhEven = 2, kPhOdd = 3, kPhDemod = 4, kPhPost = 5
 // The EFM output (core.py:847-851) from a transform staged as c2r_store
 // reads it: np.int16(np.clip(y, -32768, 32767)) (truncation toward zero)
 // for i < len; pairs on even offsets go out as one 32-bit store.
-__device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int tid, Lds& s) {
+template <class C>
+__device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int tid, LdsT<C::M>& s) {
   const rsrc_t r = make_rsrc(outp, (uint32_t)len * 2u);
   const LdsPair own(s.re, s.im, pidx(tid));
   auto q = [](float y) -> uint32_t {
     return (uint32_t)(uint16_t)(int16_t)(int)fminf(fmaxf(y, -32768.0f), 32767.0f);
   };
   const bool pairs = ((cut | len) & 1) == 0;
-  static_for<0, kP>([&](auto mc) {
+  static_for<0, C::P>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    const float2 v = own.template ld<POff<kT * m>::value>();
-    const int n = 2 * (tid + kT * m);
-    const uint32_t i0 = (uint32_t)((n - cut) & (kN - 1));
+    const float2 v = own.template ld<POff<C::T * m>::value>();
+    const int n = 2 * (tid + C::T * m);
+    const uint32_t i0 = (uint32_t)((n - cut) & (C::N - 1));
     if (pairs) {
       __builtin_amdgcn_raw_buffer_store_b32(q(v.x) | (q(-v.y) << 16), r, (int)(i0 * 2u), 0, 0);
     } else {
-      const uint32_t i1 = (uint32_t)((n + 1 - cut) & (kN - 1));
+      const uint32_t i1 = (uint32_t)((n + 1 - cut) & (C::N - 1));
       __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(v.x), r, (int)(i0 * 2u), 0, 0);
       __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(-v.y), r, (int)(i1 * 2u), 0, 0);
     }
@@ -481,31 +560,31 @@ __device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int t
 //
 // kMfma: the transforms' radix-64 stages run as split-BF16 matrix products on
 // the matrix cores (WgFft16kMfma) instead of FP32 radix-16 passes.
-template <bool kPacked, bool kEfm, bool kMfma = false>
+template <class C, bool kPacked, bool kEfm, bool kMfma = false>
 __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int bo, int bw) {
-  __shared__ Lds s;
+  __shared__ LdsT<C::M> s;
   __shared__ MfmaTw mtw;
-  __shared__ float2 aslice[2][kNA / 2 + 1];
-  __shared__ TwLds twl;
+  __shared__ float2 aslice[2][C::NA / 2 + 1];
+  __shared__ typename C::Tw twl;
   __shared__ float notch_mag[kNotchMax];
   __shared__ int notch_flag[kNotchMax + 4];
   __shared__ double notch_red[128];
-  __shared__ uint32_t notch_mask[kT];  // bit m: bin tid + T m zeroed by the spur notch
+  __shared__ uint32_t notch_mask[C::T];  // bit m: bin tid + T m zeroed by the spur notch
   // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters, 6 EFM
   __shared__ float2 bin_m[2 + 4 + 1];
   const int tid = threadIdx.x;
-  const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)bw * kM, kM * 8u);
+  const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)bw * C::M, C::M * 8u);
   const uint32_t vo = tid * 8u;
   {  // twiddle tables into LDS
-    const rsrc_t src = make_rsrc(a.tw_lds, kTwLdsCount * 8u);
+    const rsrc_t src = make_rsrc(a.tw_lds, C::Tw::count * 8u);
     float2* dst = reinterpret_cast<float2*>(&twl);
-    for (int i = tid; i < kTwLdsCount; i += kT) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
+    for (int i = tid; i < C::Tw::count; i += C::T) dst[i] = bload2(src, (uint32_t)i * 8u, 0u);
     if (tid < 2 + a.npost) {
       const float2* t = tid == 0 ? a.hpf.h : (tid == 1 ? a.h_rfv : a.post[tid - 2].h);
-      bin_m[tid] = t ? bload2(make_rsrc(t, kHalfBytes), 0u, kM * 8u) : make_float2(0.0f, 0.0f);
+      bin_m[tid] = t ? bload2(make_rsrc(t, C::HalfBytes), 0u, C::M * 8u) : make_float2(0.0f, 0.0f);
     }
     if constexpr (kEfm) {
-      if (tid == kT - 1) bin_m[6] = bload2(make_rsrc(a.h_efm, kHalfBytes), 0u, kM * 8u);
+      if (tid == C::T - 1) bin_m[6] = bload2(make_rsrc(a.h_efm, C::HalfBytes), 0u, C::M * 8u);
     }
     if constexpr (kMfma) {
       const rsrc_t mt = make_rsrc(a.mfma_tw, sizeof(MfmaTw));
@@ -513,12 +592,12 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     }
   }
   __syncthreads();
-  const TwSrcLds twf{&twl};
-  const float2 wt0 = bload2(make_rsrc(a.tw_split, kT * 8u), vo, 0u);
+  const TwSrcLdsT<typename C::Tw> twf{&twl};
+  const float2 wt0 = bload2(make_rsrc(a.tw_split, C::T * 8u), vo, 0u);
   // v: the transform's data; between phases it carries the next phase's
   // filter (bins tid + T m), loaded after the transform and before that
   // phase's output stores (see stage_own)
-  float2 v[kP], x[kP];
+  float2 v[C::P], x[C::P];
   float xM = 0.0f;
   const int last = kPhPost + a.npost;
   const bool has_hpf = a.hpf.out != nullptr;
@@ -535,14 +614,14 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     const uint32_t q5 = (uint32_t)(5 * (s0 >> 2));
     const uint32_t r = (uint32_t)(s0 & 3);
     const uint32_t A0 = q5 & ~3u;
-    const uint32_t lastb = q5 + 5u * ((r + (uint32_t)kN - 1u) >> 2) + ((r + (uint32_t)kN - 1u) & 3u) + 1u;
+    const uint32_t lastb = q5 + 5u * ((r + (uint32_t)C::N - 1u) >> 2) + ((r + (uint32_t)C::N - 1u) & 3u) + 1u;
     const int nwords = (int)((lastb - A0) / 4u + 1u);
     uint32_t* w32 = reinterpret_cast<uint32_t*>(s.re);
-    for (int i = tid; i < nwords; i += kT) w32[i] = __builtin_bit_cast(uint32_t, bload1(pk, A0 + 4u * (uint32_t)i, 0u));
+    for (int i = tid; i < nwords; i += C::T) w32[i] = __builtin_bit_cast(uint32_t, bload1(pk, A0 + 4u * (uint32_t)i, 0u));
     __syncthreads();
-    static_for<0, kP>([&](auto mc) {
+    static_for<0, C::P>([&](auto mc) {
       constexpr int m = decltype(mc)::value;
-      const uint32_t jr = r + 2u * (uint32_t)(tid + kT * m);
+      const uint32_t jr = r + 2u * (uint32_t)(tid + C::T * m);
       const uint32_t B = q5 + 5u * (jr >> 2) + (jr & 3u) - A0;
       const uint32_t W = __builtin_amdgcn_alignbyte(w32[(B >> 2) + 1u], w32[B >> 2], B & 3u);
       const uint32_t r0 = jr & 3u;
@@ -566,22 +645,22 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     // ---------------------------------------------------------- pre
     if (ph == kPhIn) {
       // spectrum of the block (core.py:761): y[j] = x[2j] + i x[2j+1]
-      const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(kN * a.in_bytes));
+      const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(C::N * a.in_bytes));
       if constexpr (kPacked) {
         // decoded before the phase loop (see the prologue)
       } else if (a.in_bytes == 4) {
 #pragma unroll
-        for (int m = 0; m < kP; ++m) v[m] = bload2(in, vo, m * kStrideB);
+        for (int m = 0; m < C::P; ++m) v[m] = bload2(in, vo, m * C::StrideB);
       } else if ((s0 & 1) == 0) {
 #pragma unroll
-        for (int m = 0; m < kP; ++m) {
-          uint32_t w = __builtin_bit_cast(uint32_t, bload1(in, tid * 4u, m * (kT * 4u)));
+        for (int m = 0; m < C::P; ++m) {
+          uint32_t w = __builtin_bit_cast(uint32_t, bload1(in, tid * 4u, m * (C::T * 4u)));
           v[m] = make_float2((float)(int16_t)(w & 0xffffu), (float)(int16_t)(w >> 16));
         }
       } else {  // odd start: 16-bit loads
 #pragma unroll
-        for (int m = 0; m < kP; ++m) {
-          uint32_t o = tid * 4u + m * (kT * 4u);
+        for (int m = 0; m < C::P; ++m) {
+          uint32_t o = tid * 4u + m * (C::T * 4u);
           int16_t lo = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o, 0, 0);
           int16_t hi = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o + 2, 0, 0);
           v[m] = make_float2((float)lo, (float)hi);
@@ -590,10 +669,10 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     } else if (ph == kPhHpf) {
       // RF high-pass branch (core.py:769-772)
       if (!has_hpf) continue;
-      c2r_pre(x, xM, bin_m[0], v, s, wt, tidv);
+      c2r_pre<C>(x, xM, bin_m[0], v, s, wt, tidv);
     } else if (kEfm && ph == kPhEfm) {
       // EFM equaliser on the input spectrum (core.py:847)
-      c2r_pre(x, xM, bin_m[6], v, s, wt, tidv);
+      c2r_pre<C>(x, xM, bin_m[6], v, s, wt, tidv);
     } else if (ph == kPhEven || ph == kPhOdd) {
       // analytic signal (core.py:800-805), conjugated: even samples from
       // Z[k] (+ Z[M] folded into bin 0), odd samples from Z[k] W_N^-k
@@ -601,15 +680,15 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       const float2 hM = cscale(bin_m[1], xM);  // RFVideo bin M
       const bool odd = (ph == kPhOdd);
       const uint32_t zmask = a.notch_hi > a.notch_lo ? notch_mask[tidv] : 0u;
-      static_for<0, kP>([&](auto mc) {
+      static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const int k = tid + kT * m;
+        const int k = tid + C::T * m;
         float2 z = cmul(x[m], v[m]);
         if (zmask & (1u << m)) z = make_float2(0.0f, 0.0f);
         if (k == 0)
           z = odd ? csub(z, hM) : cadd(z, hM);
         else if (odd)
-          z = cmulc(z, split_tw<m>(wt));
+          z = cmulc(z, split_tw<C, m>(wt));
         v[m] = cconj(z);
       });
     } else if (ph == kPhDemod) {
@@ -618,19 +697,19 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       // conjugates is minus the wanted increment.  Odd samples go to LDS
       // shifted by one so each thread reads z[2j-1] at its own index j.
       const LdsPair sh(s.re, s.im, pidx(tidv + 1)), own(s.re, s.im, pidx(tidv));
-      static_for<0, kP>([&](auto mc) {
+      static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        sh.template st<POff<kT * m>::value>(v[m]);
+        sh.template st<POff<C::T * m>::value>(v[m]);
       });
       __syncthreads();
       const float hz_per_rad = a.fs / kTwoPi;
       // every even-sample load comes before any demod_raw store, so the
       // in-order memory counter never makes a load wait for a store
-      static_for<0, kP>([&](auto mc) {
+      static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const int j = tidv + kT * m;
-        float2 e = bload2(zev, vo, m * kStrideB), od = v[m];
-        float2 prev = own.template ld<POff<kT * m>::value>();
+        const int j = tidv + C::T * m;
+        float2 e = bload2(zev, vo, m * C::StrideB), od = v[m];
+        float2 prev = own.template ld<POff<C::T * m>::value>();
         float2 p = cmulc(e, prev);
         float h0 = (j > 0) ? inc_to_hz(-atan2f(p.y, p.x), hz_per_rad, a.fs) : 0.0f;
         float2 q = cmulc(od, e);
@@ -641,8 +720,8 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
         const rsrc_t raw = make_rsrc(a.demod_raw + (int64_t)bo * a.vstride, (uint32_t)a.vlen * 4u);
         const bool pairs = ((a.vcut | a.vlen) & 1) == 0;
 #pragma unroll
-        for (int m = 0; m < kP; ++m) {  // i < 0 wraps to a huge offset: dropped
-          const uint32_t i0 = (uint32_t)(2 * (tidv + kT * m) - a.vcut);
+        for (int m = 0; m < C::P; ++m) {  // i < 0 wraps to a huge offset: dropped
+          const uint32_t i0 = (uint32_t)(2 * (tidv + C::T * m) - a.vcut);
           if (pairs) {
             bstore2(raw, v[m], i0 * 4u, 0u);  // aligned pair, wholly in or out
           } else {
@@ -652,28 +731,28 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
         }
       }
 #pragma unroll
-      for (int m = 0; m < kP; ++m)
+      for (int m = 0; m < C::P; ++m)
         v[m] = make_float2(fminf(fmaxf(v[m].x, a.clip_lo), a.clip_hi) - a.dc,
                            fminf(fmaxf(v[m].y, a.clip_lo), a.clip_hi) - a.dc);
     } else {
       // video post filters (core.py:811-820)
-      c2r_pre(x, xM, bin_m[2 + ph - kPhPost], v, s, wt, tidv);
+      c2r_pre<C>(x, xM, bin_m[2 + ph - kPhPost], v, s, wt, tidv);
     }
 
     // ---------------------------------------------------------- transform
     if constexpr (kMfma)
-      WgFft16kMfma::run(v, s.re, s.im, twf, a.mfma_g, &mtw, tidv);
+      C::FwdMfma::run(v, s.re, s.im, twf, a.mfma_g, &mtw, tidv);
     else
-      FwdM::run(v, s.re, s.im, twf, tidv);
+      C::Fwd::run(v, s.re, s.im, twf, tidv);
 
     // ---------------------------------------------------------- post
     if (ph == kPhIn || ph == kPhDemod) {
-      r2c_split(v, x, xM, s, wt, tidv);
+      r2c_split<C>(v, x, xM, s, wt, tidv);
       if (ph == kPhIn) {
         if (a.notch_hi > a.notch_lo) {  // PAL spur notch decision (core.py:774-798)
-          static_for<0, kP>([&](auto mc) {
+          static_for<0, C::P>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            const int k = tid + kT * m;
+            const int k = tid + C::T * m;
             if (k >= a.notch_lo && k < a.notch_hi)
               notch_mag[k - a.notch_lo] = sqrtf(x[m].x * x[m].x + x[m].y * x[m].y);
           });
@@ -681,9 +760,9 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
           if (tid < 64) notch_decide(notch_mag, notch_flag, notch_red, a.notch_lo, a.notch_hi);
           __syncthreads();
           uint32_t zm = 0;
-          static_for<0, kP>([&](auto mc) {
+          static_for<0, C::P>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
-            if (notch_zero(notch_flag, a.notch_lo, a.notch_hi, tid + kT * m)) zm |= 1u << m;
+            if (notch_zero(notch_flag, a.notch_lo, a.notch_hi, tid + C::T * m)) zm |= 1u << m;
           });
           notch_mask[tid] = zm;
         }
@@ -692,9 +771,9 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
           constexpr int c = decltype(cc)::value;
           if (a.out_audio[c] != nullptr) {
 #pragma unroll
-            for (int m = 0; m < kP; ++m) {
-              int i = tidv + kT * m - a.lowbin[c];
-              if (i >= 0 && i <= kNA / 2) aslice[c][i] = (zmask & (1u << m)) ? make_float2(0.0f, 0.0f) : x[m];
+            for (int m = 0; m < C::P; ++m) {
+              int i = tidv + C::T * m - a.lowbin[c];
+              if (i >= 0 && i <= C::NA / 2) aslice[c][i] = (zmask & (1u << m)) ? make_float2(0.0f, 0.0f) : x[m];
             }
           }
         });
@@ -702,29 +781,29 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       // the next phase's filter: rfhpf's or RFVideo's after the input
       // transform, the first post filter's after the demod transform
       if (ph == kPhIn)
-        load_half(has_hpf ? a.hpf.h : (kEfm ? a.h_efm : a.h_rfv), v, tidv);
+        load_half<C>(has_hpf ? a.hpf.h : (kEfm ? a.h_efm : a.h_rfv), v, tidv);
       else if (a.npost > 0)
-        load_half(a.post[0].h, v, tidv);
+        load_half<C>(a.post[0].h, v, tidv);
     } else if (ph == kPhEven) {
-      stage_own(v, s, tidv);
-      load_half(a.h_rfv, v, tidv);  // RFVideo again for the odd half
+      stage_own<C>(v, s, tidv);
+      load_half<C>(a.h_rfv, v, tidv);  // RFVideo again for the odd half
       const LdsPair own(s.re, s.im, pidx(tidv));
-      static_for<0, kP>([&](auto mc) {
+      static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        bstore2(zev, own.template ld<POff<kT * m>::value>(), vo, m * kStrideB);
+        bstore2(zev, own.template ld<POff<C::T * m>::value>(), vo, m * C::StrideB);
       });
     } else if (ph == kPhHpf) {
-      stage_own(v, s, tidv);
-      load_half(kEfm ? a.h_efm : a.h_rfv, v, tidv);
-      c2r_store(a.hpf, (int64_t)bo * a.vstride, tidv, s);
+      stage_own<C>(v, s, tidv);
+      load_half<C>(kEfm ? a.h_efm : a.h_rfv, v, tidv);
+      c2r_store<C>(a.hpf, (int64_t)bo * a.vstride, tidv, s);
     } else if (kEfm && ph == kPhEfm) {
-      stage_own(v, s, tidv);
-      load_half(a.h_rfv, v, tidv);
-      efm_store(a.efm + (int64_t)bo * a.vstride, a.vcut, a.vlen, tidv, s);
+      stage_own<C>(v, s, tidv);
+      load_half<C>(a.h_rfv, v, tidv);
+      efm_store<C>(a.efm + (int64_t)bo * a.vstride, a.vcut, a.vlen, tidv, s);
     } else if (ph >= kPhPost) {
-      stage_own(v, s, tidv);
-      if (ph + 1 < last) load_half(a.post[ph + 1 - kPhPost].h, v, tidv);
-      c2r_store(a.post[ph - kPhPost], (int64_t)bo * a.vstride, tidv, s,
+      stage_own<C>(v, s, tidv);
+      if (ph + 1 < last) load_half<C>(a.post[ph + 1 - kPhPost].h, v, tidv);
+      c2r_store<C>(a.post[ph - kPhPost], (int64_t)bo * a.vstride, tidv, s,
                 ph - kPhPost == a.below_post ? a.below + (((int64_t)bo * a.vstride) >> 5) : nullptr, a.below_highf);
     }
   }
@@ -733,20 +812,32 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   __syncthreads();
   if (tid < 128) {
     const int c = tid >> 6;
-    const TwGlobal tg{make_rsrc(a.tw_rows, TwLayout::total * 8u), TwLayout::k1k_p2, TwLayout::k1k_p3, 16, 4, 2};
+    const TwGlobal tg = C::audio_tw(make_rsrc(a.tw_rows, TwLayout::total * 8u));
     if (a.out_audio[c] != nullptr)
-      audio_wave(a, c, aslice[c], s.re + c * LdsSize<kNA>::floats, s.im + c * LdsSize<kNA>::floats, bo, tg);
+      audio_wave<C>(a, c, aslice[c], s.re + c * LdsSize<C::NA>::floats, s.im + c * LdsSize<C::NA>::floats, bo, tg);
   }
 }
 
 #define LD_DEMOD_AT(a) (a), (a).first + (a).step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x
-__global__ __launch_bounds__(kT) void k_demod_block(DemodArgs a) { demod_block<false, false>(LD_DEMOD_AT(a)); }
-__global__ __launch_bounds__(kT) void k_demod_block_packed(DemodArgs a) { demod_block<true, false>(LD_DEMOD_AT(a)); }
-__global__ __launch_bounds__(kT) void k_demod_block_efm(DemodArgs a) { demod_block<false, true>(LD_DEMOD_AT(a)); }
-__global__ __launch_bounds__(kT) void k_demod_block_packed_efm(DemodArgs a) { demod_block<true, true>(LD_DEMOD_AT(a)); }
-__global__ __launch_bounds__(kT) void k_demod_block_mfma(DemodArgs a) {
-  demod_block<false, false, true>(LD_DEMOD_AT(a));
-}
+#define LD_DEMOD_KERNELS(C, sfx)                                                                             \
+  __global__ __launch_bounds__(C::T) void k_demod_block##sfx(DemodArgs a) {                                 \
+    demod_block<C, false, false>(LD_DEMOD_AT(a));                                                            \
+  }                                                                                                          \
+  __global__ __launch_bounds__(C::T) void k_demod_block_packed##sfx(DemodArgs a) {                          \
+    demod_block<C, true, false>(LD_DEMOD_AT(a));                                                             \
+  }                                                                                                          \
+  __global__ __launch_bounds__(C::T) void k_demod_block_efm##sfx(DemodArgs a) {                             \
+    demod_block<C, false, true>(LD_DEMOD_AT(a));                                                             \
+  }                                                                                                          \
+  __global__ __launch_bounds__(C::T) void k_demod_block_packed_efm##sfx(DemodArgs a) {                      \
+    demod_block<C, true, true>(LD_DEMOD_AT(a));                                                              \
+  }                                                                                                          \
+  __global__ __launch_bounds__(C::T) void k_demod_block_mfma##sfx(DemodArgs a) {                            \
+    demod_block<C, false, false, true>(LD_DEMOD_AT(a));                                                      \
+  }
+LD_DEMOD_KERNELS(Cfg32, )
+LD_DEMOD_KERNELS(Cfg16, 16)
+#undef LD_DEMOD_KERNELS
 #undef LD_DEMOD_AT
 
 // Blocks of many captures and both systems in one launch: launch block i
@@ -763,16 +854,18 @@ struct DemodBatchArgs {
   int nslots[2];
 };
 static_assert(sizeof(DemodBatchArgs) <= 4096, "kernel arguments over 4 KiB");
-template <bool kMfma>
+template <class C, bool kMfma>
 __device__ __forceinline__ void demod_batch(const DemodBatchArgs& a) {
   const ld_batch_block d = a.blocks[blockIdx.x];
-  if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - kN || d.slot < 0 ||
+  if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - C::N || d.slot < 0 ||
       d.slot >= a.nslots[d.sys])
     return;
-  demod_block<false, false, kMfma>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
+  demod_block<C, false, false, kMfma>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
 }
-__global__ __launch_bounds__(kT) void k_demod_batch(DemodBatchArgs a) { demod_batch<false>(a); }
-__global__ __launch_bounds__(kT) void k_demod_batch_mfma(DemodBatchArgs a) { demod_batch<true>(a); }
+__global__ __launch_bounds__(Cfg32::T) void k_demod_batch(DemodBatchArgs a) { demod_batch<Cfg32, false>(a); }
+__global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma(DemodBatchArgs a) { demod_batch<Cfg32, true>(a); }
+__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16(DemodBatchArgs a) { demod_batch<Cfg16, false>(a); }
+__global__ __launch_bounds__(Cfg16::T) void k_demod_batch_mfma16(DemodBatchArgs a) { demod_batch<Cfg16, true>(a); }
 
 // The demodulator's forward M-point transform alone (FP32 or split-BF16
 // MFMA), one workgroup per transform: in[b M .. + M) -> out[b M .. + M).
@@ -1362,10 +1455,10 @@ __global__ __launch_bounds__(kT) void k_audio2_block(Audio2Args a) {
   const float2 wt = bload2(make_rsrc(a.tw_split, kT * 8u), vo, 0u);
   float xM = 0.0f;
   FwdM::run(v, s.re, s.im, twf, tid);
-  r2c_split(v, x, xM, s, wt, tid);
-  load_half(a.h[c], v, tid);
+  r2c_split<Cfg32>(v, x, xM, s, wt, tid);
+  load_half<Cfg32>(a.h[c], v, tid);
   const float2 hM = bload2(make_rsrc(a.h[c], kHalfBytes), 0u, kM * 8u);
-  c2r_pre(x, xM, hM, v, s, wt, tid);
+  c2r_pre<Cfg32>(x, xM, hM, v, s, wt, tid);
   FwdM::run(v, s.re, s.im, twf, tid);
   double* out = a.out[c] + sg.out_base;
   const double center = a.center[c];
@@ -1452,8 +1545,8 @@ int ld_create(int device, int blocklen, ld_handle** out) {
     return LD_EINVAL;
   }
   *out = nullptr;
-  if (blocklen != kN && blocklen != kN64) {
-    set_error("ld_create: blocklen %d not supported (%d or %d)", blocklen, kN, kN64);
+  if (blocklen != Cfg16::N && blocklen != kN && blocklen != kN64) {
+    set_error("ld_create: blocklen %d not supported (%d, %d or %d)", blocklen, Cfg16::N, kN, kN64);
     return LD_EINVAL;
   }
   LD_HIP(hipSetDevice(device));
@@ -1483,10 +1576,18 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   rows(TwLayout::k1k_p3, 4, 256);
   rows(TwLayout::k2k_p2, 32, 32);
   rows(TwLayout::k2k_p3, 2, 1024);
-  TwLds::fill(tw.data() + TwLayout::total, cexp);
-  // split[t] = exp(-2 pi i t / 32768): W_N^t of the 32768-sample kernel's real
-  // split, W_M^t of the 65536-sample kernel's radix-2 step and its split
-  for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, kN);
+  rows(TwLayout::k512_p2, 8, 8);
+  rows(TwLayout::k512_p3, 8, 64);
+  // split[t] = exp(-2 pi i t / 32768), t < 1024: W_N^t of the 32768-sample
+  // kernel's real split, W_M^t of the 65536-sample kernel's radix-2 step and
+  // its split; at 16384 samples split[t] = exp(-2 pi i t / 16384), t < 512
+  if (blocklen == Cfg16::N) {
+    TwLds8k::fill(tw.data() + TwLayout::total, cexp);
+    for (int t2 = 0; t2 < Cfg16::T; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, Cfg16::N);
+  } else {
+    TwLds::fill(tw.data() + TwLayout::total, cexp);
+    for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, kN);
+  }
   hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
   // split-BF16 MFMA variant: the DFT-64 matrix [[F_re, -F_im]; [F_im, F_re]]
@@ -1927,17 +2028,23 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
     }
     hipLaunchKernelGGL(h->fft_variant == LD_FFT_MFMA_BF16X3 ? k_demod_block64_mfma : k_demod_block64, dim3(nblocks),
                        dim3(kT), 0, st, a);
-  } else if (h->fft_variant == LD_FFT_MFMA_BF16X3) {
-    if (dtype == LD_IN_PACKED10 || a.efm) {
-      set_error("ld_demod_blocks: the split-BF16 MFMA variant takes int16 / float32 input without EFM output");
-      return LD_EINVAL;
-    }
-    hipLaunchKernelGGL(k_demod_block_mfma, dim3(nblocks), dim3(kT), 0, st, a);
-  } else if (dtype == LD_IN_PACKED10) {
-    hipLaunchKernelGGL(a.efm ? k_demod_block_packed_efm : k_demod_block_packed, dim3(nblocks), dim3(kT), 0, st, a);
-  } else {
-    hipLaunchKernelGGL(a.efm ? k_demod_block_efm : k_demod_block, dim3(nblocks), dim3(kT), 0, st, a);
+    LD_HIP(hipGetLastError());
+    return LD_OK;
   }
+  if (h->fft_variant == LD_FFT_MFMA_BF16X3 && (dtype == LD_IN_PACKED10 || a.efm)) {
+    set_error("ld_demod_blocks: the split-BF16 MFMA variant takes int16 / float32 input without EFM output");
+    return LD_EINVAL;
+  }
+  const bool b16 = h->blocklen == Cfg16::N;
+  void (*kern)(DemodArgs);
+  if (h->fft_variant == LD_FFT_MFMA_BF16X3)
+    kern = b16 ? k_demod_block_mfma16 : k_demod_block_mfma;
+  else if (dtype == LD_IN_PACKED10)
+    kern = a.efm ? (b16 ? k_demod_block_packed_efm16 : k_demod_block_packed_efm)
+                 : (b16 ? k_demod_block_packed16 : k_demod_block_packed);
+  else
+    kern = a.efm ? (b16 ? k_demod_block_efm16 : k_demod_block_efm) : (b16 ? k_demod_block16 : k_demod_block);
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(b16 ? Cfg16::T : kT), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }
@@ -1992,6 +2099,8 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t 
   const bool mf = h0->fft_variant == LD_FFT_MFMA_BF16X3;
   if (h0->blocklen == kN64)
     hipLaunchKernelGGL(mf ? k_demod_batch64_mfma : k_demod_batch64, dim3(nblocks), dim3(kT), 0, st, b);
+  else if (h0->blocklen == Cfg16::N)
+    hipLaunchKernelGGL(mf ? k_demod_batch_mfma16 : k_demod_batch16, dim3(nblocks), dim3(Cfg16::T), 0, st, b);
   else
     hipLaunchKernelGGL(mf ? k_demod_batch_mfma : k_demod_batch, dim3(nblocks), dim3(kT), 0, st, b);
   LD_HIP(hipGetLastError());
@@ -2010,6 +2119,10 @@ int ld_set_fft_variant(ld_handle* h, int variant) {
 int ld_fft_forward(ld_handle* h, const void* in, void* out, int ntransforms, int variant, void* stream) {
   if (!h || !in || !out || ntransforms <= 0 || (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3)) {
     set_error("ld_fft_forward: bad arguments");
+    return LD_EINVAL;
+  }
+  if (h->blocklen == Cfg16::N) {
+    set_error("ld_fft_forward: the 16384-point transform needs a 32768- or 65536-sample handle");
     return LD_EINVAL;
   }
   DemodArgs a = {};

@@ -309,7 +309,11 @@ struct TwLayout {
   // and (R=2, NS=1024)
   static constexpr int k2k_p2 = k1k_p3 + 256 * 2;
   static constexpr int k2k_p3 = k2k_p2 + 32 * 5;
-  static constexpr int total = k2k_p3 + 1024 * 1;
+  // 512 points (audio slices of 16384-sample blocks): passes (R=8, NS=8)
+  // and (R=8, NS=64)
+  static constexpr int k512_p2 = k2k_p3 + 1024 * 1;
+  static constexpr int k512_p3 = k512_p2 + 8 * 3;
+  static constexpr int total = k512_p3 + 64 * 3;
 };
 
 // Twiddle source reading the global rows through a buffer descriptor.
@@ -319,8 +323,8 @@ struct TwGlobal {
   int ns_p2, lr_p2, lr_p3;
   template <int NS>
   __device__ __forceinline__ float2 get(int k, int i) const {
-    const int off = (NS == 16 || NS == 32) ? off_p2 : off_p3;
-    const int lr = (NS == 16 || NS == 32) ? lr_p2 : lr_p3;
+    const int off = (NS == ns_p2) ? off_p2 : off_p3;
+    const int lr = (NS == ns_p2) ? lr_p2 : lr_p3;
     return bload2(tw, (uint32_t)(off + k * lr + i) * 8u, 0u);
   }
 };
@@ -356,6 +360,33 @@ struct WgFft16kT1024 {
     stockham_pass<16384, 1024, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
     stockham_pass<16384, 1024, 16, 256, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
     stockham_pass<16384, 1024, 4, 4096, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
+  }
+};
+
+// 512 points, 64 threads (one wavefront): radix 8 x 8 x 8
+template <int DIR, int SYNC>
+struct WgFft<512, DIR, SYNC> {
+  static constexpr int T = 64;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[8], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<512, 64, 8, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<512, 64, 8, 8, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<512, 64, 8, 64, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
+  }
+};
+
+// 8192 points, 512 threads (16 per thread): radix 16 x 16 x 16 x 2
+template <int DIR, int SYNC = kSyncBlock>
+struct WgFft8kT512 {
+  static constexpr int T = 512;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<8192, 512, 16, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<8192, 512, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<8192, 512, 16, 256, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<8192, 512, 2, 4096, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
   }
 };
 
@@ -503,6 +534,26 @@ __device__ __forceinline__ void mfma64_pass(float* __restrict__ lre, float* __re
   }
   __syncthreads();
 }
+
+// 8192 points, 512 threads, forward: radix 64 x 64 on the matrix cores
+// (split BF16), then radix 2 in FP32.
+struct WgFft8kMfma {
+  static constexpr int T = 512;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
+                                             const u32x4_t* gtab, const MfmaTw* tw, int tid = threadIdx.x) {
+    __syncthreads();
+    const LdsPair own(lre, lim, pidx(tid));
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      own.template st<POff<512 * m>::value>(v[m]);
+    });
+    __syncthreads();
+    mfma64_pass<8192, 512, 1>(lre, lim, gtab, tw, tid);
+    mfma64_pass<8192, 512, 64>(lre, lim, gtab, tw, tid);
+    stockham_pass<8192, 512, 2, 4096, -1, true, false, kSyncBlock>(v, lre, lim, twf, tid);
+  }
+};
 
 // 16384 points, 1024 threads, forward: radix 64 x 64 on the matrix cores
 // (split BF16), then radix 4 in FP32.  On entry v[m] = x[tid + T m]; on exit

@@ -1,9 +1,10 @@
-"""65536-sample blocks (RFDecode(blocklen=65536), the reference's blocklen
-constructor argument, core.py:284-288): the two-level transform kernel
-against the FP64 oracle built at the same block length (its filters are
-freqz samples at 65536 bins), in FP32 and with the split-BF16 MFMA radix-64
-stages, single and stitched blocks, and the batched decode at that length
-against the per-capture decode.
+"""16384- and 65536-sample blocks (RFDecode(blocklen=...), the reference's
+blocklen constructor argument, core.py:284-288): the single-level 8192-point
+kernel (16384) and the two-level transform kernel (65536) against the FP64
+oracle built at the same block length (its filters are freqz samples at
+that many bins), in FP32 and with the split-BF16 MFMA radix-64 stages,
+single and stitched blocks, and the batched decode at that length against
+the per-capture decode.
 
 Tolerances as the 32768-sample parity tests: video max |err| <= 25 Hz and
 1e-4 relative (to max(|value|, 1 MHz)); rfhpf <= 0.05 codes; audio stage 1
@@ -18,21 +19,21 @@ from oracle import dsp as O
 
 pytestmark = pytest.mark.gpu
 
-N64 = 65536
+LENGTHS = [16384, 65536]
 _RF, _ORF = {}, {}
 
 
-def _rf(system):
-    if system not in _RF:
+def _rf(system, n):
+    if (system, n) not in _RF:
         from lddecode_amd.rfdecode import RFDecode
-        _RF[system] = RFDecode(system=system, blocklen=N64, decode_analog_audio=44100, device=0)
-    return _RF[system]
+        _RF[system, n] = RFDecode(system=system, blocklen=n, decode_analog_audio=44100, device=0)
+    return _RF[system, n]
 
 
-def _orf(system):
-    if system not in _ORF:
-        _ORF[system] = O.OracleRF(system, blocklen=N64, decode_analog_audio=True)
-    return _ORF[system]
+def _orf(system, n):
+    if (system, n) not in _ORF:
+        _ORF[system, n] = O.OracleRF(system, blocklen=n, decode_analog_audio=True)
+    return _ORF[system, n]
 
 
 def _np(t):
@@ -48,11 +49,12 @@ def _check(got, ref, fields, tol=25.0, rel=1e-4):
         assert (err / np.maximum(np.abs(r), 1e6)).max() <= rel, k
 
 
+@pytest.mark.parametrize("n", LENGTHS)
 @pytest.mark.parametrize("system,cut", [("NTSC", True), ("NTSC", False), ("PAL", True)])
-def test_block64_matches_oracle(torch_cuda, system, cut):
-    rf, orf = _rf(system), _orf(system)
-    assert rf.block_stride == N64 - 1056
-    cap = synth.make_capture(system, N64, seed=7, snr_db=40.0)
+def test_block_matches_oracle(torch_cuda, n, system, cut):
+    rf, orf = _rf(system, n), _orf(system, n)
+    assert rf.block_stride == n - 1056
+    cap = synth.make_capture(system, n, seed=7, snr_db=40.0)
     got = rf.demodblock(data=cap.data, mtf_level=0, cut=cut)
     ref = orf.demodblock(data=cap.data.astype(np.float64), mtf_level=0, cut=cut)
     _check(got["video"], ref["video"], rf.video_fields)
@@ -64,10 +66,11 @@ def test_block64_matches_oracle(torch_cuda, system, cut):
         assert np.abs(g - r).max() <= 10.0, (k, np.abs(g - r).max())
 
 
+@pytest.mark.parametrize("n", LENGTHS)
 @pytest.mark.parametrize("system", ["NTSC", "PAL"])
-def test_block64_mfma_matches_oracle(torch_cuda, system):
-    rf, orf = _rf(system), _orf(system)
-    cap = synth.make_capture(system, N64, seed=8, snr_db=40.0)
+def test_block_mfma_matches_oracle(torch_cuda, n, system):
+    rf, orf = _rf(system, n), _orf(system, n)
+    cap = synth.make_capture(system, n, seed=8, snr_db=40.0)
     ref = orf.demodblock(data=cap.data.astype(np.float64), mtf_level=0, cut=True)
     try:
         rf.set_fft_variant(_lib.LD_FFT_MFMA_BF16X3)
@@ -76,15 +79,16 @@ def test_block64_mfma_matches_oracle(torch_cuda, system):
         rf.set_fft_variant(_lib.LD_FFT_FP32)
     for k in rf.video_fields:
         e = np.abs(_np(got["video"][k]) - ref["video"][k])
-        print(f"{system} 64k MFMA {k}: max {e.max():.2f} Hz, rms {np.sqrt(np.mean(e ** 2)):.3f}", flush=True)
+        print(f"{system} {n} MFMA {k}: max {e.max():.2f} Hz, rms {np.sqrt(np.mean(e ** 2)):.3f}", flush=True)
     _check(got["video"], ref["video"], rf.video_fields, tol=1e9, rel=1e-4)
 
 
+@pytest.mark.parametrize("n", LENGTHS)
 @pytest.mark.parametrize("system", ["NTSC", "PAL"])
-def test_stitched_blocks64_match_oracle(torch_cuda, system):
-    rf, orf = _rf(system), _orf(system)
+def test_stitched_blocks_match_oracle(torch_cuda, n, system):
+    rf, orf = _rf(system, n), _orf(system, n)
     S = rf.block_stride
-    cap = synth.make_capture(system, 2 * S + N64, seed=21, snr_db=40.0)
+    cap = synth.make_capture(system, 2 * S + n, seed=21, snr_db=40.0)
     got = rf.demod_blocks(cap.data, 0, 3)
     ref, start = O.demod_span(orf, cap.data.astype(np.float64), 0, 3 * S - 1)
     assert start == 0
@@ -96,13 +100,14 @@ def test_stitched_blocks64_match_oracle(torch_cuda, system):
         np.testing.assert_array_equal(_np(got["video"][k])[S:2 * S], _np(one["video"][k]))
 
 
-def test_batch64_equals_per_capture_decode(torch_cuda):
+@pytest.mark.parametrize("n", LENGTHS)
+def test_batch_equals_per_capture_decode(torch_cuda, n):
     import torch
     from lddecode_amd import batch as batchmod
     from lddecode_amd import capture
     plan = [("NTSC", 3_200_000), ("PAL", 3_600_000)]
     b = synth.make_batch(plan, "cuda", seed=12, ntemplates=1)
-    rfs = {"NTSC": _rf("NTSC"), "PAL": _rf("PAL")}
+    rfs = {"NTSC": _rf("NTSC", n), "PAL": _rf("PAL", n)}
     got = {}
 
     def on_wave(info):
@@ -118,7 +123,7 @@ def test_batch64_equals_per_capture_decode(torch_cuda):
                 row = f["out_offset"] // (W * lines)
                 got[(f["lane"], start)] = d["out_u16"][row * W * lines:(row + 1) * W * lines].cpu().numpy()
 
-    dec = batchmod.BatchDecoder(rfs, b.data, b.systems, b.offsets, b.lengths, wave_blocks=20, min_fields=1,
+    dec = batchmod.BatchDecoder(rfs, b.data, b.systems, b.offsets, b.lengths, wave_blocks=20 * 65536 // n, min_fields=1,
                                 graph=True, on_wave=on_wave)
     n = dec.run()
     torch.cuda.synchronize()
