@@ -8,8 +8,8 @@ mixed (each system with probability 1/2), int16 at 40 MSPS, resident in HBM
 with +-50 ns per-line jitter, its FM carrier phase-accumulated over the
 whole capture, with the analogue-audio carriers and 40 dB AWGN.  The batch
 is decoded in waves: every wave demodulates every capture's next window of
-32768-sample blocks in ONE launch with both systems' filter tables resident
-(replayed as a HIP graph together with the sync-pulse search), walks every
+blocks in ONE launch with both systems' filter tables resident (replayed as
+a HIP graph together with the sync-pulse search), walks every
 capture's fields on the host (the reference's field walk and line-0 logic,
 native code), and refines (FP64 hsync, burst / pilot), resamples to uint16
 4fsc fields and checks the walk, per system; two buffer sets overlap a
@@ -17,6 +17,11 @@ wave's demodulation with the previous wave's alignment and TBC.
 
 value = RF samples of the batch decoded per second (each capture sample once;
 blocks demodulated twice where windows overlap are work, not throughput).
+The block length (16384, 32768 or 65536 samples) and the transform (FP32
+passes or split-BF16 MFMA radix-64 stages) are the autotuner's choice
+(lddecode_amd.tune: the fastest demod launch on a small mixed batch) unless
+--blocklen / --fft fix them; the same batch is also decoded at the
+reference's default (32768, FP32) and with 65536-sample MFMA blocks.
 
 Also reported: the plain single-block path (one 32768-sample block per
 launch), single NTSC and PAL fields, the one-capture 64-field pipelined
@@ -56,9 +61,19 @@ def kernel_source_hash():
     return h.hexdigest()[:16]
 
 
-def decoders(torch):
+def decoders(torch, blocklen=32768, fft="fp32"):
+    from lddecode_amd import tune
     from lddecode_amd.rfdecode import RFDecode
-    return {s: RFDecode(system=s, decode_analog_audio=44100, device=0) for s in ("NTSC", "PAL")}
+    rfs = {s: RFDecode(system=s, blocklen=blocklen, decode_analog_audio=44100, device=0) for s in ("NTSC", "PAL")}
+    for rf in rfs.values():
+        rf.set_fft_variant(tune.VARIANTS[fft])
+    return rfs
+
+
+def wave_blocks_for(args, rfs):
+    """args.wave_blocks counts 32768-sample blocks (31712 usable samples);
+    other block lengths get the same samples per wave."""
+    return max(1, round(args.wave_blocks * 31712 / rfs["NTSC"].block_stride))
 
 
 def workload(torch, rfs, seed):
@@ -81,9 +96,10 @@ def batch_decoders(args, rfs, b):
     from lddecode_amd.batch import BatchDecoder, split_captures
     order = {}
     decs = []
+    wb = wave_blocks_for(args, rfs)
     for grp in split_captures(b.lengths, args.groups):
         decs.append(BatchDecoder(rfs, b.data, [b.systems[k] for k in grp], [b.offsets[k] for k in grp],
-                                 [b.lengths[k] for k in grp], wave_blocks=args.wave_blocks // args.groups,
+                                 [b.lengths[k] for k in grp], wave_blocks=wb // args.groups,
                                  graph=True, audio=True, demod_order=order))
     return decs
 
@@ -121,6 +137,7 @@ def run_c4(args, torch, rfs, b):
         for s, n in d.blocks_by_system.items():
             bys[s] = bys.get(s, 0) + n
     out = dict(ms_per_step=wall / args.steps * 1e3, fields=nf, waves=sum(d.waves for d in decs),
+               block_stride=decs[0].S, blocklen=decs[0].N,
                blocks=sum(d.blocks_total for d in decs), blocks_by_system=bys,
                relocated=sum(d.relocated for d in decs), redone_waves=sum(d.redone_waves for d in decs),
                samples=sum(d.samples_total for d in decs))
@@ -130,6 +147,9 @@ def run_c4(args, torch, rfs, b):
     # kernel's own rate without the other decoder's concurrent stage B
     iso = iso_demod(torch, decs[0])
     out.update(iso)
+    del decs
+    torch.cuda.synchronize()
+    torch.cuda.empty_cache()
     return out
 
 
@@ -342,7 +362,7 @@ def profile_step(args):
     __graft_entry__.load_package()
     import torch
     from lddecode_amd.batch import run_interleaved
-    rfs = decoders(torch)
+    rfs = decoders(torch, int(args.blocklen), args.fft)
     b = workload(torch, rfs, args.seed)
     decs = batch_decoders(args, rfs, b)
     for i in range(args.steps):
@@ -365,7 +385,9 @@ def _pmc_pass(counter, args, workdir):
     import subprocess
     out = pathlib.Path(workdir) / counter.lower()
     cmd = ["rocprofv3", "--pmc", counter, "-d", str(out), "-o", "run", "--", sys.executable, "-u",
-           str(pathlib.Path(__file__).resolve()), "--profile-step", "--steps", "1"]
+           str(pathlib.Path(__file__).resolve()), "--profile-step", "--steps", "1", "--blocklen", str(args.blocklen),
+           "--fft", args.fft, "--wave-blocks", str(args.wave_blocks), "--groups", str(args.groups),
+           "--seed", str(args.seed)]
     log(" ".join(cmd))
     env = dict(os.environ, TMPDIR="/tmp")
     subprocess.run(["timeout", "-k", "10", "400"] + cmd, check=True, cwd="/tmp", env=env)
@@ -405,17 +427,18 @@ def measure_traffic(args):
         kernels[k] = {"dispatches": len(fetch[k]), "fetch_bytes": round(f), "write_bytes": round(w),
                       "traffic_bytes": round(f + w)}
     res = {"source_hash": kernel_source_hash(), "seed": args.seed, "wave_blocks": args.wave_blocks,
-           "groups": args.groups,
+           "groups": args.groups, "blocklen": int(args.blocklen), "fft": args.fft,
            "copy_bytes_each_way": COPY_ELEMS * 4, "copy_fetch_kib": fc, "copy_write_kib": wc,
            "fetch_scale": fscale, "write_scale": wscale,
-           "command": f"python bench.py --measure-traffic --wave-blocks {args.wave_blocks} --groups {args.groups}", "kernels": kernels}
+           "command": f"python bench.py --measure-traffic --blocklen {args.blocklen} --fft {args.fft} "
+                      f"--wave-blocks {args.wave_blocks} --groups {args.groups}", "kernels": kernels}
     dest = pathlib.Path(args.traffic_out) if args.traffic_out else TRAFFIC_FILE
     dest.parent.mkdir(parents=True, exist_ok=True)
     dest.write_text(json.dumps(res, indent=1) + "\n")
     print(json.dumps(res, indent=1))
 
 
-def load_traffic(kernel, wave_blocks, seed, groups):
+def load_traffic(kernel, wave_blocks, seed, groups, blocklen, fft):
     """Measured HBM bytes per launch of `kernel` in the C4 pass, or None when
     the file is missing or was measured on other kernel sources / workload."""
     try:
@@ -424,8 +447,9 @@ def load_traffic(kernel, wave_blocks, seed, groups):
         return None, "no traffic file"
     if res.get("source_hash") != kernel_source_hash():
         return None, "traffic file measured on other kernel sources (stale): not used"
-    if res.get("wave_blocks") != wave_blocks or res.get("seed") != seed or res.get("groups") != groups:
-        return None, "traffic file measured on another workload: not used"
+    if (res.get("wave_blocks") != wave_blocks or res.get("seed") != seed or res.get("groups") != groups
+            or res.get("blocklen") != blocklen or res.get("fft") != fft):
+        return None, "traffic file measured on another workload or configuration: not used"
     k = res.get("kernels", {}).get(kernel)
     return (None, "kernel not in traffic file") if k is None else (k["traffic_bytes"], "measured")
 
@@ -440,11 +464,20 @@ def main():
     ap.add_argument("--wave-blocks", type=int, default=8192, help="blocks per wave of the batched decode")
     ap.add_argument("--groups", type=int, default=2, help="batch decoders taking turns on the host")
     ap.add_argument("--cpu-seconds", type=float, default=8.0, help="minimum CPU baseline time (0: skip)")
+    ap.add_argument("--blocklen", default="auto", choices=["auto", "16384", "32768", "65536"],
+                    help="block length of the batched decode (auto: the autotuner's choice)")
+    ap.add_argument("--fft", default="auto", choices=["auto", "fp32", "mfma", "mfma3"],
+                    help="transform variant of the batched decode (auto: the autotuner's choice)")
+    ap.add_argument("--legs", default="default,c5,c5x6",
+                    help="extra batched passes beside the headline: default (32768, fp32), c5 (65536, mfma: "
+                         "hi + lo terms), c5x6 (65536, mfma3: hi + mid + lo terms)")
     ap.add_argument("--profile-step", action="store_true", help=argparse.SUPPRESS)
     ap.add_argument("--measure-traffic", action="store_true",
                     help="run the two rocprofv3 counter passes and write " + str(TRAFFIC_FILE.name))
     ap.add_argument("--traffic-out", default=None, help="where --measure-traffic writes its JSON")
     args = ap.parse_args()
+    if (args.profile_step or args.measure_traffic) and "auto" in (args.blocklen, args.fft):
+        ap.error("--profile-step / --measure-traffic take a fixed --blocklen and --fft")
     if args.profile_step:
         return profile_step(args)
     if args.measure_traffic:
@@ -455,12 +488,41 @@ def main():
         sys.exit(2)
     __graft_entry__.load_package()
     import torch
-    rfs = decoders(torch)
+    from lddecode_amd import tune
+    tuned = None
+    if "auto" in (args.blocklen, args.fft):
+        lengths = tune.LENGTHS if args.blocklen == "auto" else (int(args.blocklen),)
+        variants = tuple(tune.VARIANTS) if args.fft == "auto" else (args.fft,)
+        tuned = tune.autotune(torch, 0, wave_samples=args.wave_blocks // args.groups * 31712, lengths=lengths,
+                              variants=variants, log=log)
+        log(f"autotuner: blocklen {tuned['blocklen']} {tuned['fft']} ({tuned['candidates']})")
+        blocklen, fft = tuned["blocklen"], tuned["fft"]
+    else:
+        blocklen, fft = int(args.blocklen), args.fft
+    rfs = decoders(torch)                       # 32768 / fp32: the plain paths and the default leg
     b = workload(torch, rfs, args.seed)
 
-    c4 = run_c4(args, torch, rfs, b)
+    head_rfs = rfs if (blocklen, fft) == (32768, "fp32") else decoders(torch, blocklen, fft)
+    c4 = run_c4(args, torch, head_rfs, b)
     value = c4["samples"] / (c4["ms_per_step"] / 1e3) / 1e6
-    log(f"C4: {c4['ms_per_step']:.1f} ms per pass, {value:.0f} MS/s, {c4['fields']} fields, {c4['waves']} waves")
+    log(f"C4 ({blocklen}, {fft}): {c4['ms_per_step']:.1f} ms per pass, {value:.0f} MS/s, {c4['fields']} fields, "
+        f"{c4['waves']} waves")
+    legs = {}
+    for leg in [x for x in args.legs.split(",") if x]:
+        cfg = {"default": (32768, "fp32"), "c5": (65536, "mfma"), "c5x6": (65536, "mfma3")}[leg]
+        if cfg == (blocklen, fft):
+            legs[leg] = "the headline configuration"
+            continue
+        lrfs = rfs if cfg == (32768, "fp32") else decoders(torch, *cfg)
+        r = run_c4(args, torch, lrfs, b)
+        v = r["samples"] / (r["ms_per_step"] / 1e3) / 1e6
+        log(f"C4 leg {leg} {cfg}: {r['ms_per_step']:.1f} ms per pass, {v:.0f} MS/s")
+        legs[leg] = {"blocklen": cfg[0], "fft": cfg[1], "value": round(v, 2), "unit": "Msamples/s",
+                     "ms_per_step": round(r["ms_per_step"], 3), "fields_per_step": r["fields"],
+                     "demod_launch_gbs": round(r["demod_bytes_avg"] / (r["demod_ms_avg"] / 1e3) / 1e9, 1),
+                     "demod_launch_ms_avg": round(r["demod_ms_avg"], 4),
+                     "isolated_demod_gbs": round(r["iso_bytes_avg"] / (r["iso_ms_avg"] / 1e3) / 1e9, 1)}
+        del lrfs
 
     # plain paths on slices of the batch's first NTSC and PAL captures
     k_ntsc = next(k for k, s in enumerate(b.systems) if s == "NTSC")
@@ -487,7 +549,10 @@ def main():
 
     achieved = c4["demod_bytes_avg"] / (c4["demod_ms_avg"] / 1e3) / 1e9
     iso_gbs = c4["iso_bytes_avg"] / (c4["iso_ms_avg"] / 1e3) / 1e9
-    traffic, traffic_note = load_traffic("ldgpu::k_demod_batch", args.wave_blocks, args.seed, args.groups)
+    kname = {16384: "ldgpu::k_demod_batch", 32768: "ldgpu::k_demod_batch", 65536: "ldgpu::k_demod_batch64"}[blocklen]
+    kname += "16" if blocklen == 16384 else ""
+    kname += {"fp32": "", "mfma": "_mfma", "mfma3": "_mfma3"}[fft]
+    traffic, traffic_note = load_traffic(kname, args.wave_blocks, args.seed, args.groups, blocklen, fft)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -521,19 +586,21 @@ def main():
             "workload": f"C4 batched: 64 captures of 1-30 s (log-uniform, seed {args.seed}), NTSC and PAL mixed "
                         f"({sum(s == 'PAL' for s in b.systems)} PAL), int16 at 40 MSPS, {c4['samples']} samples "
                         "resident in HBM; one step = one pass over the whole batch: per wave one mixed-system "
-                        "demod launch (32768-sample blocks, both systems' filter tables resident, HIP graph with "
-                        "the sync-pulse search), host field walk per capture, FP64 hsync + burst/pilot line "
-                        "positions, sinc-LUT TBC to uint16, per system",
+                        f"demod launch ({blocklen}-sample blocks, {fft} transform, both systems' filter tables "
+                        "resident, HIP graph with the sync-pulse search), host field walk per capture, FP64 hsync "
+                        "+ burst/pilot line positions, sinc-LUT TBC to uint16, per system",
             "captures": len(b.lengths),
             "samples_per_step": c4["samples"],
-            "blocklen": 32768,
+            "blocklen": blocklen,
+            "fft": fft,
+            "autotuner": tuned,
             "wave_blocks": args.wave_blocks,
             "decoders": args.groups,
         },
         "x_realtime": round(value / 40.0, 1),
         "batch": {"fields_per_step": c4["fields"], "waves": c4["waves"], "blocks_demodulated": c4["blocks"],
                   "blocks_by_system": c4["blocks_by_system"],
-                  "overlap_recompute": round(c4["blocks"] * 31712 / c4["samples"] - 1, 4),
+                  "overlap_recompute": round(c4["blocks"] * c4["block_stride"] / c4["samples"] - 1, 4),
                   "relocated_rounds": c4["relocated"], "redone_waves": c4["redone_waves"],
                   "demod_launches": c4["demod_launches"],
                   "demod_ms_per_pass": round(c4["demod_ms_total"], 2)},
@@ -549,6 +616,7 @@ def main():
             "note": "one block per launch (C1, demod only: the RFDecode.demodblock call shape) and one field read "
                     "per step (C2 / C3: demod, location, hsync, burst/pilot, TBC and audio), sequential",
         },
+        "batched_legs": legs,
         "one_capture_pipelined": {"value": round(pipe["msps"], 2), "unit": "Msamples/s",
                                   "ms_per_step": round(pipe["ms"], 3), "fields_per_step": pipe["fields"],
                                   "note": "64 NTSC fields of one capture per pass with audio stage 2 and 44.1 kHz "
@@ -560,7 +628,7 @@ def main():
                      "traffic_note": traffic_note,
                      "algorithmic_bytes": round(c4["demod_bytes_avg"]),
                      "kernel_ms": round(c4["demod_ms_avg"], 4),
-                     "kernel": "k_demod_batch: events around every wave's demod graph replay (block table upload "
+                     "kernel": f"{kname.split('::')[1]}: events around every wave's demod graph replay (block table upload "
                                "+ the launch) in the timed passes, beside the stage B kernels; algorithmic "
                                "22.25 B per usable NTSC sample, 26.25 PAL (input, video channels, rfhpf, audio "
                                "stage 1)",

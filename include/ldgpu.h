@@ -177,8 +177,12 @@ int ld_demod_blocks(ld_handle* h, const void* input, int input_dtype, int64_t in
  * LD_FFT_MFMA_BF16X3: the two radix-64 stages of 16384 = 64 x 64 x 4 as real
  * matrix products on the matrix cores with both operands split into bf16
  * hi + lo parts (hi*hi + hi*lo + lo*hi, FP32 accumulation); twiddles and the
- * radix-4 stage stay FP32.  About 2^-17 relative error per stage. */
-enum { LD_FFT_FP32 = 0, LD_FFT_MFMA_BF16X3 = 1 };
+ * radix-4 stage stay FP32.  About 2^-17 relative error per stage.
+ * LD_FFT_MFMA_BF16X6: as LD_FFT_MFMA_BF16X3 with three bf16 parts per
+ * operand (hi + mid + lo, exact for FP32 data) and the six products of
+ * order >= 2^-16: FP32-level error at twice the matrix-core work.
+ * (16384-sample blocks: 8192 = 64 x 64 x 2; 65536: each 16384-point half.) */
+enum { LD_FFT_FP32 = 0, LD_FFT_MFMA_BF16X3 = 1, LD_FFT_MFMA_BF16X6 = 2 };
 
 /* Select the transform variant of later ld_demod_blocks / ld_demod_batch
  * launches on this handle (int16 / float32 input without EFM output for

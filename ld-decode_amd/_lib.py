@@ -14,7 +14,7 @@ LIB_PATH = pathlib.Path(__file__).resolve().with_name("libldgpu.so")
 LD_OK, LD_EINVAL, LD_ERANGE, LD_EHIP, LD_ESTATE = 0, -1, -2, -3, -4
 LD_FILT_RFHPF, LD_FILT_RFVIDEO, LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT, LD_FILT_EFM = range(7)
 LD_IN_INT16, LD_IN_FLOAT32, LD_IN_PACKED10 = 0, 1, 2
-LD_FFT_FP32, LD_FFT_MFMA_BF16X3 = 0, 1
+LD_FFT_FP32, LD_FFT_MFMA_BF16X3, LD_FFT_MFMA_BF16X6 = 0, 1, 2
 ABI_VERSION = 1
 
 # every symbol include/ldgpu.h declares

@@ -558,9 +558,10 @@ __device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int t
 // s0: the block's first input sample; bo: its output slot (outputs at bo
 // times the strides); bw: its workspace slot.
 //
-// kMfma: the transforms' radix-64 stages run as split-BF16 matrix products on
-// the matrix cores (WgFft16kMfma) instead of FP32 radix-16 passes.
-template <class C, bool kPacked, bool kEfm, bool kMfma = false>
+// kMfma = 2 or 3: the transforms' radix-64 stages run as split-BF16 matrix
+// products of that many bf16 terms per operand on the matrix cores
+// (C::FwdMfma) instead of FP32 radix-16 passes; 0: FP32 passes.
+template <class C, bool kPacked, bool kEfm, int kMfma = 0>
 __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int bo, int bw) {
   __shared__ LdsT<C::M> s;
   __shared__ MfmaTw mtw;
@@ -741,7 +742,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
 
     // ---------------------------------------------------------- transform
     if constexpr (kMfma)
-      C::FwdMfma::run(v, s.re, s.im, twf, a.mfma_g, &mtw, tidv);
+      C::FwdMfma::template run<kMfma>(v, s.re, s.im, twf, a.mfma_g, &mtw, tidv);
     else
       C::Fwd::run(v, s.re, s.im, twf, tidv);
 
@@ -823,17 +824,20 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   __global__ __launch_bounds__(C::T) void k_demod_block##sfx(DemodArgs a) {                                 \
     demod_block<C, false, false>(LD_DEMOD_AT(a));                                                            \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block_packed##sfx(DemodArgs a) {                          \
+  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_packed(DemodArgs a) {                        \
     demod_block<C, true, false>(LD_DEMOD_AT(a));                                                             \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block_efm##sfx(DemodArgs a) {                             \
+  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_efm(DemodArgs a) {                           \
     demod_block<C, false, true>(LD_DEMOD_AT(a));                                                             \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block_packed_efm##sfx(DemodArgs a) {                      \
+  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_packed_efm(DemodArgs a) {                    \
     demod_block<C, true, true>(LD_DEMOD_AT(a));                                                              \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block_mfma##sfx(DemodArgs a) {                            \
-    demod_block<C, false, false, true>(LD_DEMOD_AT(a));                                                      \
+  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_mfma(DemodArgs a) {                          \
+    demod_block<C, false, false, 2>(LD_DEMOD_AT(a));                                                         \
+  }                                                                                                          \
+  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_mfma3(DemodArgs a) {                         \
+    demod_block<C, false, false, 3>(LD_DEMOD_AT(a));                                                         \
   }
 LD_DEMOD_KERNELS(Cfg32, )
 LD_DEMOD_KERNELS(Cfg16, 16)
@@ -854,7 +858,7 @@ struct DemodBatchArgs {
   int nslots[2];
 };
 static_assert(sizeof(DemodBatchArgs) <= 4096, "kernel arguments over 4 KiB");
-template <class C, bool kMfma>
+template <class C, int kMfma>
 __device__ __forceinline__ void demod_batch(const DemodBatchArgs& a) {
   const ld_batch_block d = a.blocks[blockIdx.x];
   if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - C::N || d.slot < 0 ||
@@ -862,14 +866,16 @@ __device__ __forceinline__ void demod_batch(const DemodBatchArgs& a) {
     return;
   demod_block<C, false, false, kMfma>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
 }
-__global__ __launch_bounds__(Cfg32::T) void k_demod_batch(DemodBatchArgs a) { demod_batch<Cfg32, false>(a); }
-__global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma(DemodBatchArgs a) { demod_batch<Cfg32, true>(a); }
-__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16(DemodBatchArgs a) { demod_batch<Cfg16, false>(a); }
-__global__ __launch_bounds__(Cfg16::T) void k_demod_batch_mfma16(DemodBatchArgs a) { demod_batch<Cfg16, true>(a); }
+__global__ __launch_bounds__(Cfg32::T) void k_demod_batch(DemodBatchArgs a) { demod_batch<Cfg32, 0>(a); }
+__global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma(DemodBatchArgs a) { demod_batch<Cfg32, 2>(a); }
+__global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma3(DemodBatchArgs a) { demod_batch<Cfg32, 3>(a); }
+__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16(DemodBatchArgs a) { demod_batch<Cfg16, 0>(a); }
+__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16_mfma(DemodBatchArgs a) { demod_batch<Cfg16, 2>(a); }
+__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16_mfma3(DemodBatchArgs a) { demod_batch<Cfg16, 3>(a); }
 
 // The demodulator's forward M-point transform alone (FP32 or split-BF16
 // MFMA), one workgroup per transform: in[b M .. + M) -> out[b M .. + M).
-template <bool kMfma>
+template <int kMfma>
 __device__ __forceinline__ void fft_only(const float2* in, float2* out, const DemodArgs& a) {
   __shared__ Lds s;
   __shared__ TwLds twl;
@@ -890,17 +896,20 @@ __device__ __forceinline__ void fft_only(const float2* in, float2* out, const De
 #pragma unroll
   for (int m = 0; m < kP; ++m) v[m] = bload2(ri, tid * 8u, m * kStrideB);
   if constexpr (kMfma)
-    WgFft16kMfma::run(v, s.re, s.im, twf, a.mfma_g, &mtw, tid);
+    WgFft16kMfma::template run<kMfma>(v, s.re, s.im, twf, a.mfma_g, &mtw, tid);
   else
     FwdM::run(v, s.re, s.im, twf, tid);
 #pragma unroll
   for (int m = 0; m < kP; ++m) bstore2(ro, v[m], tid * 8u, m * kStrideB);
 }
 __global__ __launch_bounds__(kT) void k_fft_fp32(const float2* in, float2* out, DemodArgs a) {
-  fft_only<false>(in, out, a);
+  fft_only<0>(in, out, a);
 }
 __global__ __launch_bounds__(kT) void k_fft_mfma(const float2* in, float2* out, DemodArgs a) {
-  fft_only<true>(in, out, a);
+  fft_only<2>(in, out, a);
+}
+__global__ __launch_bounds__(kT) void k_fft_mfma3(const float2* in, float2* out, DemodArgs a) {
+  fft_only<3>(in, out, a);
 }
 
 // =================================================== 65536-sample blocks
@@ -936,18 +945,18 @@ __device__ __forceinline__ float2 wm64(float2 wt) {
   return cmul(wt, make_float2(Tw<kM64, kT * M_, -1>::re, Tw<kM64, kT * M_, -1>::im));
 }
 
-template <bool kMfma>
+template <int kMfma>
 __device__ __forceinline__ void half_fft(float2* hv, Lds& s, const TwSrcLds& twf, const DemodArgs& a,
                                          const MfmaTw* mtw, int tid) {
   float2 (&h)[16] = *reinterpret_cast<float2(*)[16]>(hv);
   if constexpr (kMfma)
-    WgFft16kMfma::run(h, s.re, s.im, twf, a.mfma_g, mtw, tid);
+    WgFft16kMfma::template run<kMfma>(h, s.re, s.im, twf, a.mfma_g, mtw, tid);
   else
     FwdM::run(h, s.re, s.im, twf, tid);
 }
 
 // natural order in, pair order out (radix-2 decimation in frequency)
-template <bool kMfma>
+template <int kMfma>
 __device__ __forceinline__ void fft_dif64(float2 (&v)[32], Lds& s, const TwSrcLds& twf, const DemodArgs& a,
                                           const MfmaTw* mtw, float2 wt, int tid) {
   static_for<0, 16>([&](auto mc) {
@@ -961,7 +970,7 @@ __device__ __forceinline__ void fft_dif64(float2 (&v)[32], Lds& s, const TwSrcLd
 }
 
 // pair order in, natural order out (radix-2 decimation in time)
-template <bool kMfma>
+template <int kMfma>
 __device__ __forceinline__ void fft_dit64(float2 (&v)[32], Lds& s, const TwSrcLds& twf, const DemodArgs& a,
                                           const MfmaTw* mtw, float2 wt, int tid) {
   half_fft<kMfma>(v, s, twf, a, mtw, tid);
@@ -1136,7 +1145,7 @@ __device__ void audio_wave64(const DemodArgs& a, int c, const float2* slice, flo
   }
 }
 
-template <bool kMfma>
+template <int kMfma>
 __device__ __forceinline__ void demod_block64(const DemodArgs& a, int64_t s0, int bo, int bw) {
   __shared__ Lds s;
   __shared__ TwLds twl;
@@ -1348,12 +1357,15 @@ __device__ __forceinline__ void demod_block64(const DemodArgs& a, int64_t s0, in
 }
 
 __global__ __launch_bounds__(kT) void k_demod_block64(DemodArgs a) {
-  demod_block64<false>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
+  demod_block64<0>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
 }
 __global__ __launch_bounds__(kT) void k_demod_block64_mfma(DemodArgs a) {
-  demod_block64<true>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
+  demod_block64<2>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
 }
-template <bool kMfma>
+__global__ __launch_bounds__(kT) void k_demod_block64_mfma3(DemodArgs a) {
+  demod_block64<3>(a, a.first + a.step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x);
+}
+template <int kMfma>
 __device__ __forceinline__ void demod_batch64(const DemodBatchArgs& a) {
   const ld_batch_block d = a.blocks[blockIdx.x];
   if (d.sys < 0 || d.sys > 1 || d.first < 0 || d.first > a.input_len - kN64 || d.slot < 0 ||
@@ -1361,8 +1373,9 @@ __device__ __forceinline__ void demod_batch64(const DemodBatchArgs& a) {
     return;
   demod_block64<kMfma>(a.sys[d.sys], d.first, d.slot, (int)blockIdx.x);
 }
-__global__ __launch_bounds__(kT) void k_demod_batch64(DemodBatchArgs a) { demod_batch64<false>(a); }
-__global__ __launch_bounds__(kT) void k_demod_batch64_mfma(DemodBatchArgs a) { demod_batch64<true>(a); }
+__global__ __launch_bounds__(kT) void k_demod_batch64(DemodBatchArgs a) { demod_batch64<0>(a); }
+__global__ __launch_bounds__(kT) void k_demod_batch64_mfma(DemodBatchArgs a) { demod_batch64<2>(a); }
+__global__ __launch_bounds__(kT) void k_demod_batch64_mfma3(DemodBatchArgs a) { demod_batch64<3>(a); }
 
 // ------------------------------------------------------- audio stage 2
 // One blocklen segment of a field read's stage-1 audio: y = the filtered
@@ -1590,8 +1603,8 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   }
   hipError_t e = hipMalloc(&h->twiddle, sizeof(float2) * tw.size());
   if (e == hipSuccess) e = hipMemcpy(h->twiddle, tw.data(), sizeof(float2) * tw.size(), hipMemcpyHostToDevice);
-  // split-BF16 MFMA variant: the DFT-64 matrix [[F_re, -F_im]; [F_im, F_re]]
-  // (F = exp(-2 pi i r s / 64)) as hi / lo bf16 fragments in the lane layout
+  // split-BF16 MFMA variants: the DFT-64 matrix [[F_re, -F_im]; [F_im, F_re]]
+  // (F = exp(-2 pi i r s / 64)) as three bf16 terms (hi, mid, lo) in the lane layout
   // of v_mfma_f32_32x32x16_bf16, and the NS = 64 twiddle factor tables
   std::vector<uint32_t> gt((size_t)kMfmaTabEntries * 4);
   auto bf16 = [](float x) -> uint32_t {
@@ -1608,20 +1621,25 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   };
   for (int rt = 0; rt < 4; ++rt)
     for (int ks = 0; ks < 8; ++ks)
-      for (int hl = 0; hl < 2; ++hl)
+      for (int term = 0; term < kMfmaTerms; ++term)
         for (int l = 0; l < 64; ++l) {
-          uint32_t half[8];
+          uint32_t part[8];
           for (int el = 0; el < 8; ++el) {
             const int row = 32 * rt + (l & 31), col = 16 * ks + 8 * (l >> 5) + el;
             const int sidx = row & 63, r = col & 63;
             const double ang = 2.0 * M_PI * (double)((r * sidx) & 63) / 64.0;
             const double c = cos(ang), sn = sin(ang);
-            const double g = row < 64 ? (col < 64 ? c : sn) : (col < 64 ? -sn : c);
-            const uint32_t hi = bf16((float)g);
-            half[el] = hl == 0 ? hi : bf16((float)(g - bf2f(hi)));
+            double g = row < 64 ? (col < 64 ? c : sn) : (col < 64 ? -sn : c);
+            // term k: the bf16 rounding of what terms 0..k-1 leave of g
+            uint32_t t = bf16((float)g);
+            for (int k = 1; k <= term; ++k) {
+              g -= bf2f(t);
+              t = bf16((float)g);
+            }
+            part[el] = t;
           }
-          uint32_t* dst = gt.data() + ((size_t)((rt * 8 + ks) * 2 + hl) * 64 + l) * 4;
-          for (int w = 0; w < 4; ++w) dst[w] = half[2 * w] | (half[2 * w + 1] << 16);
+          uint32_t* dst = gt.data() + ((size_t)((rt * 8 + ks) * kMfmaTerms + term) * 64 + l) * 4;
+          for (int w = 0; w < 4; ++w) dst[w] = part[2 * w] | (part[2 * w + 1] << 16);
         }
   MfmaTw mt;
   for (int q = 0; q < 64; ++q) {
@@ -2026,24 +2044,28 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
       set_error("ld_demod_blocks: 65536-sample blocks take int16 / float32 input without EFM output");
       return LD_EINVAL;
     }
-    hipLaunchKernelGGL(h->fft_variant == LD_FFT_MFMA_BF16X3 ? k_demod_block64_mfma : k_demod_block64, dim3(nblocks),
-                       dim3(kT), 0, st, a);
+    hipLaunchKernelGGL(h->fft_variant == LD_FFT_MFMA_BF16X3   ? k_demod_block64_mfma
+                       : h->fft_variant == LD_FFT_MFMA_BF16X6 ? k_demod_block64_mfma3
+                                                              : k_demod_block64,
+                       dim3(nblocks), dim3(kT), 0, st, a);
     LD_HIP(hipGetLastError());
     return LD_OK;
   }
-  if (h->fft_variant == LD_FFT_MFMA_BF16X3 && (dtype == LD_IN_PACKED10 || a.efm)) {
+  if (h->fft_variant != LD_FFT_FP32 && (dtype == LD_IN_PACKED10 || a.efm)) {
     set_error("ld_demod_blocks: the split-BF16 MFMA variant takes int16 / float32 input without EFM output");
     return LD_EINVAL;
   }
   const bool b16 = h->blocklen == Cfg16::N;
   void (*kern)(DemodArgs);
   if (h->fft_variant == LD_FFT_MFMA_BF16X3)
-    kern = b16 ? k_demod_block_mfma16 : k_demod_block_mfma;
+    kern = b16 ? k_demod_block16_mfma : k_demod_block_mfma;
+  else if (h->fft_variant == LD_FFT_MFMA_BF16X6)
+    kern = b16 ? k_demod_block16_mfma3 : k_demod_block_mfma3;
   else if (dtype == LD_IN_PACKED10)
-    kern = a.efm ? (b16 ? k_demod_block_packed_efm16 : k_demod_block_packed_efm)
-                 : (b16 ? k_demod_block_packed16 : k_demod_block_packed);
+    kern = a.efm ? (b16 ? k_demod_block16_packed_efm : k_demod_block_packed_efm)
+                 : (b16 ? k_demod_block16_packed : k_demod_block_packed);
   else
-    kern = a.efm ? (b16 ? k_demod_block_efm16 : k_demod_block_efm) : (b16 ? k_demod_block16 : k_demod_block);
+    kern = a.efm ? (b16 ? k_demod_block16_efm : k_demod_block_efm) : (b16 ? k_demod_block16 : k_demod_block);
   hipLaunchKernelGGL(kern, dim3(nblocks), dim3(b16 ? Cfg16::T : kT), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
@@ -2096,19 +2118,32 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t 
   b.input_len = input_len;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h0->device));
-  const bool mf = h0->fft_variant == LD_FFT_MFMA_BF16X3;
-  if (h0->blocklen == kN64)
-    hipLaunchKernelGGL(mf ? k_demod_batch64_mfma : k_demod_batch64, dim3(nblocks), dim3(kT), 0, st, b);
-  else if (h0->blocklen == Cfg16::N)
-    hipLaunchKernelGGL(mf ? k_demod_batch_mfma16 : k_demod_batch16, dim3(nblocks), dim3(Cfg16::T), 0, st, b);
-  else
-    hipLaunchKernelGGL(mf ? k_demod_batch_mfma : k_demod_batch, dim3(nblocks), dim3(kT), 0, st, b);
+  for (int k = 0; k < 2; ++k)
+    if (sys[k] && sys[k]->fft_variant != h0->fft_variant) {
+      set_error("ld_demod_batch: the system handles select different transform variants");
+      return LD_EINVAL;
+    }
+  const int v = h0->fft_variant;
+  void (*kern)(DemodBatchArgs);
+  int threads = kT;
+  if (h0->blocklen == kN64) {
+    kern = v == LD_FFT_MFMA_BF16X3 ? k_demod_batch64_mfma : v == LD_FFT_MFMA_BF16X6 ? k_demod_batch64_mfma3
+                                                                                     : k_demod_batch64;
+  } else if (h0->blocklen == Cfg16::N) {
+    kern = v == LD_FFT_MFMA_BF16X3 ? k_demod_batch16_mfma : v == LD_FFT_MFMA_BF16X6 ? k_demod_batch16_mfma3
+                                                                                     : k_demod_batch16;
+    threads = Cfg16::T;
+  } else {
+    kern = v == LD_FFT_MFMA_BF16X3 ? k_demod_batch_mfma : v == LD_FFT_MFMA_BF16X6 ? k_demod_batch_mfma3
+                                                                                   : k_demod_batch;
+  }
+  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(threads), 0, st, b);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }
 
 int ld_set_fft_variant(ld_handle* h, int variant) {
-  if (!h || (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3)) {
+  if (!h || (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3 && variant != LD_FFT_MFMA_BF16X6)) {
     set_error("ld_set_fft_variant: bad handle or variant %d", variant);
     return LD_EINVAL;
   }
@@ -2117,7 +2152,8 @@ int ld_set_fft_variant(ld_handle* h, int variant) {
 }
 
 int ld_fft_forward(ld_handle* h, const void* in, void* out, int ntransforms, int variant, void* stream) {
-  if (!h || !in || !out || ntransforms <= 0 || (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3)) {
+  if (!h || !in || !out || ntransforms <= 0 ||
+      (variant != LD_FFT_FP32 && variant != LD_FFT_MFMA_BF16X3 && variant != LD_FFT_MFMA_BF16X6)) {
     set_error("ld_fft_forward: bad arguments");
     return LD_EINVAL;
   }
@@ -2131,7 +2167,10 @@ int ld_fft_forward(ld_handle* h, const void* in, void* out, int ntransforms, int
   a.mfma_tw = (const MfmaTw*)h->mfma_tw;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
-  hipLaunchKernelGGL(variant == LD_FFT_MFMA_BF16X3 ? k_fft_mfma : k_fft_fp32, dim3(ntransforms), dim3(kT), 0, st,
+  hipLaunchKernelGGL(variant == LD_FFT_MFMA_BF16X3   ? k_fft_mfma
+                     : variant == LD_FFT_MFMA_BF16X6 ? k_fft_mfma3
+                                                     : k_fft_fp32,
+                     dim3(ntransforms), dim3(kT), 0, st,
                      (const float2*)in, (float2*)out, a);
   LD_HIP(hipGetLastError());
   return LD_OK;

@@ -422,16 +422,22 @@ struct WgFft<1024, DIR, SYNC> {
 // by W_{64 NS}^{r (j mod NS)} in FP32), the DFT-64 is
 //   [Y_re; Y_im] (128 x C) = [[F_re, -F_im]; [F_im, F_re]] (128 x 128) [U_re; U_im]
 // with F[s][r] = exp(-2 pi i r s / 64).  Both operands are split into
-// bf16 hi + lo parts (x = hi + lo to about 2^-17 relative) and the product
-// is G_hi U_hi + G_hi U_lo + G_lo U_hi, accumulated in FP32 by
-// v_mfma_f32_32x32x16_bf16 (the G_lo U_lo term, about 2^-18 relative, is
-// dropped).  The G fragments come precomputed in the instruction's lane
-// layout from a global table (L1/L2 resident, 64 KiB).
+// KT bf16 terms and the product summed over the term pairs that matter,
+// accumulated in FP32 by v_mfma_f32_32x32x16_bf16:
+//   KT = 2: x = hi + lo (to about 2^-17 relative); G_hi U_hi + G_hi U_lo +
+//           G_lo U_hi (the lo*lo term, about 2^-18, dropped): 3 products;
+//   KT = 3: x = hi + mid + lo (exact for an FP32 U: 3 x 8 significand
+//           bits); the six products of order >= 2^-16 (hi*hi, hi*mid,
+//           mid*hi, hi*lo, lo*hi, mid*mid; the dropped mid*lo, lo*mid are
+//           about 2^-24): FP32-level error.
+// The G fragments come precomputed in the instruction's lane layout from a
+// global table (L1/L2 resident, 96 KiB: three terms).
 //
 // Lane layout of v_mfma_f32_32x32x16_bf16 (checked on gfx950): lane l holds
 // A[l % 32][8 (l / 32) + e] and B[8 (l / 32) + e][l % 32] for e < 8, and the
 // result D[8 (q / 4) + 4 (l / 32) + q % 4][l % 32] in accumulator q < 16.
 typedef __attribute__((ext_vector_type(8))) __bf16 bf16x8_t;
+typedef __attribute__((ext_vector_type(2))) __bf16 bf16x2_t;
 typedef __attribute__((ext_vector_type(16))) float f32x16_t;
 typedef __attribute__((ext_vector_type(4))) unsigned int u32x4_t;
 
@@ -441,15 +447,36 @@ __device__ __forceinline__ uint32_t bf16_rne(float x) {
   u += 0x7fffu + ((u >> 16) & 1u);
   return u >> 16;
 }
-// x = hi + lo, both bf16 (bits in the low halves)
-__device__ __forceinline__ void bf16_split(float x, uint32_t& hi, uint32_t& lo) {
-  hi = bf16_rne(x);
-  lo = bf16_rne(x - __uint_as_float(hi << 16));
+// two floats -> packed bf16 pair (a in the low half), round to nearest
+// even: one v_cvt_pk_bf16_f32
+__device__ __forceinline__ uint32_t bf16_pack(float a, float b) {
+  const bf16x2_t h = {(__bf16)a, (__bf16)b};
+  return __builtin_bit_cast(uint32_t, h);
+}
+// the packed pair's values as floats
+__device__ __forceinline__ float bf16_lo_f(uint32_t p) { return __uint_as_float(p << 16); }
+__device__ __forceinline__ float bf16_hi_f(uint32_t p) { return __uint_as_float(p & 0xffff0000u); }
+// (a, b) = sum of KT packed bf16 pairs t[0..KT), each the rounded residual
+// of the terms before it
+template <int KT>
+__device__ __forceinline__ void bf16_split_pair(float a, float b, uint32_t (&t)[KT]) {
+  t[0] = bf16_pack(a, b);
+  if constexpr (KT > 1) {
+    a -= bf16_lo_f(t[0]);
+    b -= bf16_hi_f(t[0]);
+    t[1] = bf16_pack(a, b);
+  }
+  if constexpr (KT > 2) {
+    a -= bf16_lo_f(t[1]);
+    b -= bf16_hi_f(t[1]);
+    t[2] = bf16_pack(a, b);
+  }
 }
 
-// G fragment table layout: [row tile 0..3][k step 0..7][hi, lo][lane 0..63]
+// G fragment table layout: [row tile 0..3][k step 0..7][term 0..2][lane 0..63]
 // of 8 bf16 (one u32x4 per lane).
-constexpr int kMfmaTabEntries = 4 * 8 * 2 * 64;
+constexpr int kMfmaTerms = 3;
+constexpr int kMfmaTabEntries = 4 * 8 * kMfmaTerms * 64;
 
 // W_{64 NS}^q for q < 64 NS as the product of a coarse and a fine table:
 // q = 64 qh + ql, W^q = W_{NS}^{qh} W_{64 NS}^{ql} (NS = 64 here: both 64).
@@ -458,13 +485,37 @@ struct MfmaTw {
   float2 fine[64];    // exp(-2 pi i ql / (64 NS))
 };
 
-template <int M, int T, int NS>
+// acc += G[rows of tile] x U over one k step pair, for KT-term operands:
+// a_re / a_im: the G fragments of the U_re and U_im rows, b_re / b_im: U's
+template <int KT>
+__device__ __forceinline__ f32x16_t mfma_terms(f32x16_t acc, const bf16x8_t (&a_re)[KT], const bf16x8_t (&a_im)[KT],
+                                               const bf16x8_t (&b_re)[KT], const bf16x8_t (&b_im)[KT]) {
+  // smallest terms first
+  if constexpr (KT == 3) {
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_re[1], b_re[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_im[1], b_im[1], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_re[0], b_re[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_im[0], b_im[2], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_re[2], b_re[0], acc, 0, 0, 0);
+    acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_im[2], b_im[0], acc, 0, 0, 0);
+  }
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_re[0], b_re[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_im[0], b_im[1], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_re[1], b_re[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_im[1], b_im[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_re[0], b_re[0], acc, 0, 0, 0);
+  acc = __builtin_amdgcn_mfma_f32_32x32x16_bf16(a_im[0], b_im[0], acc, 0, 0, 0);
+  return acc;
+}
+
+template <int M, int T, int NS, int KT>
 __device__ __forceinline__ void mfma64_pass(float* __restrict__ lre, float* __restrict__ lim,
                                             const u32x4_t* __restrict__ gtab, const MfmaTw* tw, int tid) {
   constexpr int C = M / 64;           // columns
   constexpr int NW = T / 64;          // wavefronts
   constexpr int NCT = C / 32;         // column tiles of 32
   static_assert(NW % 4 == 0 && NCT % (NW / 4) == 0, "tiles must divide over the wavefronts");
+  static_assert(KT == 2 || KT == 3, "two or three bf16 terms");
   constexpr int CTW = NCT / (NW / 4); // column tiles per wavefront
   static_assert(NS == 1 || NS == 64, "twiddle tables cover NS = 1 and 64");
   const int w = tid >> 6, l = tid & 63;
@@ -476,47 +527,50 @@ __device__ __forceinline__ void mfma64_pass(float* __restrict__ lre, float* __re
   for (int c = 0; c < CTW; ++c)
 #pragma unroll
     for (int q = 0; q < 16; ++q) acc[c][q] = 0.0f;
-  const u32x4_t* ga = gtab + (size_t)rt * (8 * 2 * 64) + l;
+  const u32x4_t* ga = gtab + (size_t)rt * (8 * kMfmaTerms * 64) + l;
 #pragma unroll 1
   for (int ks = 0; ks < 4; ++ks) {
     // G fragments of k steps ks (U_re rows) and ks + 4 (U_im rows)
-    const bf16x8_t are_h = __builtin_bit_cast(bf16x8_t, ga[(ks * 2 + 0) * 64]);
-    const bf16x8_t are_l = __builtin_bit_cast(bf16x8_t, ga[(ks * 2 + 1) * 64]);
-    const bf16x8_t aim_h = __builtin_bit_cast(bf16x8_t, ga[((ks + 4) * 2 + 0) * 64]);
-    const bf16x8_t aim_l = __builtin_bit_cast(bf16x8_t, ga[((ks + 4) * 2 + 1) * 64]);
+    bf16x8_t are[KT], aim[KT];
+#pragma unroll
+    for (int k = 0; k < KT; ++k) {
+      are[k] = __builtin_bit_cast(bf16x8_t, ga[(ks * kMfmaTerms + k) * 64]);
+      aim[k] = __builtin_bit_cast(bf16x8_t, ga[((ks + 4) * kMfmaTerms + k) * 64]);
+    }
     const int r0 = 16 * ks + 8 * half;
 #pragma unroll
     for (int c = 0; c < CTW; ++c) {
       const int j = 32 * (cg * CTW + c) + col;
-      u32x4_t bre_h, bre_l, bim_h, bim_l;
+      u32x4_t bre[KT], bim[KT];
 #pragma unroll
       for (int e2 = 0; e2 < 4; ++e2) {
-        uint32_t rh[2], rl[2], ih[2], il[2];
+        float2 u[2];
 #pragma unroll
         for (int t = 0; t < 2; ++t) {
           const int r = r0 + 2 * e2 + t;
           const int idx = pidx(j + r * C);
-          float2 u = make_float2(lre[idx], lim[idx]);
+          u[t] = make_float2(lre[idx], lim[idx]);
           if constexpr (NS > 1) {
             const int q = (r * (j & (NS - 1))) & (64 * NS - 1);
-            u = cmul(u, cmul(tw->coarse[q >> 6], tw->fine[q & 63]));
+            u[t] = cmul(u[t], cmul(tw->coarse[q >> 6], tw->fine[q & 63]));
           }
-          bf16_split(u.x, rh[t], rl[t]);
-          bf16_split(u.y, ih[t], il[t]);
         }
-        bre_h[e2] = rh[0] | (rh[1] << 16);
-        bre_l[e2] = rl[0] | (rl[1] << 16);
-        bim_h[e2] = ih[0] | (ih[1] << 16);
-        bim_l[e2] = il[0] | (il[1] << 16);
+        uint32_t pr[KT], pi[KT];
+        bf16_split_pair<KT>(u[0].x, u[1].x, pr);
+        bf16_split_pair<KT>(u[0].y, u[1].y, pi);
+#pragma unroll
+        for (int k = 0; k < KT; ++k) {
+          bre[k][e2] = pr[k];
+          bim[k][e2] = pi[k];
+        }
       }
-      const bf16x8_t brh = __builtin_bit_cast(bf16x8_t, bre_h), brl = __builtin_bit_cast(bf16x8_t, bre_l);
-      const bf16x8_t bih = __builtin_bit_cast(bf16x8_t, bim_h), bil = __builtin_bit_cast(bf16x8_t, bim_l);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(are_h, brh, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(are_h, brl, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(are_l, brh, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aim_h, bih, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aim_h, bil, acc[c], 0, 0, 0);
-      acc[c] = __builtin_amdgcn_mfma_f32_32x32x16_bf16(aim_l, bih, acc[c], 0, 0, 0);
+      bf16x8_t br[KT], bi[KT];
+#pragma unroll
+      for (int k = 0; k < KT; ++k) {
+        br[k] = __builtin_bit_cast(bf16x8_t, bre[k]);
+        bi[k] = __builtin_bit_cast(bf16x8_t, bim[k]);
+      }
+      acc[c] = mfma_terms<KT>(acc[c], are, aim, br, bi);
     }
   }
   __syncthreads();  // every input read before any output lands on it
@@ -539,7 +593,7 @@ __device__ __forceinline__ void mfma64_pass(float* __restrict__ lre, float* __re
 // (split BF16), then radix 2 in FP32.
 struct WgFft8kMfma {
   static constexpr int T = 512;
-  template <class TWF>
+  template <int KT, class TWF>
   static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
                                              const u32x4_t* gtab, const MfmaTw* tw, int tid = threadIdx.x) {
     __syncthreads();
@@ -549,8 +603,8 @@ struct WgFft8kMfma {
       own.template st<POff<512 * m>::value>(v[m]);
     });
     __syncthreads();
-    mfma64_pass<8192, 512, 1>(lre, lim, gtab, tw, tid);
-    mfma64_pass<8192, 512, 64>(lre, lim, gtab, tw, tid);
+    mfma64_pass<8192, 512, 1, KT>(lre, lim, gtab, tw, tid);
+    mfma64_pass<8192, 512, 64, KT>(lre, lim, gtab, tw, tid);
     stockham_pass<8192, 512, 2, 4096, -1, true, false, kSyncBlock>(v, lre, lim, twf, tid);
   }
 };
@@ -560,7 +614,7 @@ struct WgFft8kMfma {
 // v[m] = X[tid + T m].  TWF serves the last pass (NS = 4096).
 struct WgFft16kMfma {
   static constexpr int T = 1024;
-  template <class TWF>
+  template <int KT, class TWF>
   static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
                                              const u32x4_t* gtab, const MfmaTw* tw, int tid = threadIdx.x) {
     __syncthreads();  // LDS may still be read by a previous stage
@@ -570,8 +624,8 @@ struct WgFft16kMfma {
       own.template st<POff<1024 * m>::value>(v[m]);
     });
     __syncthreads();
-    mfma64_pass<16384, 1024, 1>(lre, lim, gtab, tw, tid);
-    mfma64_pass<16384, 1024, 64>(lre, lim, gtab, tw, tid);
+    mfma64_pass<16384, 1024, 1, KT>(lre, lim, gtab, tw, tid);
+    mfma64_pass<16384, 1024, 64, KT>(lre, lim, gtab, tw, tid);
     stockham_pass<16384, 1024, 4, 4096, -1, true, false, kSyncBlock>(v, lre, lim, twf, tid);
   }
 };

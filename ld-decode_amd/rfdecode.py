@@ -34,6 +34,8 @@ def _torch():
 
 
 def _device(torch, index):
+    if isinstance(index, torch.device):
+        return index
     return torch.device("cuda", torch.cuda.current_device() if index is None else index)
 
 
@@ -114,6 +116,7 @@ class RFDecode:
         h = ctypes.c_void_p()
         _lib.check(self.lib.ld_create(self.device.index or 0, blocklen, ctypes.byref(h)))
         self.handle = h
+        self.fft_variant = _lib.LD_FFT_FP32
         if self.system == "PAL" and self.PAL_V4300D_NotchFilter:
             # the spur window of core.py:785-788, applied per block in the kernel
             lo, hi = int(blocklen * (8.42 / self.freq)), int(1 + (blocklen * (8.6 / self.freq)))
@@ -127,8 +130,9 @@ class RFDecode:
 
     def set_fft_variant(self, variant):
         """The transform variant of later demodulations: _lib.LD_FFT_FP32
-        (default) or _lib.LD_FFT_MFMA_BF16X3 (radix-64 stages as split-BF16
-        matrix products on the matrix cores)."""
+        (default), _lib.LD_FFT_MFMA_BF16X3 (radix-64 stages as split-BF16
+        matrix products on the matrix cores, hi + lo terms) or
+        _lib.LD_FFT_MFMA_BF16X6 (hi + mid + lo terms, FP32-level error)."""
         _lib.check(self.lib.ld_set_fft_variant(self.handle, int(variant)))
         self.fft_variant = int(variant)
 

@@ -106,7 +106,8 @@ class LocParams:
                    vsync_ire=dp["vsync_ire"], hsyncPulseUS=sp["hsyncPulseUS"], eqPulseUS=sp["eqPulseUS"],
                    vsyncPulseUS=sp["vsyncPulseUS"], numPulses=sp["numPulses"], firstFieldH=tuple(sp["firstFieldH"]),
                    field_lines=tuple(sp["field_lines"]), frame_lines=sp["frame_lines"],
-                   line_period=sp["line_period"], blocklen=getattr(rf, "blocklen", 32768))
+                   line_period=sp["line_period"], blocklen=getattr(rf, "blocklen", 32768),
+                   block_stride=getattr(rf, "blocklen", 32768) - 1024 - 32)
 
     @property
     def freq_hz_half(self):

@@ -66,20 +66,21 @@ def test_block_matches_oracle(torch_cuda, n, system, cut):
         assert np.abs(g - r).max() <= 10.0, (k, np.abs(g - r).max())
 
 
+@pytest.mark.parametrize("variant", [_lib.LD_FFT_MFMA_BF16X3, _lib.LD_FFT_MFMA_BF16X6])
 @pytest.mark.parametrize("n", LENGTHS)
 @pytest.mark.parametrize("system", ["NTSC", "PAL"])
-def test_block_mfma_matches_oracle(torch_cuda, n, system):
+def test_block_mfma_matches_oracle(torch_cuda, n, system, variant):
     rf, orf = _rf(system, n), _orf(system, n)
     cap = synth.make_capture(system, n, seed=8, snr_db=40.0)
     ref = orf.demodblock(data=cap.data.astype(np.float64), mtf_level=0, cut=True)
     try:
-        rf.set_fft_variant(_lib.LD_FFT_MFMA_BF16X3)
+        rf.set_fft_variant(variant)
         got = rf.demodblock(data=cap.data, mtf_level=0, cut=True)
     finally:
         rf.set_fft_variant(_lib.LD_FFT_FP32)
     for k in rf.video_fields:
         e = np.abs(_np(got["video"][k]) - ref["video"][k])
-        print(f"{system} {n} MFMA {k}: max {e.max():.2f} Hz, rms {np.sqrt(np.mean(e ** 2)):.3f}", flush=True)
+        print(f"{system} {n} MFMA variant {variant} {k}: max {e.max():.2f} Hz, rms {np.sqrt(np.mean(e ** 2)):.3f}", flush=True)
     _check(got["video"], ref["video"], rf.video_fields, tol=1e9, rel=1e-4)
 
 

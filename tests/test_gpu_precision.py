@@ -1,0 +1,165 @@
+"""Precision of every output against the FP64 oracle, per noise level, block
+length and transform variant (FP32 passes / split-BF16 MFMA radix-64
+stages): synthetic NTSC and PAL captures of about five fields with AWGN at
+30, 36, 40 and 46 dB SNR.
+
+Demodulation: every video channel (demod, demod_raw, demod_05, burst,
+pilot), rfhpf and both audio stage-1 channels of the whole stitched capture
+(rf.demod_blocks) against the oracle's stitched blocks on the same block
+partition (oracle.dsp.demod_span): max and RMS |error| in Hz, and the max
+error relative to max(|value|, 1 MHz), which must stay <= 1e-4 for the
+video channels.
+
+TBC: the GPU's whole field decode (CaptureDecoder, locate=True: pulse
+search, field walk, FP64 hsync refine, burst / pilot alignment, sinc-LUT
+resampler to uint16) against the oracle's own decode of its FP64 demod (its
+field walk, hsync refine, burst / pilot alignment and resampler): the
+histogram of |code difference| over every pixel of the fields both find,
+>= 99.9% of which must be within 1 LSB (FP32 and the three-term MFMA
+variant; >= 99% for the two-term one, whose 2^-17 transform error moves
+the line positions about ten times as far).  The few pixels off by many
+LSB are the resampler's own discontinuity: it casts each absolute source
+coordinate (up to about 1e6 samples into the field buffer) to float32
+(utils.py:160), a grid of 1/32 to 1/16 sample there, so a position change
+of any size can move a pixel by that step, which on a sync or
+colour-burst edge is hundreds of codes.
+
+With LDGPU_PRECISION_OUT set, the figures are written there as JSON.
+"""
+
+import json
+import os
+
+import numpy as np
+import pytest
+
+from lddecode_amd import _lib, capture, synth
+from oracle import ctbc
+from oracle import dsp as O
+from oracle import lines as L
+from oracle import linelocs as LL
+from oracle import tbc as T
+from oracle.params import out_scale
+
+pytestmark = pytest.mark.gpu
+
+SNRS = (30.0, 36.0, 40.0, 46.0)
+VARIANTS = {"fp32": _lib.LD_FFT_FP32, "mfma": _lib.LD_FFT_MFMA_BF16X3, "mfma3": _lib.LD_FFT_MFMA_BF16X6}
+CASES = [(n, v, s, snr) for n in (32768, 65536) for v in VARIANTS for s in ("NTSC", "PAL") for snr in SNRS]
+CASES += [(16384, v, s, 40.0) for v in VARIANTS for s in ("NTSC", "PAL")]
+FIELD_SAMPLES = {"NTSC": 667_334, "PAL": 800_000}
+RESULTS = []
+_RF, _ORF = {}, {}
+
+
+@pytest.fixture(scope="module", autouse=True)
+def _write_results():
+    yield
+    path = os.environ.get("LDGPU_PRECISION_OUT")
+    if path and RESULTS:
+        with open(path, "w") as f:
+            json.dump({"cases": RESULTS}, f, indent=1)
+
+
+def _rf(system, n, variant):
+    if (system, n) not in _RF:
+        from lddecode_amd.rfdecode import RFDecode
+        _RF[system, n] = RFDecode(system=system, blocklen=n, decode_analog_audio=44100, device=0)
+    rf = _RF[system, n]
+    rf.set_fft_variant(VARIANTS[variant])
+    return rf
+
+
+def _orf(system, n):
+    if (system, n) not in _ORF:
+        _ORF[system, n] = O.OracleRF(system, blocklen=n, decode_analog_audio=True)
+    return _ORF[system, n]
+
+
+def _oracle_fields(orf, ref, system, nfields):
+    """The oracle's field decode of its stitched demod: {buffer origin: uint16 field}."""
+    lut = T.build_kaiser_lut()
+    d05 = ref["demod_05"].astype(np.float32)
+    dm = ref["demod"].astype(np.float32)
+    aux = ref["demod_burst" if system == "NTSC" else "demod_pilot"].astype(np.float32)
+    p = LL.LocParams.from_rf(orf)
+    lp = L.LineRefineParams(orf)
+    sp, dp = orf.SysParams, orf.DecoderParams
+    W, lines = sp["outlinelen"], sp["frame_lines"] // 2 + 1
+    state = {"phase": 0.0}
+    outs = {}
+
+    def finish(k, res, org):
+        bad = L.compute_deriv_error(res.linelocs1, res.linebad)
+        ll2, bad = T.refine_linelocs_hsync(d05[org:], res.linelocs1, bad, system, 40.0, orf.iretohz,
+                                           dp["vsync_ire"])
+        bad = L.compute_deriv_error(ll2, bad)
+        if system == "NTSC":
+            ll4, _, _, ph = L.ntsc_field_lines(lp, dm[org:], aux[org:], ll2, bad, res.linecount, res.isFirstField,
+                                               state["phase"])
+            state["phase"] = ph
+            lo = 0
+        else:
+            ll4, _ = L.pal_field_lines(lp, aux[org:], ll2, bad)
+            lo = 2 if res.isFirstField else 3
+        locs, wow = T.computewow_scaled(ll4, orf.linelen, W, lines, lo)
+        ds = ctbc.scale_field(dm[org:], locs, wow, lut, lo, W, lines * W)
+        outs[org] = T.hz_to_output_array(ds.astype(np.float32), dp["ire0"], dp["hz_ire"], sp["outputZero"],
+                                         dp["vsync_ire"], out_scale(system, dp["vsync_ire"]))
+        return ll4
+
+    LL.walk_fields(p, d05, nfields, 0, finish, demod=dm)
+    return outs
+
+
+def _np(t):
+    return t.detach().cpu().numpy().astype(np.float64)
+
+
+@pytest.mark.parametrize("n,variant,system,snr", CASES)
+def test_precision_vs_oracle(torch_cuda, n, variant, system, snr):
+    torch = torch_cuda
+    rf, orf = _rf(system, n, variant), _orf(system, n)
+    cap = synth.make_capture(system, int(5.3 * FIELD_SAMPLES[system]), seed=int(100 + snr), snr_db=snr)
+    S = rf.block_stride
+    nb = (len(cap.data) - n) // S + 1
+    print(f"case blocklen {n} {variant} {system} {snr} dB: {nb} blocks", flush=True)
+    ref, start = O.demod_span(orf, cap.data.astype(np.float64), 0, nb * S - 1)
+    assert start == 0
+    got = rf.demod_blocks(cap.data, 0, nb, cut=True, audio=True)
+    torch.cuda.synchronize()
+    rec = {"blocklen": n, "fft": variant, "system": system, "snr_db": snr, "blocks": nb, "channels": {}}
+    worst = {}
+    chans = [(k, got["video"][k]) for k in rf.video_fields] + [("rfhpf", got["rfhpf"])] + \
+        [(k, got["audio"][k]) for k in ("audio_left", "audio_right")]
+    for k, t in chans:
+        g, r = _np(t), ref[k]
+        assert g.shape == r.shape, k
+        e = np.abs(g - r)
+        rel = float((e / np.maximum(np.abs(r), 1e6)).max())
+        rec["channels"][k] = {"max": round(float(e.max()), 4), "rms": round(float(np.sqrt(np.mean(e ** 2))), 4),
+                              "max_rel": rel, "unit": "codes" if k == "rfhpf" else "Hz"}
+        if k in rf.video_fields:
+            worst[k] = rel
+    # TBC: the GPU's whole decode against the oracle's
+    x = torch.from_numpy(cap.data).to(rf.device)
+    dec = capture.CaptureDecoder(rf, x, locate=True, max_fields=8)
+    out = dec.step().clone()
+    torch.cuda.synchronize()
+    W, lines = dec.W, dec.linesout
+    mine = {f["demod_offset"]: out[j * W * lines:(j + 1) * W * lines].cpu().numpy() for j, f in enumerate(dec.fields)}
+    ora = _oracle_fields(orf, ref, system, 8)
+    common = sorted(set(mine) & set(ora))
+    d = np.concatenate([np.abs(mine[o].astype(np.int64) - ora[o].astype(np.int64)) for o in common]) \
+        if common else np.zeros(0, dtype=np.int64)
+    hist = {str(v): int(np.sum(d == v)) for v in range(3)}
+    hist["3+"] = int(np.sum(d >= 3))
+    rec["tbc"] = {"fields_gpu": len(mine), "fields_oracle": len(ora), "fields_compared": len(common),
+                  "pixels": int(d.size), "hist_abs_lsb": hist,
+                  "within_1": float(np.mean(d <= 1)) if d.size else None, "max": int(d.max()) if d.size else None}
+    RESULTS.append(rec)
+    print(json.dumps(rec), flush=True)
+    for k, rel in worst.items():
+        assert rel <= 1e-4, (k, rel)
+    assert len(common) >= 3, (sorted(mine), sorted(ora))
+    assert rec["tbc"]["within_1"] >= (0.99 if variant == "mfma" else 0.999), rec["tbc"]
