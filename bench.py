@@ -378,16 +378,15 @@ def profile_step(args):
     log("profile step done")
 
 
-def _pmc_pass(counter, args, workdir):
-    """One rocprofv3 counter pass over `bench.py --profile-step` (a child
-    process); returns {kernel name: [values per dispatch]}."""
+def _pmc_run(counters, target, workdir, tag):
+    """One rocprofv3 counter pass (the counters collected together) over
+    `bench.py <target>` in a child process; returns {kernel name: {counter:
+    [(sum, max) over the counter's instances, per dispatch]}}."""
     import sqlite3
     import subprocess
-    out = pathlib.Path(workdir) / counter.lower()
-    cmd = ["rocprofv3", "--pmc", counter, "-d", str(out), "-o", "run", "--", sys.executable, "-u",
-           str(pathlib.Path(__file__).resolve()), "--profile-step", "--steps", "1", "--blocklen", str(args.blocklen),
-           "--fft", args.fft, "--wave-blocks", str(args.wave_blocks), "--groups", str(args.groups),
-           "--seed", str(args.seed)]
+    out = pathlib.Path(workdir) / tag
+    cmd = ["rocprofv3", "--pmc", *counters, "-d", str(out), "-o", "run", "--", sys.executable, "-u",
+           str(pathlib.Path(__file__).resolve()), *target]
     log(" ".join(cmd))
     env = dict(os.environ, TMPDIR="/tmp")
     subprocess.run(["timeout", "-k", "10", "400"] + cmd, check=True, cwd="/tmp", env=env)
@@ -395,15 +394,123 @@ def _pmc_pass(counter, args, workdir):
     if not dbs:
         raise RuntimeError(f"rocprofv3 wrote no results under {out}")
     con = sqlite3.connect(str(dbs[0]))
-    rows = con.execute("select dispatch_id, kernel_name, grid_size, sum(value) from counters_collection "
-                       "where counter_name = ? group by dispatch_id", (counter,)).fetchall()
     got = {}
-    for _, name, grid, v in rows:
-        short = name.split("(")[0].replace("void ", "")
-        if "neg_kernel" in name:
-            short = "copy"
-        got.setdefault(short, []).append(float(v))
+    for counter in counters:
+        rows = con.execute("select dispatch_id, kernel_name, sum(value), max(value) from counters_collection "
+                           "where counter_name = ? group by dispatch_id order by dispatch_id", (counter,)).fetchall()
+        for _, name, vsum, vmax in rows:
+            short = name.split("(")[0].replace("void ", "")
+            if "neg_kernel" in name:
+                short = "copy"
+            got.setdefault(short, {}).setdefault(counter, []).append((float(vsum), float(vmax)))
     return got
+
+
+def _pmc_pass(counter, args, workdir):
+    """One counter over `bench.py --profile-step` (the C4 pass);
+    returns {kernel name: [values per dispatch]}."""
+    target = ["--profile-step", "--steps", "1", "--blocklen", str(args.blocklen), "--fft", args.fft,
+              "--wave-blocks", str(args.wave_blocks), "--groups", str(args.groups), "--seed", str(args.seed)]
+    got = _pmc_run([counter], target, workdir, counter.lower())
+    return {k: [v[0] for v in c[counter]] for k, c in got.items()}
+
+
+FFT_KERNELS = ("k_fft_fp32", "k_fft_mfma", "k_fft_mfma3", "k_demod_block", "k_demod_block_mfma",
+               "k_demod_block_mfma3")
+MFMA_COUNTERS = ("SQ_VALU_MFMA_BUSY_CYCLES", "GRBM_GUI_ACTIVE", "SQ_INSTS_VALU_MFMA_MOPS_BF16", "SQ_INSTS_MFMA",
+                 "SQ_INSTS_VALU", "SQ_INSTS_LDS", "SQ_BUSY_CYCLES")
+FFT_TRANSFORMS, DEMOD_BLOCKS = 2048, 512
+CLOCK_HZ = 2.4e9               # MI355X peak engine clock
+
+
+def profile_fft(args):
+    """Profiling target for the matrix-core counters: the demodulator's
+    16384-point transform alone (ld_fft_forward over FFT_TRANSFORMS
+    transforms), then DEMOD_BLOCKS 32768-sample NTSC blocks through
+    ld_demod_blocks (every video channel, rfhpf, audio), three times each
+    in each transform variant."""
+    __graft_entry__.load_package()
+    import torch
+    from lddecode_amd import _lib, synth, tune
+    from lddecode_amd.rfdecode import RFDecode
+    rf = RFDecode(system="NTSC", decode_analog_audio=44100, device=0)
+    M = 16384
+    g = torch.Generator(device=rf.device).manual_seed(args.seed)
+    x = torch.randn(FFT_TRANSFORMS * M, dtype=torch.complex64, device=rf.device, generator=g)
+    y = torch.empty_like(x)
+    for name, v in tune.VARIANTS.items():
+        for _ in range(3):
+            _lib.check(rf.lib.ld_fft_forward(rf.handle, x.data_ptr(), y.data_ptr(), FFT_TRANSFORMS, v, None))
+        torch.cuda.synchronize()
+        log(f"fft {name} done")
+    cap = torch.from_numpy(synth.make_capture("NTSC", 2_000_000, seed=args.seed, snr_db=40.0).data).to(rf.device)
+    need = (DEMOD_BLOCKS - 1) * rf.block_stride + rf.blocklen
+    data = cap.repeat(-(-need // cap.numel()))[:need].contiguous()
+    for name, v in tune.VARIANTS.items():
+        rf.set_fft_variant(v)
+        for _ in range(3):
+            rf.demod_blocks(data, 0, DEMOD_BLOCKS, audio=True)
+        torch.cuda.synchronize()
+        log(f"demod {name} done")
+
+
+def measure_mfma(args):
+    """Matrix-core use of the transform variants: one counter pass
+    (MFMA_COUNTERS) and one kernel-trace pass over `bench.py --profile-fft`;
+    per kernel the MFMA busy share of the SIMDs (SQ_VALU_MFMA_BUSY_CYCLES
+    over 1024 SIMDs x the kernel's traced duration x the 2.4 GHz engine
+    clock; also over GRBM_GUI_ACTIVE, rocprofv3's MfmaUtil, whose window is
+    the counter pass's longer dispatch), MFMA FLOPs (MOPS_BF16 x 512), the
+    instruction mix and the average duration; stamped with the kernel
+    sources' hash."""
+    import csv
+    import subprocess
+    import tempfile
+    work = tempfile.mkdtemp(prefix="ld_mfma_", dir="/tmp")
+    pmc = _pmc_run(list(MFMA_COUNTERS), ["--profile-fft", "--seed", str(args.seed)], work, "pmc")
+    out = pathlib.Path(work) / "trace"
+    cmd = ["rocprofv3", "--kernel-trace", "--stats", "--output-format", "csv", "-d", str(out), "-o", "run", "--",
+           sys.executable, "-u", str(pathlib.Path(__file__).resolve()), "--profile-fft", "--seed", str(args.seed)]
+    log(" ".join(cmd))
+    subprocess.run(["timeout", "-k", "10", "400"] + cmd, check=True, cwd="/tmp", env=dict(os.environ, TMPDIR="/tmp"))
+    stats = sorted(out.rglob("*kernel_stats.csv"))
+    if not stats:
+        raise RuntimeError(f"rocprofv3 wrote no kernel statistics under {out}")
+    avg_ns = {}
+    with open(stats[0]) as f:
+        for row in csv.DictReader(f):
+            avg_ns[row["Name"].split("(")[0].replace("void ", "")] = float(row["AverageNs"])
+    simds = 256 * 4
+    kernels = {}
+    for k in FFT_KERNELS:
+        c = pmc.get("ldgpu::" + k)
+        if c is None:
+            continue
+        busy = np.mean([v[0] for v in c["SQ_VALU_MFMA_BUSY_CYCLES"]])
+        gui = np.mean([v[1] for v in c["GRBM_GUI_ACTIVE"]])
+        mops = np.mean([v[0] for v in c["SQ_INSTS_VALU_MFMA_MOPS_BF16"]])
+        ns = avg_ns.get("ldgpu::" + k)
+        rec = {"dispatches": len(c["GRBM_GUI_ACTIVE"]), "avg_ns": ns, "mfma_busy_cycles": busy,
+               "mfma_busy_pct_of_window": round(100.0 * busy / (gui * simds), 3),
+               "mfma_flops": mops * 512, "insts_mfma": np.mean([v[0] for v in c["SQ_INSTS_MFMA"]]),
+               "insts_valu": np.mean([v[0] for v in c["SQ_INSTS_VALU"]]),
+               "insts_lds": np.mean([v[0] for v in c["SQ_INSTS_LDS"]]),
+               "gui_active_cycles": gui}
+        if ns:
+            # busy SIMD cycles over the SIMD cycles of the kernel's own
+            # duration (GRBM_GUI_ACTIVE spans the counter pass's longer
+            # dispatch window)
+            rec["mfma_busy_pct"] = round(100.0 * busy / (ns * 1e-9 * CLOCK_HZ * simds), 3)
+            rec["mfma_tflops"] = round(mops * 512 / ns / 1e3, 2)
+            rec["mfma_frac_of_bf16_peak"] = round(mops * 512 / ns / 1e3 / 2500.0, 5)
+        kernels[k] = rec
+    res = {"source_hash": kernel_source_hash(), "seed": args.seed, "fft_transforms": FFT_TRANSFORMS,
+           "demod_blocks": DEMOD_BLOCKS, "simds": simds, "counters": list(MFMA_COUNTERS),
+           "command": "python bench.py --measure-mfma", "kernels": kernels}
+    dest = pathlib.Path(args.traffic_out) if args.traffic_out else ROOT / "profiles" / "mfma_util.json"
+    dest.parent.mkdir(parents=True, exist_ok=True)
+    dest.write_text(json.dumps(res, indent=1) + "\n")
+    print(json.dumps(res, indent=1))
 
 
 def measure_traffic(args):
@@ -472,12 +579,19 @@ def main():
                     help="extra batched passes beside the headline: default (32768, fp32), c5 (65536, mfma: "
                          "hi + lo terms), c5x6 (65536, mfma3: hi + mid + lo terms)")
     ap.add_argument("--profile-step", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--profile-fft", action="store_true", help=argparse.SUPPRESS)
+    ap.add_argument("--measure-mfma", action="store_true",
+                    help="run the matrix-core counter and trace passes over the transform variants")
     ap.add_argument("--measure-traffic", action="store_true",
                     help="run the two rocprofv3 counter passes and write " + str(TRAFFIC_FILE.name))
-    ap.add_argument("--traffic-out", default=None, help="where --measure-traffic writes its JSON")
+    ap.add_argument("--traffic-out", default=None, help="where --measure-traffic / --measure-mfma write their JSON")
     args = ap.parse_args()
     if (args.profile_step or args.measure_traffic) and "auto" in (args.blocklen, args.fft):
         ap.error("--profile-step / --measure-traffic take a fixed --blocklen and --fft")
+    if args.profile_fft:
+        return profile_fft(args)
+    if args.measure_mfma:
+        return measure_mfma(args)
     if args.profile_step:
         return profile_step(args)
     if args.measure_traffic:

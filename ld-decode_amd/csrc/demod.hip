@@ -187,6 +187,8 @@ struct DemodCfg;
 template <>
 struct DemodCfg<32768> {
   static constexpr int N = 32768, M = 16384, T = 1024, P = 16, NA = 1024;
+  static constexpr int Wg = 1;  // workgroups per CU (LDS: 156 KB each)
+  static constexpr int WavesPerSimd = Wg * T / 64 / 4;  // __launch_bounds__'s second argument
   using Fwd = WgFft16kT1024<-1>;
   using FwdMfma = WgFft16kMfma;
   using Tw = TwLdsT<1024>;
@@ -199,6 +201,8 @@ struct DemodCfg<32768> {
 template <>
 struct DemodCfg<16384> {
   static constexpr int N = 16384, M = 8192, T = 512, P = 16, NA = 512;
+  static constexpr int Wg = 2;  // two workgroups per CU (LDS: 80 KB each; 128 VGPRs)
+  static constexpr int WavesPerSimd = Wg * T / 64 / 4;
   using Fwd = WgFft8kT512<-1>;
   using FwdMfma = WgFft8kMfma;
   using Tw = TwLds8k;
@@ -821,22 +825,22 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
 
 #define LD_DEMOD_AT(a) (a), (a).first + (a).step * (int64_t)blockIdx.x, (int)blockIdx.x, (int)blockIdx.x
 #define LD_DEMOD_KERNELS(C, sfx)                                                                             \
-  __global__ __launch_bounds__(C::T) void k_demod_block##sfx(DemodArgs a) {                                 \
+  __global__ __launch_bounds__(C::T, C::WavesPerSimd) void k_demod_block##sfx(DemodArgs a) {                                 \
     demod_block<C, false, false>(LD_DEMOD_AT(a));                                                            \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_packed(DemodArgs a) {                        \
+  __global__ __launch_bounds__(C::T, C::WavesPerSimd) void k_demod_block##sfx##_packed(DemodArgs a) {                        \
     demod_block<C, true, false>(LD_DEMOD_AT(a));                                                             \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_efm(DemodArgs a) {                           \
+  __global__ __launch_bounds__(C::T, C::WavesPerSimd) void k_demod_block##sfx##_efm(DemodArgs a) {                           \
     demod_block<C, false, true>(LD_DEMOD_AT(a));                                                             \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_packed_efm(DemodArgs a) {                    \
+  __global__ __launch_bounds__(C::T, C::WavesPerSimd) void k_demod_block##sfx##_packed_efm(DemodArgs a) {                    \
     demod_block<C, true, true>(LD_DEMOD_AT(a));                                                              \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_mfma(DemodArgs a) {                          \
+  __global__ __launch_bounds__(C::T, C::WavesPerSimd) void k_demod_block##sfx##_mfma(DemodArgs a) {                          \
     demod_block<C, false, false, 2>(LD_DEMOD_AT(a));                                                         \
   }                                                                                                          \
-  __global__ __launch_bounds__(C::T) void k_demod_block##sfx##_mfma3(DemodArgs a) {                         \
+  __global__ __launch_bounds__(C::T, C::WavesPerSimd) void k_demod_block##sfx##_mfma3(DemodArgs a) {                         \
     demod_block<C, false, false, 3>(LD_DEMOD_AT(a));                                                         \
   }
 LD_DEMOD_KERNELS(Cfg32, )
@@ -869,9 +873,9 @@ __device__ __forceinline__ void demod_batch(const DemodBatchArgs& a) {
 __global__ __launch_bounds__(Cfg32::T) void k_demod_batch(DemodBatchArgs a) { demod_batch<Cfg32, 0>(a); }
 __global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma(DemodBatchArgs a) { demod_batch<Cfg32, 2>(a); }
 __global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma3(DemodBatchArgs a) { demod_batch<Cfg32, 3>(a); }
-__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16(DemodBatchArgs a) { demod_batch<Cfg16, 0>(a); }
-__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16_mfma(DemodBatchArgs a) { demod_batch<Cfg16, 2>(a); }
-__global__ __launch_bounds__(Cfg16::T) void k_demod_batch16_mfma3(DemodBatchArgs a) { demod_batch<Cfg16, 3>(a); }
+__global__ __launch_bounds__(Cfg16::T, Cfg16::WavesPerSimd) void k_demod_batch16(DemodBatchArgs a) { demod_batch<Cfg16, 0>(a); }
+__global__ __launch_bounds__(Cfg16::T, Cfg16::WavesPerSimd) void k_demod_batch16_mfma(DemodBatchArgs a) { demod_batch<Cfg16, 2>(a); }
+__global__ __launch_bounds__(Cfg16::T, Cfg16::WavesPerSimd) void k_demod_batch16_mfma3(DemodBatchArgs a) { demod_batch<Cfg16, 3>(a); }
 
 // The demodulator's forward M-point transform alone (FP32 or split-BF16
 // MFMA), one workgroup per transform: in[b M .. + M) -> out[b M .. + M).
