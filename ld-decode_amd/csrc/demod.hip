@@ -382,7 +382,7 @@ __device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid
       int n = 2 * (tid + C::T * m);
       uint32_t i0 = (uint32_t)((n - off) & (C::N - 1));
       const float y0 = v.x + dc, y1 = dc - v.y;
-      bstore2(r, make_float2(y0, y1), i0 * 4u, 0u);
+      bstore2<kStream>(r, make_float2(y0, y1), i0 * 4u, 0u);
       if (words != nullptr) {
         const unsigned long long e0 = __ballot(y0 <= highf), e1 = __ballot(y1 <= highf);
         const uint32_t w = ((uint32_t)(e0 >> (16 * q)) & 0xffffu) | ((uint32_t)(e1 >> (16 * q)) << 16);
@@ -401,8 +401,8 @@ __device__ __forceinline__ void c2r_store(const C2rOut& o, int64_t base, int tid
     int n = 2 * (tid + C::T * m);
     uint32_t i0 = (uint32_t)((n - off) & (C::N - 1));
     uint32_t i1 = (uint32_t)((n + 1 - off) & (C::N - 1));
-    bstore1(r, v.x + dc, i0 * 4u, 0u);
-    bstore1(r, dc - v.y, i1 * 4u, 0u);
+    bstore1<kStream>(r, v.x + dc, i0 * 4u, 0u);
+    bstore1<kStream>(r, dc - v.y, i1 * 4u, 0u);
   });
 }
 
@@ -454,7 +454,7 @@ __device__ void audio_wave(const DemodArgs& a, int c, const float2* __restrict__
       hz = inc_to_hz(atan2f(p.y, p.x), hz_per_rad, rate);
     }
     int i = n - a.acut;  // i < 0 wraps to a huge offset: dropped
-    bstore1(out, hz + lf, (uint32_t)i * 4u, 0u);
+    bstore1<kStream>(out, hz + lf, (uint32_t)i * 4u, 0u);
   }
 }
 
@@ -545,11 +545,11 @@ __device__ __forceinline__ void efm_store(int16_t* outp, int cut, int len, int t
     const int n = 2 * (tid + C::T * m);
     const uint32_t i0 = (uint32_t)((n - cut) & (C::N - 1));
     if (pairs) {
-      __builtin_amdgcn_raw_buffer_store_b32(q(v.x) | (q(-v.y) << 16), r, (int)(i0 * 2u), 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b32(q(v.x) | (q(-v.y) << 16), r, (int)(i0 * 2u), 0, kStream);
     } else {
       const uint32_t i1 = (uint32_t)((n + 1 - cut) & (C::N - 1));
-      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(v.x), r, (int)(i0 * 2u), 0, 0);
-      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(-v.y), r, (int)(i1 * 2u), 0, 0);
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(v.x), r, (int)(i0 * 2u), 0, kStream);
+      __builtin_amdgcn_raw_buffer_store_b16((uint16_t)q(-v.y), r, (int)(i1 * 2u), 0, kStream);
     }
   });
 }
@@ -728,10 +728,10 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
         for (int m = 0; m < C::P; ++m) {  // i < 0 wraps to a huge offset: dropped
           const uint32_t i0 = (uint32_t)(2 * (tidv + C::T * m) - a.vcut);
           if (pairs) {
-            bstore2(raw, v[m], i0 * 4u, 0u);  // aligned pair, wholly in or out
+            bstore2<kStream>(raw, v[m], i0 * 4u, 0u);  // aligned pair, wholly in or out
           } else {
-            bstore1(raw, v[m].x, i0 * 4u, 0u);
-            bstore1(raw, v[m].y, i0 * 4u + 4u, 0u);
+            bstore1<kStream>(raw, v[m].x, i0 * 4u, 0u);
+            bstore1<kStream>(raw, v[m].y, i0 * 4u + 4u, 0u);
           }
         }
       }
@@ -1076,7 +1076,7 @@ __device__ __forceinline__ void c2r_store64(const C2rOut& o, int64_t base, const
     const float y0 = v[m].x + dc, y1 = dc - v[m].y;
     const uint32_t i0 = (uint32_t)((n - off) & (kN64 - 1));
     if (pairs) {
-      bstore2(r, make_float2(y0, y1), i0 * 4u, 0u);
+      bstore2<kStream>(r, make_float2(y0, y1), i0 * 4u, 0u);
       if (words != nullptr) {
         const unsigned long long e0 = __ballot(y0 <= highf), e1 = __ballot(y1 <= highf);
         const uint32_t w = ((uint32_t)(e0 >> (16 * q)) & 0xffffu) | ((uint32_t)(e1 >> (16 * q)) << 16);
@@ -1088,8 +1088,8 @@ __device__ __forceinline__ void c2r_store64(const C2rOut& o, int64_t base, const
       }
     } else {
       const uint32_t i1 = (uint32_t)((n + 1 - off) & (kN64 - 1));
-      bstore1(r, y0, i0 * 4u, 0u);
-      bstore1(r, y1, i1 * 4u, 0u);
+      bstore1<kStream>(r, y0, i0 * 4u, 0u);
+      bstore1<kStream>(r, y1, i1 * 4u, 0u);
     }
   });
 }
@@ -1145,7 +1145,7 @@ __device__ void audio_wave64(const DemodArgs& a, int c, const float2* slice, flo
       const float2 p = cmulc(v[m], make_float2(lre[pidx(n - 1)], lim[pidx(n - 1)]));
       hz = inc_to_hz(atan2f(p.y, p.x), hz_per_rad, rate);
     }
-    bstore1(out, hz + lf, (uint32_t)(n - a.acut) * 4u, 0u);
+    bstore1<kStream>(out, hz + lf, (uint32_t)(n - a.acut) * 4u, 0u);
   }
 }
 
@@ -1315,10 +1315,10 @@ __device__ __forceinline__ void demod_block64(const DemodArgs& a, int64_t s0, in
       for (int m = 0; m < 32; ++m) {
         const uint32_t i0 = (uint32_t)(2 * (tid + kT * m) - a.vcut);
         if (pairs) {
-          bstore2(raw, v[m], i0 * 4u, 0u);
+          bstore2<kStream>(raw, v[m], i0 * 4u, 0u);
         } else {
-          bstore1(raw, v[m].x, i0 * 4u, 0u);
-          bstore1(raw, v[m].y, i0 * 4u + 4u, 0u);
+          bstore1<kStream>(raw, v[m].x, i0 * 4u, 0u);
+          bstore1<kStream>(raw, v[m].y, i0 * 4u + 4u, 0u);
         }
       }
     }

@@ -34,12 +34,18 @@ __device__ __forceinline__ float2 bload2(rsrc_t r, uint32_t voff, uint32_t soff)
 __device__ __forceinline__ float bload1(rsrc_t r, uint32_t voff, uint32_t soff) {
   return __builtin_bit_cast(float, __builtin_amdgcn_raw_buffer_load_b32(r, (int)voff, (int)soff, 0));
 }
+// Stores take the cache policy as a template argument: 0 (default) or
+// kStream, the non-temporal hint ("nt" on gfx950), for outputs this kernel
+// never reads again, so that they do not push the filter tables out of L2.
+constexpr int kStream = 2;
+template <int AUX = 0>
 __device__ __forceinline__ void bstore1(rsrc_t r, float v, uint32_t voff, uint32_t soff) {
-  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, 0);
+  __builtin_amdgcn_raw_buffer_store_b32(__builtin_bit_cast(uint32_t, v), r, (int)voff, (int)soff, AUX);
 }
+template <int AUX = 0>
 __device__ __forceinline__ void bstore2(rsrc_t r, float2 v, uint32_t voff, uint32_t soff) {
   __builtin_amdgcn_raw_buffer_store_b64(__builtin_bit_cast(decltype(__builtin_amdgcn_raw_buffer_load_b64(r, 0, 0, 0)), v),
-                                        r, (int)voff, (int)soff, 0);
+                                        r, (int)voff, (int)soff, AUX);
 }
 
 // ---------------------------------------------------------------- complex ops
