@@ -230,7 +230,8 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int input_dtype, in
  * loading the library holds until the process ends; nothing is allocated or
  * freed.  The caller still passes the input and the outputs.  nblocks is
  * limited to what fits the array (ld_demod_workspace_bytes(h, nblocks) <=
- * 32 MiB: 256 blocks of 32768); larger requests return LD_EINVAL.  Nothing in
+ * 32 MiB: 512 blocks of 16384, 256 of 32768, 61 of 65536 samples); larger
+ * requests return LD_EINVAL.  Nothing in
  * the array needs to start at zero (every block writes its workspace before
  * reading it).  All calls on one GPU share the array: calls that overlap on
  * one GPU (concurrent streams, host threads) race and must be serialised by
