@@ -568,9 +568,10 @@ def main():
     ap.add_argument("--steps", type=int, default=3, help="timed passes over the whole batch")
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--seed", type=int, default=4)
-    ap.add_argument("--wave-blocks", type=int, default=8192, help="blocks per wave of the batched decode")
+    ap.add_argument("--wave-blocks", type=int, default=16384,
+                    help="32768-sample blocks per wave of the batched decode (scaled for other block lengths)")
     ap.add_argument("--groups", type=int, default=2, help="batch decoders taking turns on the host")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0, help="minimum CPU baseline time (0: skip)")
+    ap.add_argument("--cpu-seconds", type=float, default=12.0, help="minimum CPU baseline time (0: skip)")
     ap.add_argument("--blocklen", default="auto", choices=["auto", "16384", "32768", "65536"],
                     help="block length of the batched decode (auto: the autotuner's choice)")
     ap.add_argument("--fft", default="auto", choices=["auto", "fp32", "mfma", "mfma3"],

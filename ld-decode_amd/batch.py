@@ -203,8 +203,9 @@ class BatchDecoder:
             streams = (torch.cuda.Stream(device=self.device), torch.cuda.Stream(device=self.device))
         self.stream_a, self.stream_b = streams
         # demod_order: a dict shared by decoders taking turns; each demod
-        # launch waits for the one queued before it by any of them (one
-        # demodulation fills the GPU; their order then follows the host's)
+        # launch waits for the one queued before it by any of them and for
+        # that wave's pulse search (one demodulation fills the GPU; their
+        # order then follows the host's)
         self.demod_order = demod_order
         # demod_timing: a list receiving (start event, end event, algorithmic
         # bytes) of every wave's demod graph (the block table upload and the
@@ -337,13 +338,14 @@ class BatchDecoder:
             if timing:
                 e1.record(self.stream_a)
                 self.demod_timing.append((e0, e1, sum(n * self.S * DEMOD_BYTES[s] for s, n in st["used"].items())))
-            if self.demod_order is not None:
-                done = torch.cuda.Event()
-                done.record(self.stream_a)
-                self.demod_order["last"] = done
             self._replay(st, "graph_pulses", self._pulses_body)
             ev = torch.cuda.Event()
             ev.record(self.stream_a)
+            if self.demod_order is not None:
+                # the next decoder's demod waits for this wave's pulse search
+                # too: those short kernels would otherwise wait for CUs behind
+                # it, and the host's walk of this wave behind them
+                self.demod_order["last"] = ev
         st["pulses_done"] = ev
         st["wave"] = k
 
