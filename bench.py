@@ -17,7 +17,7 @@ wave's demodulation with the previous wave's alignment and TBC.
 
 value = RF samples of the batch decoded per second (each capture sample once;
 blocks demodulated twice where windows overlap are work, not throughput).
-The block length (16384, 32768 or 65536 samples) and the transform (FP32
+The block length (8192, 16384, 32768 or 65536 samples) and the transform (FP32
 passes or split-BF16 MFMA radix-64 stages) are the autotuner's choice
 (lddecode_amd.tune: the fastest demod launch on a small mixed batch) unless
 --blocklen / --fft fix them; the same batch is also decoded at the
@@ -572,7 +572,7 @@ def main():
                     help="32768-sample blocks per wave of the batched decode (scaled for other block lengths)")
     ap.add_argument("--groups", type=int, default=2, help="batch decoders taking turns on the host")
     ap.add_argument("--cpu-seconds", type=float, default=12.0, help="minimum CPU baseline time (0: skip)")
-    ap.add_argument("--blocklen", default="auto", choices=["auto", "16384", "32768", "65536"],
+    ap.add_argument("--blocklen", default="auto", choices=["auto", "8192", "16384", "32768", "65536"],
                     help="block length of the batched decode (auto: the autotuner's choice)")
     ap.add_argument("--fft", default="auto", choices=["auto", "fp32", "mfma", "mfma3"],
                     help="transform variant of the batched decode (auto: the autotuner's choice)")
@@ -664,8 +664,8 @@ def main():
 
     achieved = c4["demod_bytes_avg"] / (c4["demod_ms_avg"] / 1e3) / 1e9
     iso_gbs = c4["iso_bytes_avg"] / (c4["iso_ms_avg"] / 1e3) / 1e9
-    kname = {16384: "ldgpu::k_demod_batch", 32768: "ldgpu::k_demod_batch", 65536: "ldgpu::k_demod_batch64"}[blocklen]
-    kname += "16" if blocklen == 16384 else ""
+    kname = {8192: "ldgpu::k_demod_batch8", 16384: "ldgpu::k_demod_batch16", 32768: "ldgpu::k_demod_batch",
+             65536: "ldgpu::k_demod_batch64"}[blocklen]
     kname += {"fp32": "", "mfma": "_mfma", "mfma3": "_mfma3"}[fft]
     traffic, traffic_note = load_traffic(kname, args.wave_blocks, args.seed, args.groups, blocklen, fft)
 

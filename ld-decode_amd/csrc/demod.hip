@@ -173,6 +173,32 @@ struct TwLds8k {
 };
 static_assert(TwLds8k::count <= TwLds::count, "the handle table holds either image");
 
+// The 4096-point, 256-thread transform's pass twiddles: passes (R 16, NS 16),
+// (R 16, NS 256 = 16 h + l).
+struct TwLds4k {
+  float2 p16[16 * 4];
+  float2 a256[16 * 4];
+  float2 b256[16 * 4];
+  static constexpr int count = 16 * 4 * 3;
+  template <int NS>
+  __device__ __forceinline__ float2 get(int k, int i) const {
+    if constexpr (NS == 16) {
+      return p16[k * 4 + i];
+    } else {
+      static_assert(NS == 256, "M-point transform passes");
+      return cmul(a256[(k >> 4) * 4 + i], b256[(k & 15) * 4 + i]);
+    }
+  }
+  static void fill(float2* t, float2 (*cexp)(double, double)) {
+    for (int k = 0; k < 16; ++k)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * k, 256);
+    for (int h = 0; h < 16; ++h)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 16 * h, 4096);
+    for (int l = 0; l < 16; ++l)
+      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * l, 4096);
+  }
+};
+
 template <int M>
 struct LdsT {
   float re[LdsSize<M>::floats];
@@ -187,6 +213,7 @@ struct DemodCfg;
 template <>
 struct DemodCfg<32768> {
   static constexpr int N = 32768, M = 16384, T = 1024, P = 16, NA = 1024;
+  static constexpr int NotchMax = 508;  // bins of the spur notch window (148 used at 40 MSPS)
   static constexpr int Wg = 1;  // workgroups per CU (LDS: 156 KB each)
   static constexpr int WavesPerSimd = Wg * T / 64 / 4;  // __launch_bounds__'s second argument
   using Fwd = WgFft16kT1024<-1>;
@@ -201,6 +228,7 @@ struct DemodCfg<32768> {
 template <>
 struct DemodCfg<16384> {
   static constexpr int N = 16384, M = 8192, T = 512, P = 16, NA = 512;
+  static constexpr int NotchMax = 508;
   static constexpr int Wg = 2;  // two workgroups per CU (LDS: 80 KB each; 128 VGPRs)
   static constexpr int WavesPerSimd = Wg * T / 64 / 4;
   using Fwd = WgFft8kT512<-1>;
@@ -215,6 +243,26 @@ struct DemodCfg<16384> {
 using Cfg32 = DemodCfg<32768>;
 using Lds = LdsT<kM>;
 using Cfg16 = DemodCfg<16384>;
+template <>
+struct DemodCfg<8192> {
+  static constexpr int N = 8192, M = 4096, T = 256, P = 16, NA = 256;
+  using Fwd = WgFft4kT256<-1>;
+  using FwdMfma = WgFft4kMfma;
+  using Tw = TwLds4k;
+  using Inv = WgFft<256, +1, kSyncWave>;
+  static constexpr uint32_t HalfBytes = (M + 1) * 8, StrideB = T * 8;
+  static constexpr int NotchMax = 128;  // 37 bins used at 40 MSPS
+  static constexpr int Wg = 4;          // four workgroups per CU (LDS: 40 KB each; 128 VGPRs)
+  static constexpr int WavesPerSimd = Wg * T / 64 / 4;
+  static __device__ __forceinline__ TwGlobal audio_tw(rsrc_t r) {
+    TwGlobal g{r, TwLayout::k256_p2, TwLayout::k256_p3, 4, 2, 2};
+    g.off_p4 = TwLayout::k256_p4;
+    g.ns_p3 = 16;
+    g.lr_p4 = 2;
+    return g;
+  }
+};
+using Cfg8 = DemodCfg<8192>;
 
 // W_N^(tid + T m) from the thread's split[tid] = exp(-2 pi i tid / N) and a
 // compile-time constant exp(-2 pi i m T / N)
@@ -571,8 +619,8 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   __shared__ MfmaTw mtw;
   __shared__ float2 aslice[2][C::NA / 2 + 1];
   __shared__ typename C::Tw twl;
-  __shared__ float notch_mag[kNotchMax];
-  __shared__ int notch_flag[kNotchMax + 4];
+  __shared__ float notch_mag[C::NotchMax];
+  __shared__ int notch_flag[C::NotchMax + 4];
   __shared__ double notch_red[128];
   __shared__ uint32_t notch_mask[C::T];  // bit m: bin tid + T m zeroed by the spur notch
   // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters, 6 EFM
@@ -845,6 +893,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   }
 LD_DEMOD_KERNELS(Cfg32, )
 LD_DEMOD_KERNELS(Cfg16, 16)
+LD_DEMOD_KERNELS(Cfg8, 8)
 #undef LD_DEMOD_KERNELS
 #undef LD_DEMOD_AT
 
@@ -876,6 +925,9 @@ __global__ __launch_bounds__(Cfg32::T) void k_demod_batch_mfma3(DemodBatchArgs a
 __global__ __launch_bounds__(Cfg16::T, Cfg16::WavesPerSimd) void k_demod_batch16(DemodBatchArgs a) { demod_batch<Cfg16, 0>(a); }
 __global__ __launch_bounds__(Cfg16::T, Cfg16::WavesPerSimd) void k_demod_batch16_mfma(DemodBatchArgs a) { demod_batch<Cfg16, 2>(a); }
 __global__ __launch_bounds__(Cfg16::T, Cfg16::WavesPerSimd) void k_demod_batch16_mfma3(DemodBatchArgs a) { demod_batch<Cfg16, 3>(a); }
+__global__ __launch_bounds__(Cfg8::T, Cfg8::WavesPerSimd) void k_demod_batch8(DemodBatchArgs a) { demod_batch<Cfg8, 0>(a); }
+__global__ __launch_bounds__(Cfg8::T, Cfg8::WavesPerSimd) void k_demod_batch8_mfma(DemodBatchArgs a) { demod_batch<Cfg8, 2>(a); }
+__global__ __launch_bounds__(Cfg8::T, Cfg8::WavesPerSimd) void k_demod_batch8_mfma3(DemodBatchArgs a) { demod_batch<Cfg8, 3>(a); }
 
 // The demodulator's forward M-point transform alone (FP32 or split-BF16
 // MFMA), one workgroup per transform: in[b M .. + M) -> out[b M .. + M).
@@ -1562,8 +1614,8 @@ int ld_create(int device, int blocklen, ld_handle** out) {
     return LD_EINVAL;
   }
   *out = nullptr;
-  if (blocklen != Cfg16::N && blocklen != kN && blocklen != kN64) {
-    set_error("ld_create: blocklen %d not supported (%d, %d or %d)", blocklen, Cfg16::N, kN, kN64);
+  if (blocklen != Cfg8::N && blocklen != Cfg16::N && blocklen != kN && blocklen != kN64) {
+    set_error("ld_create: blocklen %d not supported (%d, %d, %d or %d)", blocklen, Cfg8::N, Cfg16::N, kN, kN64);
     return LD_EINVAL;
   }
   LD_HIP(hipSetDevice(device));
@@ -1595,12 +1647,18 @@ int ld_create(int device, int blocklen, ld_handle** out) {
   rows(TwLayout::k2k_p3, 2, 1024);
   rows(TwLayout::k512_p2, 8, 8);
   rows(TwLayout::k512_p3, 8, 64);
+  rows(TwLayout::k256_p2, 4, 4);
+  rows(TwLayout::k256_p3, 4, 16);
+  rows(TwLayout::k256_p4, 4, 64);
   // split[t] = exp(-2 pi i t / 32768), t < 1024: W_N^t of the 32768-sample
   // kernel's real split, W_M^t of the 65536-sample kernel's radix-2 step and
   // its split; at 16384 samples split[t] = exp(-2 pi i t / 16384), t < 512
   if (blocklen == Cfg16::N) {
     TwLds8k::fill(tw.data() + TwLayout::total, cexp);
     for (int t2 = 0; t2 < Cfg16::T; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, Cfg16::N);
+  } else if (blocklen == Cfg8::N) {
+    TwLds4k::fill(tw.data() + TwLayout::total, cexp);
+    for (int t2 = 0; t2 < Cfg8::T; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, Cfg8::N);
   } else {
     TwLds::fill(tw.data() + TwLayout::total, cexp);
     for (int t2 = 0; t2 < kT; ++t2) tw[TwLayout::total + kTwLdsCount + t2] = cexp(t2, kN);
@@ -1880,9 +1938,9 @@ int ld_set_notch(ld_handle* h, int lo, int hi) {
     set_error("ld_set_notch: null handle");
     return LD_EINVAL;
   }
-  if (!(lo == 0 && hi == 0) && !(lo >= 1 && lo < hi && hi < h->blocklen / 2 && hi - lo <= kNotchMax)) {
-    set_error("ld_set_notch: window [%d, %d) outside [1, %d) or wider than %d bins", lo, hi, h->blocklen / 2,
-              kNotchMax);
+  const int wmax = h->blocklen == Cfg8::N ? Cfg8::NotchMax : kNotchMax;
+  if (!(lo == 0 && hi == 0) && !(lo >= 1 && lo < hi && hi < h->blocklen / 2 && hi - lo <= wmax)) {
+    set_error("ld_set_notch: window [%d, %d) outside [1, %d) or wider than %d bins", lo, hi, h->blocklen / 2, wmax);
     return LD_EINVAL;
   }
   h->notch_lo = lo;
@@ -2059,18 +2117,20 @@ int ld_demod_blocks(ld_handle* h, const void* input, int dtype, int64_t input_le
     set_error("ld_demod_blocks: the split-BF16 MFMA variant takes int16 / float32 input without EFM output");
     return LD_EINVAL;
   }
-  const bool b16 = h->blocklen == Cfg16::N;
-  void (*kern)(DemodArgs);
-  if (h->fft_variant == LD_FFT_MFMA_BF16X3)
-    kern = b16 ? k_demod_block16_mfma : k_demod_block_mfma;
-  else if (h->fft_variant == LD_FFT_MFMA_BF16X6)
-    kern = b16 ? k_demod_block16_mfma3 : k_demod_block_mfma3;
-  else if (dtype == LD_IN_PACKED10)
-    kern = a.efm ? (b16 ? k_demod_block16_packed_efm : k_demod_block_packed_efm)
-                 : (b16 ? k_demod_block16_packed : k_demod_block_packed);
-  else
-    kern = a.efm ? (b16 ? k_demod_block16_efm : k_demod_block_efm) : (b16 ? k_demod_block16 : k_demod_block);
-  hipLaunchKernelGGL(kern, dim3(nblocks), dim3(b16 ? Cfg16::T : kT), 0, st, a);
+  // kernels per block length: {plain, packed, efm, packed_efm, mfma, mfma3}
+  using K = void (*)(DemodArgs);
+  static const K k32[6] = {k_demod_block, k_demod_block_packed, k_demod_block_efm, k_demod_block_packed_efm,
+                           k_demod_block_mfma, k_demod_block_mfma3};
+  static const K k16[6] = {k_demod_block16, k_demod_block16_packed, k_demod_block16_efm,
+                           k_demod_block16_packed_efm, k_demod_block16_mfma, k_demod_block16_mfma3};
+  static const K k8[6] = {k_demod_block8, k_demod_block8_packed, k_demod_block8_efm, k_demod_block8_packed_efm,
+                          k_demod_block8_mfma, k_demod_block8_mfma3};
+  const K* ks = h->blocklen == Cfg16::N ? k16 : (h->blocklen == Cfg8::N ? k8 : k32);
+  const int threads = h->blocklen == Cfg16::N ? Cfg16::T : (h->blocklen == Cfg8::N ? Cfg8::T : kT);
+  int which = (dtype == LD_IN_PACKED10 ? 1 : 0) + (a.efm ? 2 : 0);
+  if (h->fft_variant == LD_FFT_MFMA_BF16X3) which = 4;
+  if (h->fft_variant == LD_FFT_MFMA_BF16X6) which = 5;
+  hipLaunchKernelGGL(ks[which], dim3(nblocks), dim3(threads), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }
@@ -2137,6 +2197,10 @@ int ld_demod_batch(ld_handle* const* sys, const void* input, int dtype, int64_t 
     kern = v == LD_FFT_MFMA_BF16X3 ? k_demod_batch16_mfma : v == LD_FFT_MFMA_BF16X6 ? k_demod_batch16_mfma3
                                                                                      : k_demod_batch16;
     threads = Cfg16::T;
+  } else if (h0->blocklen == Cfg8::N) {
+    kern = v == LD_FFT_MFMA_BF16X3 ? k_demod_batch8_mfma : v == LD_FFT_MFMA_BF16X6 ? k_demod_batch8_mfma3
+                                                                                   : k_demod_batch8;
+    threads = Cfg8::T;
   } else {
     kern = v == LD_FFT_MFMA_BF16X3 ? k_demod_batch_mfma : v == LD_FFT_MFMA_BF16X6 ? k_demod_batch_mfma3
                                                                                    : k_demod_batch;
@@ -2161,7 +2225,7 @@ int ld_fft_forward(ld_handle* h, const void* in, void* out, int ntransforms, int
     set_error("ld_fft_forward: bad arguments");
     return LD_EINVAL;
   }
-  if (h->blocklen == Cfg16::N) {
+  if (h->blocklen == Cfg16::N || h->blocklen == Cfg8::N) {
     set_error("ld_fft_forward: the 16384-point transform needs a 32768- or 65536-sample handle");
     return LD_EINVAL;
   }

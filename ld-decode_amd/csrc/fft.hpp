@@ -319,7 +319,12 @@ struct TwLayout {
   // and (R=8, NS=64)
   static constexpr int k512_p2 = k2k_p3 + 1024 * 1;
   static constexpr int k512_p3 = k512_p2 + 8 * 3;
-  static constexpr int total = k512_p3 + 64 * 3;
+  // 256 points (audio slices of 8192-sample blocks): passes (R=4, NS=4),
+  // (R=4, NS=16) and (R=4, NS=64)
+  static constexpr int k256_p2 = k512_p3 + 64 * 3;
+  static constexpr int k256_p3 = k256_p2 + 4 * 2;
+  static constexpr int k256_p4 = k256_p3 + 16 * 2;
+  static constexpr int total = k256_p4 + 64 * 2;
 };
 
 // Twiddle source reading the global rows through a buffer descriptor.
@@ -327,10 +332,12 @@ struct TwGlobal {
   rsrc_t tw;
   int off_p2, off_p3;  // row tables of the second and third pass
   int ns_p2, lr_p2, lr_p3;
+  int off_p4 = 0, ns_p3 = 0, lr_p4 = 0;  // a fourth pass (ns_p3 = 0: none)
   template <int NS>
   __device__ __forceinline__ float2 get(int k, int i) const {
-    const int off = (NS == ns_p2) ? off_p2 : off_p3;
-    const int lr = (NS == ns_p2) ? lr_p2 : lr_p3;
+    const bool p2 = NS == ns_p2, p3 = !p2 && (ns_p3 == 0 || NS == ns_p3);
+    const int off = p2 ? off_p2 : (p3 ? off_p3 : off_p4);
+    const int lr = p2 ? lr_p2 : (p3 ? lr_p3 : lr_p4);
     return bload2(tw, (uint32_t)(off + k * lr + i) * 8u, 0u);
   }
 };
@@ -379,6 +386,33 @@ struct WgFft<512, DIR, SYNC> {
     stockham_pass<512, 64, 8, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
     stockham_pass<512, 64, 8, 8, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
     stockham_pass<512, 64, 8, 64, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
+  }
+};
+
+// 256 points, 64 threads (one wavefront): radix 4 x 4 x 4 x 4
+template <int DIR, int SYNC>
+struct WgFft<256, DIR, SYNC> {
+  static constexpr int T = 64;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[4], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<256, 64, 4, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<256, 64, 4, 4, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<256, 64, 4, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<256, 64, 4, 64, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
+  }
+};
+
+// 4096 points, 256 threads (16 per thread): radix 16 x 16 x 16
+template <int DIR, int SYNC = kSyncBlock>
+struct WgFft4kT256 {
+  static constexpr int T = 256;
+  template <class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF& twf,
+                                             int tid = threadIdx.x) {
+    stockham_pass<4096, 256, 16, 1, DIR, false, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<4096, 256, 16, 16, DIR, true, true, SYNC>(v, lre, lim, twf, tid);
+    stockham_pass<4096, 256, 16, 256, DIR, true, false, SYNC>(v, lre, lim, twf, tid);
   }
 };
 
@@ -594,6 +628,30 @@ __device__ __forceinline__ void mfma64_pass(float* __restrict__ lre, float* __re
   }
   __syncthreads();
 }
+
+// 4096 points, 256 threads, forward: radix 64 x 64 on the matrix cores
+// (split BF16); the result read back from LDS in natural order.
+struct WgFft4kMfma {
+  static constexpr int T = 256;
+  template <int KT, class TWF>
+  static __device__ __forceinline__ void run(float2 (&v)[16], float* lre, float* lim, const TWF&,
+                                             const u32x4_t* gtab, const MfmaTw* tw, int tid = threadIdx.x) {
+    __syncthreads();
+    const LdsPair own(lre, lim, pidx(tid));
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      own.template st<POff<256 * m>::value>(v[m]);
+    });
+    __syncthreads();
+    mfma64_pass<4096, 256, 1, KT>(lre, lim, gtab, tw, tid);
+    mfma64_pass<4096, 256, 64, KT>(lre, lim, gtab, tw, tid);
+    static_for<0, 16>([&](auto mc) {
+      constexpr int m = decltype(mc)::value;
+      v[m] = own.template ld<POff<256 * m>::value>();
+    });
+    __syncthreads();
+  }
+};
 
 // 8192 points, 512 threads, forward: radix 64 x 64 on the matrix cores
 // (split BF16), then radix 2 in FP32.

@@ -2,7 +2,7 @@
 
 The reference takes its block length as a constructor argument
 (RFDecode(blocklen=...), core.py:284-288; 32768 by default) and has one
-transform.  This build offers three block lengths (16384, 32768, 65536) and
+transform.  This build offers four block lengths (8192, 16384, 32768, 65536) and
 three transform variants (FP32 Stockham passes of radix 16 / 16 / 16 / 2-4,
 or radix-64 stages as split-BF16 matrix products of two or three bf16
 terms, LD_FFT_MFMA_BF16X3 / LD_FFT_MFMA_BF16X6), each checked against the
@@ -28,7 +28,7 @@ import time
 
 from . import _lib
 
-LENGTHS = (16384, 32768, 65536)
+LENGTHS = (8192, 16384, 32768, 65536)
 VARIANTS = {"fp32": _lib.LD_FFT_FP32, "mfma": _lib.LD_FFT_MFMA_BF16X3, "mfma3": _lib.LD_FFT_MFMA_BF16X6}
 DEFAULT = {"blocklen": 32768, "fft": "fp32"}
 

@@ -1,6 +1,6 @@
-"""16384- and 65536-sample blocks (RFDecode(blocklen=...), the reference's
-blocklen constructor argument, core.py:284-288): the single-level 8192-point
-kernel (16384) and the two-level transform kernel (65536) against the FP64
+"""8192-, 16384- and 65536-sample blocks (RFDecode(blocklen=...), the
+reference's blocklen constructor argument, core.py:284-288): the single-level
+4096- and 8192-point kernels and the two-level transform kernel (65536) against the FP64
 oracle built at the same block length (its filters are freqz samples at
 that many bins), in FP32 and with the split-BF16 MFMA radix-64 stages,
 single and stitched blocks, and the batched decode at that length against
@@ -8,7 +8,7 @@ the per-capture decode.
 
 Tolerances as the 32768-sample parity tests: video max |err| <= 25 Hz and
 1e-4 relative (to max(|value|, 1 MHz)); rfhpf <= 0.05 codes; audio stage 1
-<= 10 Hz.  MFMA variant: the north star's 1e-4 relative on every channel.
+<= 10 Hz.  MFMA variants: 1e-4 relative on every channel.
 """
 
 import numpy as np
@@ -19,7 +19,7 @@ from oracle import dsp as O
 
 pytestmark = pytest.mark.gpu
 
-LENGTHS = [16384, 65536]
+LENGTHS = [8192, 16384, 65536]
 _RF, _ORF = {}, {}
 
 

@@ -3,7 +3,7 @@
 Tolerances: line locations |err| <= 1e-3 samples with identical bad-line
 flags; TBC uint16 codes bit-exact when both sides resample the same demod
 buffer; end to end (GPU FP32 demod vs FP64 oracle demod) every code within
-+-1 LSB and >= 99.9% of codes exact or +-1 (the north-star bar), in both the
++-1 LSB and >= 99.9% of codes exact or +-1, in both the
 float32-coordinate (reference) and FP64-coordinate modes.
 """
 
