@@ -584,6 +584,37 @@ int ld_locate_fields(const ld_locate_params* p, const int64_t* edges, int64_t ne
                      int max_fields, ld_field_loc* out, double* linelocs1, uint8_t* linebad, int max_lines,
                      ld_vpulse* vpulses, int max_vpulses, ld_prev_field* prev_out, int* nfields_out);
 
+/* One ld_locate_fields call of a batch (arguments as there; nfields gets
+ * *nfields_out, status the call's return code). */
+typedef struct ld_walk_job {
+  const ld_locate_params* p;
+  const int64_t* edges;
+  int64_t nedges;
+  const float* means;
+  int64_t total;
+  double start;
+  const ld_prev_field* prev;
+  const ld_field_facts* facts;
+  ld_field_loc* out;
+  double* linelocs1;
+  uint8_t* linebad;
+  ld_vpulse* vpulses;
+  ld_prev_field* prev_out;
+  int32_t max_fields;
+  int32_t max_lines;
+  int32_t max_vpulses;
+  int32_t nfields;
+  int32_t status;
+  int32_t reserved;
+} ld_walk_job;
+
+/* The walks of many captures at once (a batch's captures are independent)
+ * on up to nthreads host threads.  No counterpart in the reference (one
+ * LDdecode per capture, core.py:3498); each job equals its own
+ * ld_locate_fields call.  Returns LD_OK or the first failing job's code.
+ * Host only. */
+int ld_locate_fields_many(ld_walk_job* jobs, int njobs, int nthreads);
+
 /* Check a walk made with provisional previous-field values against the
  * measured facts (facts[k] belongs to locs[k]; prev is the state the walk
  * started from; HOST arrays as ld_locate_fields wrote them, vpulses

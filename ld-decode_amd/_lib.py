@@ -30,7 +30,7 @@ EXPORTS = (
     "ld_dropout_workspace_bytes", "ld_dropout_detect", "ld_dropout_list",
     "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
     "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc", "ld_refine_burst_ntsc_chains",
-    "ld_demod_batch", "ld_set_fft_variant", "ld_fft_forward",
+    "ld_demod_batch", "ld_set_fft_variant", "ld_fft_forward", "ld_locate_fields_many",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -128,6 +128,16 @@ class VPulse(ctypes.Structure):
     _fields_ = [("start", ctypes.c_int64), ("len", ctypes.c_int32), ("kind", ctypes.c_int16), ("good", ctypes.c_int16)]
 
 
+class WalkJob(ctypes.Structure):
+    """ld_walk_job: one ld_locate_fields call of ld_locate_fields_many."""
+    _fields_ = [("p", ctypes.c_void_p), ("edges", ctypes.c_void_p), ("nedges", ctypes.c_int64),
+                ("means", ctypes.c_void_p), ("total", ctypes.c_int64), ("start", ctypes.c_double),
+                ("prev", ctypes.c_void_p), ("facts", ctypes.c_void_p), ("out", ctypes.c_void_p),
+                ("linelocs1", ctypes.c_void_p), ("linebad", ctypes.c_void_p), ("vpulses", ctypes.c_void_p),
+                ("prev_out", ctypes.c_void_p)] + [
+        (n, ctypes.c_int32) for n in ("max_fields", "max_lines", "max_vpulses", "nfields", "status", "reserved")]
+
+
 class DropoutField(ctypes.Structure):
     _fields_ = [("demod_offset", ctypes.c_int64), ("demod_len", ctypes.c_int64), ("lineloc_offset", ctypes.c_int64),
                 ("nlines", ctypes.c_int32), ("lineoffset", ctypes.c_int32), ("linecount", ctypes.c_int32),
@@ -206,6 +216,7 @@ def _declare(lib):
         "ld_locate_fields": (c_int, [P(LocateParams), c_vp, c_i64, c_vp, c_i64, c_dbl, P(PrevField),
                                      P(FieldFacts), c_int, P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int,
                                      P(PrevField), P(c_int)]),
+        "ld_locate_fields_many": (c_int, [P(WalkJob), c_int, c_int]),
         "ld_skip_score": (c_int, [P(LocateParams), P(c_dbl)]),
         "ld_unpack_10bit": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
         "ld_locate_recheck": (c_int, [P(LocateParams), P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int, c_int,

@@ -27,6 +27,7 @@ blocks of its next field read, which are demodulated again.
 
 import ctypes
 import math
+import os
 
 import numpy as np
 
@@ -159,6 +160,8 @@ class BatchDecoder:
         self.coord_f32 = coord_f32
         self.on_wave = on_wave
         self.audio = bool(audio)
+        # host threads of the lanes' native field walks (ld_locate_fields_many)
+        self.walk_threads = min(8, os.cpu_count() or 1)
         S = self.S
         # per system: blocks of one field and of one field read
         self.field_blocks, self.read_blocks, self.min_blocks, self.P = {}, {}, {}, {}
@@ -363,9 +366,9 @@ class BatchDecoder:
             if cnt > sset.finder.cap:
                 raise RuntimeError(f"{s}: {cnt} pulse edges exceed the search capacity {sset.finder.cap}")
             pulses[s] = (sset.finder.h_edges.numpy()[:cnt], sset.finder.h_means.numpy()[:cnt // 2])
+        todo = []
         for ln in st["lanes"]:
             if facts_for is not None and ln.idx not in facts_for:
-                per_sys[ln.system].append(ln)
                 continue
             rf = self.rfs[ln.system]
             sset = st["sys"][ln.system]
@@ -387,16 +390,21 @@ class BatchDecoder:
             if facts_for is None:
                 ln.prev_at_start = ln.prev
                 ln.start_at_wave = ln.start
-            view = sset.buffers["demod_05"][lo:hi]
             fk = None if facts_for is None else facts_for[ln.idx]
             # the walk runs in capture coordinates (stitched index of the
             # capture's block 0 = 0), as over the whole capture: its read
             # positions round to multiples of blocklen
-            locs, ll1, bad1, prev_out, nxt, vp = w.walk(view, ln.start_at_wave, ln.prev_at_start, fk,
-                                                        edges_means=(e, m), origin=b0 * S)
+            todo.append((ln, sset.buffers["demod_05"][lo:hi], fk, (e, m), b0 * S,
+                         w.first_call(ln.start_at_wave, ln.prev_at_start, fk, edges_means=(e, m))))
+        # every lane's first native walk call at once, over host threads
+        firsts = locatemod.walk_many([t[5] for t in todo], threads=self.walk_threads)
+        for (ln, view, fk, em, origin, _), first in zip(todo, firsts):
+            locs, ll1, bad1, prev_out, nxt, vp = ln.walker.walk(view, ln.start_at_wave, ln.prev_at_start, fk,
+                                                                edges_means=em, origin=origin, first=first)
             # views of the lane's own walker buffers: valid until its next walk
             ln.res = (list(locs), ll1, bad1, prev_out, nxt, vp)
             ln.valid = [j for j, c in enumerate(locs) if c.status == _lib.LD_LOC_OK]
+        for ln in st["lanes"]:
             per_sys[ln.system].append(ln)
         return per_sys
 

@@ -10,8 +10,11 @@
 // reference can index with -1.
 
 #include <algorithm>
+#include <atomic>
 #include <cmath>
 #include <cstring>
+#include <string>
+#include <thread>
 #include <vector>
 
 #include "internal.hpp"
@@ -1005,5 +1008,39 @@ extern "C" int ld_dropout_list(const int64_t* errmap, int64_t n, const double* l
     }
   }
   *count = k;
+  return LD_OK;
+}
+
+// Many independent walks (one per capture of a batch) on a few host threads:
+// each job is one ld_locate_fields call; job i's return code goes to
+// jobs[i].status.  Returns the first failing job's code (its message in
+// ld_last_error) or LD_OK.
+extern "C" int ld_locate_fields_many(ld_walk_job* jobs, int njobs, int nthreads) {
+  if (njobs < 0 || (njobs > 0 && !jobs)) {
+    set_error("ld_locate_fields_many: bad arguments");
+    return LD_EINVAL;
+  }
+  std::vector<std::string> errs((size_t)njobs);
+  std::atomic<int> next{0};
+  auto work = [&]() {
+    for (int i = next++; i < njobs; i = next++) {
+      ld_walk_job& j = jobs[i];
+      j.nfields = 0;
+      j.status = ld_locate_fields(j.p, j.edges, j.nedges, j.means, j.total, j.start, j.prev, j.facts,
+                                  j.max_fields, j.out, j.linelocs1, j.linebad, j.max_lines, j.vpulses,
+                                  j.max_vpulses, j.prev_out, &j.nfields);
+      if (j.status != LD_OK) errs[(size_t)i] = ld_last_error();
+    }
+  };
+  const int nt = std::max(1, std::min(nthreads, njobs));
+  std::vector<std::thread> pool;
+  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
+  work();
+  for (auto& t : pool) t.join();
+  for (int i = 0; i < njobs; ++i)
+    if (jobs[i].status != LD_OK) {
+      set_error("ld_locate_fields_many: job %d: %s", i, errs[(size_t)i].c_str());
+      return jobs[i].status;
+    }
   return LD_OK;
 }

@@ -16,6 +16,7 @@ sequential answer.
 """
 
 import ctypes
+import types
 
 import numpy as np
 
@@ -158,12 +159,8 @@ class WalkBuffers:
         self.vp[:] = 0          # touch the pages once
 
 
-def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None, bufs=None, at=0):
-    """ld_locate_fields: returns (locs, linelocs1, linebad, prev_out,
-    vpulses) with locs a list of _lib.FieldLoc, linelocs1 / linebad arrays
-    of shape (fields, proclines) and vpulses the fields' classified pulses
-    (fields, MAX_VPULSES).  With bufs (WalkBuffers), the results are written
-    from row `at` on and the arrays returned are views of those rows."""
+def _walk_args(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None, bufs=None, at=0):
+    """The arguments of one ld_locate_fields call (kept alive with it)."""
     P = params or locate_params(rf)
     L = proclines(rf)
     m = max(max_fields, 1)
@@ -173,23 +170,60 @@ def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, p
     if at + max_fields > bufs.cap:
         raise ValueError("walk buffers too small")
     fsz = ctypes.sizeof(_lib.FieldLoc)
-    out = ctypes.cast(ctypes.addressof(bufs.out) + at * fsz, ctypes.POINTER(_lib.FieldLoc))
-    ll, bad, vp = bufs.ll[at:], bufs.bad[at:], bufs.vp[at:]
-    po = _lib.PrevField()
-    nf = ctypes.c_int(0)
-    edges = np.ascontiguousarray(edges, dtype=np.int64)
-    means = np.ascontiguousarray(means, dtype=np.float32)
-    fa = _facts_array(facts, m)
-    _lib.check(rf.lib.ld_locate_fields(ctypes.byref(P), _ptr(edges), len(edges), _ptr(means), int(total),
-                                       float(start), ctypes.byref(prev) if prev is not None else None, fa,
-                                       max_fields, out, _ptr(ll), _ptr(bad), L, _ptr(vp), MAX_VPULSES,
-                                       ctypes.byref(po), ctypes.byref(nf)))
-    n = nf.value
+    return types.SimpleNamespace(
+        rf=rf, P=P, L=L, max_fields=int(max_fields), total=int(total), start=float(start), prev=prev, bufs=bufs,
+        out=ctypes.cast(ctypes.addressof(bufs.out) + at * fsz, ctypes.POINTER(_lib.FieldLoc)),
+        ll=bufs.ll[at:], bad=bufs.bad[at:], vp=bufs.vp[at:], po=_lib.PrevField(),
+        edges=np.ascontiguousarray(edges, dtype=np.int64), means=np.ascontiguousarray(means, dtype=np.float32),
+        fa=_facts_array(facts, m))
+
+
+def _walk_result(a, n):
+    """walk()'s results from the buffers of a finished call of n fields."""
+    out, ll, bad = a.out, a.ll, a.bad
     for k in range(n):
         if out[k].status != _lib.LD_LOC_OK:
             ll[k] = np.nan
             bad[k] = 0
-    return [_lib.FieldLoc.from_buffer_copy(out[k]) for k in range(n)], ll[:n], bad[:n], po, vp[:n]
+    return [_lib.FieldLoc.from_buffer_copy(out[k]) for k in range(n)], ll[:n], bad[:n], a.po, a.vp[:n]
+
+
+def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None, bufs=None, at=0):
+    """ld_locate_fields: returns (locs, linelocs1, linebad, prev_out,
+    vpulses) with locs a list of _lib.FieldLoc, linelocs1 / linebad arrays
+    of shape (fields, proclines) and vpulses the fields' classified pulses
+    (fields, MAX_VPULSES).  With bufs (WalkBuffers), the results are written
+    from row `at` on and the arrays returned are views of those rows."""
+    a = _walk_args(rf, edges, means, total, start, prev, facts, max_fields, params, bufs, at)
+    nf = ctypes.c_int(0)
+    _lib.check(rf.lib.ld_locate_fields(ctypes.byref(a.P), _ptr(a.edges), len(a.edges), _ptr(a.means), a.total,
+                                       a.start, ctypes.byref(prev) if prev is not None else None, a.fa,
+                                       a.max_fields, a.out, _ptr(a.ll), _ptr(a.bad), a.L, _ptr(a.vp), MAX_VPULSES,
+                                       ctypes.byref(a.po), ctypes.byref(nf)))
+    return _walk_result(a, nf.value)
+
+
+def walk_many(calls, threads=8):
+    """Several independent walk() calls (dicts of its keyword arguments) in
+    one ld_locate_fields_many call on up to `threads` host threads; returns
+    each call's walk() result, in order."""
+    if not calls:
+        return []
+    args = [_walk_args(**c) for c in calls]
+    jobs = (_lib.WalkJob * len(args))()
+    for j, a in zip(jobs, args):
+        j.p = ctypes.addressof(a.P)
+        j.edges, j.nedges = _ptr(a.edges), len(a.edges)
+        j.means, j.total, j.start = _ptr(a.means), a.total, a.start
+        j.prev = ctypes.addressof(a.prev) if a.prev is not None else None
+        j.facts = ctypes.addressof(a.fa) if a.fa is not None else None
+        j.out = ctypes.cast(a.out, ctypes.c_void_p).value
+        j.linelocs1, j.linebad, j.vpulses = _ptr(a.ll), _ptr(a.bad), _ptr(a.vp)
+        j.prev_out = ctypes.addressof(a.po)
+        j.max_fields, j.max_lines, j.max_vpulses = a.max_fields, a.L, MAX_VPULSES
+    rf = args[0].rf
+    _lib.check(rf.lib.ld_locate_fields_many(jobs, len(args), int(threads)))
+    return [_walk_result(a, j.nfields) for a, j in zip(args, jobs)]
 
 
 def recheck(rf, locs, linelocs1, linebad, vpulses, facts, prev=None, params=None):
@@ -330,13 +364,26 @@ class FieldWalker:
     def pulses(self, demod05, stream=None, bits=None):
         self.finder.launch(demod05, self.total, stream, bits=bits)
 
-    def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None, edges_means=None, origin=0):
+    def first_call(self, start=0.0, prev=None, facts=None, edges_means=None):
+        """The walk() keyword arguments of this walker's first native call
+        for walk(start, prev, facts, edges_means=...), so that a batch of
+        walkers can make those calls together (walk_many) and then each
+        continue with walk(..., first=its result)."""
+        edges, means = self.finder.result() if edges_means is None else edges_means
+        if self.max_fields > self.bufs.cap:
+            self.bufs = WalkBuffers(self.rf, 2 * self.bufs.cap + self.max_fields)
+        return dict(rf=self.rf, edges=edges, means=means, total=self.total, start=start, prev=prev, facts=facts,
+                    max_fields=self.max_fields, params=locate_params(self.rf, self.prov_delta), bufs=self.bufs,
+                    at=0)
+
+    def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None, edges_means=None, origin=0, first=None):
         """Returns (locs, linelocs1, linebad, prev_out, next start,
         vpulses); the arrays are views of this walker's buffers, valid until
         its next walk.  edges_means: the buffer's pulses (edge pairs and
         levels, numpy, in walk coordinates) when the caller searched them
         itself; origin: the walk coordinate of demod05[0] (a window of a
-        longer capture)."""
+        longer capture); first: the result of the native call first_call()
+        describes, already made."""
         rf = self.rf
         edges, means = self.finder.result() if edges_means is None else edges_means
         P = locate_params(rf, self.prov_delta)
@@ -345,13 +392,18 @@ class FieldWalker:
         nvalid = 0
         while nvalid < self.max_fields:
             n = len(locs)
-            if n + (self.max_fields - nvalid) > B.cap:
-                old, B = B, WalkBuffers(rf, 2 * B.cap + self.max_fields)
-                B.ll[:n], B.bad[:n], B.vp[:n] = old.ll[:n], old.bad[:n], old.vp[:n]
-                self.bufs = B
-            fk = None if facts is None else facts[n:]
+            if first is not None:
+                got, _, _, po, _ = first
+                first = None
+            else:
+                if n + (self.max_fields - nvalid) > B.cap:
+                    old, B = B, WalkBuffers(rf, 2 * B.cap + self.max_fields)
+                    B.ll[:n], B.bad[:n], B.vp[:n] = old.ll[:n], old.bad[:n], old.vp[:n]
+                    self.bufs = B
+                fk = None if facts is None else facts[n:]
+                requested = self.max_fields - nvalid
+                got, _, _, po, _ = walk(rf, edges, means, self.total, start, prev, fk, requested, P, B, n)
             requested = self.max_fields - nvalid
-            got, _, _, po, _ = walk(rf, edges, means, self.total, start, prev, fk, requested, P, B, n)
             if not got:
                 break
             tail = got[-1]

@@ -112,6 +112,32 @@ def test_walk_matches_oracle(system):
     np.testing.assert_array_equal(ll2, ll)
 
 
+def test_walk_many_equals_sequential_walks():
+    """ld_locate_fields_many over several jobs on several threads (NTSC and
+    PAL, with and without facts, different start points) returns exactly
+    what each ld_locate_fields call returns alone."""
+    calls = []
+    for system in ("NTSC", "PAL"):
+        rf, orf = _rf(system)
+        p = LL.LocParams.from_rf(orf)
+        d05, _ = _capture(system)
+        edges, means = _edges_means(p, d05)
+        for start, facts in ((0.0, None), (float(len(d05) // 3), None), (0.0, [_lib.FieldFacts()] * 2)):
+            calls.append(dict(rf=rf, edges=edges, means=means, total=len(d05), start=start, facts=facts,
+                              max_fields=4))
+    want = [locate.walk(**c) for c in calls]
+    for threads in (1, 3, 16):
+        got = locate.walk_many(calls, threads=threads)
+        assert len(got) == len(want)
+        for (l1, a1, b1, p1, v1), (l2, a2, b2, p2, v2) in zip(want, got):
+            assert [bytes(c) for c in l1] == [bytes(c) for c in l2]
+            np.testing.assert_array_equal(a1, a2)
+            np.testing.assert_array_equal(b1, b2)
+            np.testing.assert_array_equal(v1, v2)
+            assert bytes(p1) == bytes(p2)
+    assert locate.walk_many([]) == []
+
+
 @pytest.mark.parametrize("system", ["NTSC", "PAL"])
 def test_prev_field_dependence(system):
     """getLine0's inputs from the previous field: a final location moved by
