@@ -615,6 +615,17 @@ typedef struct ld_walk_job {
  * Host only. */
 int ld_locate_fields_many(ld_walk_job* jobs, int njobs, int nthreads);
 
+/* ld_locate_fields_many started on host threads and returned from at once:
+ * *run gets a handle that ld_locate_fields_many_wait must receive exactly
+ * once (it joins the threads, frees the handle and returns what
+ * ld_locate_fields_many would have).  The jobs, and everything they point
+ * to, must stay valid and untouched until then.  Jobs are taken longest
+ * first (by pulse count).  No counterpart in the reference: it lets the
+ * caller queue GPU work for other captures while these walks run.  Host
+ * only. */
+int ld_locate_fields_many_start(ld_walk_job* jobs, int njobs, int nthreads, void** run);
+int ld_locate_fields_many_wait(void* run);
+
 /* Check a walk made with provisional previous-field values against the
  * measured facts (facts[k] belongs to locs[k]; prev is the state the walk
  * started from; HOST arrays as ld_locate_fields wrote them, vpulses

@@ -31,6 +31,7 @@ EXPORTS = (
     "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
     "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc", "ld_refine_burst_ntsc_chains",
     "ld_demod_batch", "ld_set_fft_variant", "ld_fft_forward", "ld_locate_fields_many",
+    "ld_locate_fields_many_start", "ld_locate_fields_many_wait",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -217,6 +218,8 @@ def _declare(lib):
                                      P(FieldFacts), c_int, P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int,
                                      P(PrevField), P(c_int)]),
         "ld_locate_fields_many": (c_int, [P(WalkJob), c_int, c_int]),
+        "ld_locate_fields_many_start": (c_int, [P(WalkJob), c_int, c_int, P(c_vp)]),
+        "ld_locate_fields_many_wait": (c_int, [c_vp]),
         "ld_skip_score": (c_int, [P(LocateParams), P(c_dbl)]),
         "ld_unpack_10bit": (c_int, [c_vp, c_vp, c_i64, c_i64, c_i64, c_vp, c_vp]),
         "ld_locate_recheck": (c_int, [P(LocateParams), P(FieldLoc), c_vp, c_vp, c_int, c_vp, c_int, c_int,

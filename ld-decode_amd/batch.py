@@ -353,13 +353,13 @@ class BatchDecoder:
         st["wave"] = k
 
     # ------------------------------------------------------------ host walk
-    def _walk(self, k, facts_for=None):
-        """Walk every lane of wave k (after its pulses arrive).  facts_for:
+    def _walk_start(self, k, facts_for=None):
+        """Start the native walks of wave k's lanes (after its pulses
+        arrive) on host threads; _walk_finish collects them.  facts_for:
         {lane idx: measured facts per visited field} for a repeated walk."""
         st = self.sets[k % 2]
         st["pulses_done"].synchronize()
         S = self.S
-        per_sys = {s: [] for s in st["sys"]}
         pulses = {}
         for s, sset in st["sys"].items():
             cnt = int(sset.finder.h_count[0])
@@ -394,65 +394,77 @@ class BatchDecoder:
             # the walk runs in capture coordinates (stitched index of the
             # capture's block 0 = 0), as over the whole capture: its read
             # positions round to multiples of blocklen
-            todo.append((ln, sset.buffers["demod_05"][lo:hi], fk, (e, m), b0 * S,
-                         w.first_call(ln.start_at_wave, ln.prev_at_start, fk, edges_means=(e, m))))
+            call = w.first_call(ln.start_at_wave, ln.prev_at_start, fk, edges_means=(e, m),
+                                params=locatemod.locate_params_from(self.P[ln.system], ln.prov_delta))
+            todo.append((ln, sset.buffers["demod_05"][lo:hi], fk, (e, m), b0 * S, call))
         # every lane's first native walk call at once, over host threads
-        firsts = locatemod.walk_many([t[5] for t in todo], threads=self.walk_threads)
-        for (ln, view, fk, em, origin, _), first in zip(todo, firsts):
-            locs, ll1, bad1, prev_out, nxt, vp = ln.walker.walk(view, ln.start_at_wave, ln.prev_at_start, fk,
-                                                                edges_means=em, origin=origin, first=first)
-            # views of the lane's own walker buffers: valid until its next walk
-            ln.res = (list(locs), ll1, bad1, prev_out, nxt, vp)
-            ln.valid = [j for j, c in enumerate(locs) if c.status == _lib.LD_LOC_OK]
+        return todo, locatemod.PendingWalk([t[5] for t in todo], threads=self.walk_threads)
+
+    def _walk_finish(self, k, pending):
+        st = self.sets[k % 2]
+        todo, walks = pending
+        per_sys = {s: [] for s in st["sys"]}
+        for (ln, view, fk, em, origin, _), (args, nf) in zip(todo, walks.wait()):
+            # the records and arrays are views of the lane's own walker
+            # buffers: valid until its next walk
+            ln.res = ln.walker.walk_records(view, ln.start_at_wave, ln.prev_at_start, fk, args, nf,
+                                            edges_means=em, origin=origin)
+            ln.valid = np.flatnonzero(ln.res[0]["status"] == _lib.LD_LOC_OK)
         for ln in st["lanes"]:
             per_sys[ln.system].append(ln)
         return per_sys
 
+    def _walk(self, k, facts_for=None):
+        return self._walk_finish(k, self._walk_start(k, facts_for))
+
     def _fields(self, st, s, lanes):
-        """The system's field plan and line arrays from its lanes' walks."""
+        """The system's field plan and line arrays from its lanes' walks
+        (column-wise over each lane's valid fields)."""
         sset = st["sys"][s]
         rf = self.rfs[s]
         S, L = self.S, sset.L
         pal = s == "PAL"
-        cols = {k: [] for k in ("demod_offset", "demod_len", "linecount", "first_field", "lineoffset", "chain",
-                                "lane")}
+        parts = {k: [] for k in ("demod_offset", "demod_len", "linecount", "first_field", "chain", "lane")}
         lls, bads = [], []
         chain = 0
         row = 0
         for ln in lanes:
-            locs, ll1, bad1, _, _, _ = ln.res
+            locs, ll1, bad1 = ln.res[:3]
             b0, _, slot0 = st["geom"][ln.idx]
             ln.row0 = row
-            if not ln.valid:
+            v = ln.valid
+            if not len(v):
                 continue
             base = (slot0 - b0) * S
-            for j in ln.valid:
-                c = locs[j]
-                first = int(c.is_first_field)
-                cols["demod_offset"].append(base + int(c.startloc))
-                cols["demod_len"].append(min(int(c.buflen), sset.total - base - int(c.startloc)))
-                cols["linecount"].append(int(c.linecount))
-                cols["first_field"].append(first)
-                cols["lineoffset"].append((2 if first else 3) if pal else 0)
-                cols["chain"].append(chain)
-                cols["lane"].append(ln.idx)
-            row += len(ln.valid)
-            lls.append(ll1[ln.valid])
-            bads.append(fieldmod.compute_deriv_error_rows(ll1[ln.valid], bad1[ln.valid].astype(bool)))
+            sl = locs["startloc"][v]
+            parts["demod_offset"].append(base + sl)
+            parts["demod_len"].append(np.minimum(locs["buflen"][v], sset.total - base - sl))
+            parts["linecount"].append(locs["linecount"][v])
+            parts["first_field"].append(locs["is_first_field"][v])
+            parts["chain"].append(np.full(len(v), chain, dtype=np.int32))
+            parts["lane"].append(np.full(len(v), ln.idx, dtype=np.int64))
+            row += len(v)
+            lls.append(ll1[v])
+            bads.append(bad1[v])
             chain += 1
         if row > sset.max_fields:
             raise RuntimeError(f"{s}: {row} fields exceed the set's {sset.max_fields}")
+        cols = {k: (np.concatenate(v) if v else np.zeros(0, dtype=np.int64)) for k, v in parts.items()}
+        first = cols["first_field"]
+        cols["lineoffset"] = np.where(first != 0, 2, 3).astype(np.int32) if pal else np.zeros(row, np.int32)
         r = np.arange(row, dtype=np.int64)
-        cols.update(nlines=[L] * row, linesout=[sset.linesout] * row, outwidth=[sset.W] * row,
-                    inlinelen=[float(rf.linelen)] * row, lineloc_offset=(r * L).tolist(),
-                    out_offset=(r * (sset.linesout * sset.W)).tolist())
-        fields = fieldmod.plan_from_columns(cols)
+        cols.update(nlines=np.full(row, L, np.int32), linesout=np.full(row, sset.linesout, np.int32),
+                    outwidth=np.full(row, sset.W, np.int32), inlinelen=np.full(row, float(rf.linelen)),
+                    lineloc_offset=r * L, out_offset=r * (sset.linesout * sset.W))
+        fields = fieldmod.ColumnPlan(cols)
         sset.fields = fields
         sset.nchains = chain
         if row:
             n = row * L
-            sset.ll_host[:n].copy_(self.torch.from_numpy(np.concatenate(lls).reshape(-1)))
-            sset.bad_host[:n].copy_(self.torch.from_numpy(np.concatenate(bads).reshape(-1).astype(np.uint8)))
+            ll = np.concatenate(lls)
+            bad = fieldmod.compute_deriv_error_rows(ll, np.concatenate(bads).astype(bool))
+            sset.ll_host.numpy()[:n] = ll.reshape(-1)
+            sset.bad_host.numpy()[:n] = bad.reshape(-1)
         return fields
 
     # ------------------------------------------------------------ stage B
@@ -469,7 +481,7 @@ class BatchDecoder:
         sset.bad1[:n].copy_(sset.bad_host[:n], non_blocking=True)
         fieldmod.refine_linelocs_hsync_fields(rf, sset.buffers["demod_05"], fields, sset.ll1[:n], sset.bad1[:n],
                                              out=(sset.ll2, sset.bad2), workspace=sset.hs_ws)
-        seeds = [ln.phase_seed for ln in lanes if ln.valid] if s == "NTSC" else None
+        seeds = [ln.phase_seed for ln in lanes if len(ln.valid)] if s == "NTSC" else None
         fieldmod.refine_lines_fields(rf, sset.buffers, fields, sset.ll2[:n], sset.bad2[:n],
                                      out=(sset.ll, sset.bad, sset.phase, sset.fpid), workspace=sset.ln_ws,
                                      chain_seeds=seeds)
@@ -504,9 +516,11 @@ class BatchDecoder:
 
     # ------------------------------------------------------------ verify
     def _facts(self, sset):
+        """The measured facts of the set's fields (a FACTS_DTYPE copy of the
+        pinned buffer, which the next use of the set overwrites)."""
         n = len(sset.fields)
-        raw = sset.facts_host[:n * ctypes.sizeof(_lib.FieldFacts)].numpy()
-        return list((_lib.FieldFacts * n).from_buffer_copy(raw.tobytes())) if n else []
+        raw = sset.facts_host[:n * locatemod.FACTS_DTYPE.itemsize].numpy()
+        return raw.view(locatemod.FACTS_DTYPE).copy()
 
     def _verify(self, k, max_rounds=4):
         """Check wave k's walks against its measured facts (re-walking and
@@ -524,18 +538,17 @@ class BatchDecoder:
                 rf = self.rfs[s]
                 for ln in st["per_sys"][s]:
                     locs, ll1, bad1, _, nxt0, vp = ln.res
-                    facts = [_lib.FieldFacts() for _ in locs]
-                    for r, j in enumerate(ln.valid):
-                        facts[j] = measured[ln.row0 + r]
+                    facts = np.zeros(len(locs), locatemod.FACTS_DTYPE)
+                    facts[ln.valid] = measured[ln.row0:ln.row0 + len(ln.valid)]
                     ln.facts = facts
                     self._tune(ln, measured)
-                    P = locatemod.locate_params(rf, ln.prov_delta)
-                    first, checked = locatemod.recheck(rf, locs, ll1, bad1, vp, facts, ln.prev_at_start, P)
-                    if first < 0:
-                        ln.res = (checked,) + ln.res[1:]
-                    else:
-                        changed.setdefault(s, {})[ln.idx] = facts[:first] + [_lib.FieldFacts()
-                                                                            for _ in facts[first:]]
+                    P = locatemod.locate_params_from(self.P[s], ln.prov_delta)
+                    # the records are checked (and updated) in place
+                    first = locatemod.recheck_records(rf, locs, ll1, bad1, vp, facts, ln.prev_at_start, P)
+                    if first >= 0:
+                        fk = facts.copy()
+                        fk[first:] = 0
+                        changed.setdefault(s, {})[ln.idx] = fk
             if not changed:
                 break
             if rounds == max_rounds:
@@ -560,45 +573,63 @@ class BatchDecoder:
             rf = self.rfs[s]
             for ln in st["per_sys"][s]:
                 locs, ll1, bad1, prev_out, nxt, vp = ln.res
-                P = locatemod.locate_params(rf, ln.prov_delta)
-                if locs:
-                    written = prev_out.fields_written - (1 if locs[-1].status == _lib.LD_LOC_OK else 0)
-                    ln.prev = locatemod._prev_from(locs[-1], ll1[-1], rf, P, written, ln.facts[-1])
-                if ln.valid and s == "NTSC":
+                P = locatemod.locate_params_from(self.P[s], ln.prov_delta)
+                if len(locs):
+                    last = _lib.FieldLoc.from_buffer_copy(locs[-1:].tobytes())
+                    written = prev_out.fields_written - (1 if last.status == _lib.LD_LOC_OK else 0)
+                    ln.prev = locatemod._prev_from(last, ll1[-1], rf, P, written,
+                                                   _lib.FieldFacts.from_buffer_copy(ln.facts[-1:].tobytes()))
+                if len(ln.valid) and s == "NTSC":
                     ln.phase_seed = float(phases[ln.row0 + len(ln.valid) - 1])
                 ln.fields_done += len(ln.valid)
                 self.fields_total += len(ln.valid)
-                if nxt <= ln.start and not ln.valid:
+                if nxt <= ln.start and not len(ln.valid):
                     ln.done = True            # no progress: the capture's end
                 ln.start = nxt
         return moved
 
     def _tune(self, ln, measured):
-        rf = self.rfs[ln.system]
+        """The lane's provisional line-count offset: the median over its
+        valid fields of the measured final position of the linecount line
+        minus its linelocs1 entry."""
+        v = ln.valid
+        if not len(v):
+            return
         locs, ll1 = ln.res[0], ln.res[1]
-        d = []
-        for r, j in enumerate(ln.valid):
-            c = locs[j]
-            lc = (312 if c.is_first_field else 313) if rf.system == "PAL" else c.linecount
-            f = measured[ln.row0 + r]
-            if f.known and np.isfinite(f.ll_linecount):
-                d.append(f.ll_linecount - ll1[j][lc])
-        if d:
+        if ln.system == "PAL":
+            lc = np.where(locs["is_first_field"][v] != 0, 312, 313)
+        else:
+            lc = locs["linecount"][v]
+        f = measured[ln.row0:ln.row0 + len(v)]
+        ok = (f["known"] != 0) & np.isfinite(f["ll_linecount"])
+        if np.any(ok):
+            d = f["ll_linecount"][ok] - ll1[v[ok], lc[ok]]
             ln.prov_delta = float(np.median(d))
 
     # ------------------------------------------------------------ driver
-    def waves_iter(self, max_waves=None):
+    def waves_iter(self, max_waves=None, offset=0):
         """The decode as a generator that yields before each call that waits
         for the GPU (a wave's pulses before its walk, its facts before its
-        check), so that several decoders can take turns on the host while
-        the GPU works on the others' queued stages (run_interleaved)."""
+        check) and while its walks run on host threads, so that several
+        decoders can take turns on the host while the GPU works on the
+        others' queued stages (run_interleaved).  Each wave takes three
+        turns (start the walks; collect them, queue stage B and the next
+        demod; check the walks); `offset` empty turns at the start stagger
+        decoders taking turns, so that one's walks run during another's
+        check."""
         k = 0
         if not self._plan(0):
             return
         self._stage_a(0)
+        for _ in range(offset):
+            yield
         while True:
             yield
-            per_sys = self._walk(k)
+            pending = self._walk_start(k)
+            # the walks run on host threads while the other decoders take
+            # their turns
+            yield
+            per_sys = self._walk_finish(k, pending)
             self._queue_stage_b(k, per_sys)
             if self.on_wave is not None:
                 self.on_wave(self._info(k))
@@ -661,7 +692,7 @@ def run_interleaved(decoders):
     """Run several BatchDecoders (each with its own streams and buffers)
     taking turns on the host: while one waits for or walks its wave, the
     GPU runs the stages the others have queued.  Returns the fields decoded."""
-    its = [d.waves_iter() for d in decoders]
+    its = [d.waves_iter(offset=i % 3 if len(decoders) > 1 else 0) for i, d in enumerate(decoders)]
     while its:
         nxt = []
         for it in its:

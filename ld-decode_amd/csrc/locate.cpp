@@ -1015,32 +1015,71 @@ extern "C" int ld_dropout_list(const int64_t* errmap, int64_t n, const double* l
 // each job is one ld_locate_fields call; job i's return code goes to
 // jobs[i].status.  Returns the first failing job's code (its message in
 // ld_last_error) or LD_OK.
-extern "C" int ld_locate_fields_many(ld_walk_job* jobs, int njobs, int nthreads) {
-  if (njobs < 0 || (njobs > 0 && !jobs)) {
-    set_error("ld_locate_fields_many: bad arguments");
-    return LD_EINVAL;
-  }
-  std::vector<std::string> errs((size_t)njobs);
+// The walks of a batch on host threads, longest job (most pulses) first so
+// that the last threads to finish do not start a long walk late.
+struct WalkRun {
+  ld_walk_job* jobs = nullptr;
+  int njobs = 0;
+  std::vector<int> order;
+  std::vector<std::string> errs;
   std::atomic<int> next{0};
-  auto work = [&]() {
+  std::vector<std::thread> pool;
+  void work() {
     for (int i = next++; i < njobs; i = next++) {
-      ld_walk_job& j = jobs[i];
+      ld_walk_job& j = jobs[order[(size_t)i]];
       j.nfields = 0;
       j.status = ld_locate_fields(j.p, j.edges, j.nedges, j.means, j.total, j.start, j.prev, j.facts,
                                   j.max_fields, j.out, j.linelocs1, j.linebad, j.max_lines, j.vpulses,
                                   j.max_vpulses, j.prev_out, &j.nfields);
-      if (j.status != LD_OK) errs[(size_t)i] = ld_last_error();
+      if (j.status != LD_OK) errs[(size_t)order[(size_t)i]] = ld_last_error();
     }
-  };
+  }
+};
+
+extern "C" int ld_locate_fields_many_start(ld_walk_job* jobs, int njobs, int nthreads, void** run) {
+  if (njobs < 0 || (njobs > 0 && !jobs) || !run) {
+    set_error("ld_locate_fields_many_start: bad arguments");
+    return LD_EINVAL;
+  }
+  WalkRun* r = new WalkRun;
+  r->jobs = jobs;
+  r->njobs = njobs;
+  r->errs.resize((size_t)njobs);
+  r->order.resize((size_t)njobs);
+  for (int i = 0; i < njobs; ++i) r->order[(size_t)i] = i;
+  std::stable_sort(r->order.begin(), r->order.end(),
+                   [jobs](int a, int b) { return jobs[a].nedges > jobs[b].nedges; });
   const int nt = std::max(1, std::min(nthreads, njobs));
-  std::vector<std::thread> pool;
-  for (int t = 1; t < nt; ++t) pool.emplace_back(work);
-  work();
-  for (auto& t : pool) t.join();
-  for (int i = 0; i < njobs; ++i)
-    if (jobs[i].status != LD_OK) {
-      set_error("ld_locate_fields_many: job %d: %s", i, errs[(size_t)i].c_str());
-      return jobs[i].status;
-    }
+  try {
+    for (int t = 0; t < nt; ++t) r->pool.emplace_back([r] { r->work(); });
+  } catch (...) {
+    r->work();          // no more threads: the calling thread finishes the rest
+  }
+  *run = r;
   return LD_OK;
+}
+
+extern "C" int ld_locate_fields_many_wait(void* run) {
+  if (!run) {
+    set_error("ld_locate_fields_many_wait: no run");
+    return LD_EINVAL;
+  }
+  WalkRun* r = static_cast<WalkRun*>(run);
+  for (auto& t : r->pool) t.join();
+  int rc = LD_OK;
+  for (int i = 0; i < r->njobs; ++i)
+    if (r->jobs[i].status != LD_OK) {
+      set_error("ld_locate_fields_many: job %d: %s", i, r->errs[(size_t)i].c_str());
+      rc = r->jobs[i].status;
+      break;
+    }
+  delete r;
+  return rc;
+}
+
+extern "C" int ld_locate_fields_many(ld_walk_job* jobs, int njobs, int nthreads) {
+  void* run = nullptr;
+  const int rc = ld_locate_fields_many_start(jobs, njobs, nthreads, &run);
+  if (rc != LD_OK) return rc;
+  return ld_locate_fields_many_wait(run);
 }

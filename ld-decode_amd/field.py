@@ -115,6 +115,44 @@ def _columns_array(ctype, n, cols):
     return (ctype * n).from_buffer(a)
 
 
+class ColumnPlan:
+    """A pass's fields held as columns (numpy arrays of equal length): the C
+    struct arrays are built from the columns at once, and the per-field
+    dicts only when something iterates or indexes the plan."""
+
+    def __init__(self, cols):
+        self.cols = {k: np.asarray(v) for k, v in cols.items()}
+        keys = list(self.cols)
+        self.n = len(self.cols[keys[0]]) if keys else 0
+        self._dicts = None
+        self._arrays = {_lib.TbcField: _columns_array(_lib.TbcField, self.n, self.cols),
+                        _lib.LineField: _columns_array(_lib.LineField, self.n, self.cols)}
+        self.total_out = None
+        if self.n:
+            c = self.cols
+            self.total_out = int(np.max(c["out_offset"] + c["linesout"].astype(np.int64) * c["outwidth"]))
+
+    def struct_array(self, ctype, build):
+        if ctype not in self._arrays:
+            self._arrays[ctype] = build(self)
+        return self._arrays[ctype]
+
+    def _rows(self):
+        if self._dicts is None:
+            keys = list(self.cols)
+            self._dicts = [dict(zip(keys, vals)) for vals in zip(*(self.cols[k].tolist() for k in keys))]
+        return self._dicts
+
+    def __len__(self):
+        return self.n
+
+    def __iter__(self):
+        return iter(self._rows())
+
+    def __getitem__(self, i):
+        return self._rows()[i]
+
+
 def plan_from_columns(cols):
     """A FieldPlan from per-key columns (equal-length sequences): the field
     dicts plus the TBC and line struct arrays built column-wise at once."""
@@ -157,7 +195,7 @@ def _build_lines(fields):
 
 
 def _field_array(fields):
-    if isinstance(fields, FieldPlan):
+    if hasattr(fields, "struct_array"):
         return fields.struct_array(_lib.TbcField, _build_tbc)
     return _build_tbc(fields)
 
@@ -236,7 +274,7 @@ def line_params(rf):
 
 
 def _line_fields(fields):
-    if isinstance(fields, FieldPlan):
+    if hasattr(fields, "struct_array"):
         return fields.struct_array(_lib.LineField, _build_lines)
     return _build_lines(fields)
 

@@ -37,6 +37,10 @@ def readlen(rf):
     return ((rf.linelen * 350) // 16384) * 16384
 
 
+FIELDLOC_DTYPE = np.dtype(_lib.FieldLoc)
+FACTS_DTYPE = np.dtype(_lib.FieldFacts)
+
+
 def locate_params(rf, prov_delta=0.0):
     sp, dp = rf.SysParams, rf.DecoderParams
     p = _lib.LocateParams()
@@ -56,6 +60,15 @@ def locate_params(rf, prov_delta=0.0):
     p.blockcut = int(rf.blockcut)
     p.block_stride = int(rf.block_stride)
     p.readlen = int(readlen(rf))
+    return p
+
+
+def locate_params_from(base, prov_delta):
+    """A copy of `base` (locate_params of the same decoder) with prov_delta
+    set: the per-lane form of locate_params without reading the decoder's
+    tables again."""
+    p = _lib.LocateParams.from_buffer_copy(base)
+    p.prov_delta = float(prov_delta)
     return p
 
 
@@ -135,6 +148,11 @@ def _facts_array(facts, n):
     """n FieldFacts: the given ones first, the rest unknown."""
     if facts is None:
         return None
+    if isinstance(facts, np.ndarray):
+        a = np.zeros(max(n, 1), FACTS_DTYPE)
+        k = min(len(facts), n)
+        a[:k] = facts[:k]
+        return (_lib.FieldFacts * max(n, 1)).from_buffer(a)
     arr = (_lib.FieldFacts * max(n, 1))()
     for i, f in enumerate(list(facts)[:n]):
         arr[i] = f
@@ -157,6 +175,7 @@ class WalkBuffers:
         self.bad = np.empty((self.cap, L), dtype=np.uint8)
         self.vp = np.empty((self.cap, MAX_VPULSES), dtype=VPULSE_DTYPE)
         self.vp[:] = 0          # touch the pages once
+        self.out_np = np.frombuffer(self.out, dtype=FIELDLOC_DTYPE)    # a view of out
 
 
 def _walk_args(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, params=None, bufs=None, at=0):
@@ -203,27 +222,50 @@ def walk(rf, edges, means, total, start, prev=None, facts=None, max_fields=64, p
     return _walk_result(a, nf.value)
 
 
+class PendingWalk:
+    """Several independent walk() calls (dicts of its keyword arguments)
+    running in one ld_locate_fields_many_start call on up to `threads` host
+    threads.  The calls' arguments stay referenced here until wait()."""
+
+    def __init__(self, calls, threads=8):
+        self.args = [_walk_args(**c) for c in calls]
+        self.jobs = jobs = (_lib.WalkJob * max(len(self.args), 1))()
+        for j, a in zip(jobs, self.args):
+            j.p = ctypes.addressof(a.P)
+            j.edges, j.nedges = _ptr(a.edges), len(a.edges)
+            j.means, j.total, j.start = _ptr(a.means), a.total, a.start
+            j.prev = ctypes.addressof(a.prev) if a.prev is not None else None
+            j.facts = ctypes.addressof(a.fa) if a.fa is not None else None
+            j.out = ctypes.cast(a.out, ctypes.c_void_p).value
+            j.linelocs1, j.linebad, j.vpulses = _ptr(a.ll), _ptr(a.bad), _ptr(a.vp)
+            j.prev_out = ctypes.addressof(a.po)
+            j.max_fields, j.max_lines, j.max_vpulses = a.max_fields, a.L, MAX_VPULSES
+        self.run = None
+        if self.args:
+            run = ctypes.c_void_p()
+            _lib.check(self.args[0].rf.lib.ld_locate_fields_many_start(jobs, len(self.args), int(threads),
+                                                                      ctypes.byref(run)))
+            self.run = run
+
+    def wait(self):
+        """Join the walks; returns [(walk arguments, fields visited)] in call order."""
+        if self.run is not None:
+            run, self.run = self.run, None
+            _lib.check(self.args[0].rf.lib.ld_locate_fields_many_wait(run))
+        return [(a, j.nfields) for a, j in zip(self.args, self.jobs)]
+
+    def __del__(self):
+        if getattr(self, "run", None) is not None:   # never leave the threads running on freed arguments
+            self.args[0].rf.lib.ld_locate_fields_many_wait(self.run)
+
+
 def walk_many(calls, threads=8):
     """Several independent walk() calls (dicts of its keyword arguments) in
     one ld_locate_fields_many call on up to `threads` host threads; returns
     each call's walk() result, in order."""
     if not calls:
         return []
-    args = [_walk_args(**c) for c in calls]
-    jobs = (_lib.WalkJob * len(args))()
-    for j, a in zip(jobs, args):
-        j.p = ctypes.addressof(a.P)
-        j.edges, j.nedges = _ptr(a.edges), len(a.edges)
-        j.means, j.total, j.start = _ptr(a.means), a.total, a.start
-        j.prev = ctypes.addressof(a.prev) if a.prev is not None else None
-        j.facts = ctypes.addressof(a.fa) if a.fa is not None else None
-        j.out = ctypes.cast(a.out, ctypes.c_void_p).value
-        j.linelocs1, j.linebad, j.vpulses = _ptr(a.ll), _ptr(a.bad), _ptr(a.vp)
-        j.prev_out = ctypes.addressof(a.po)
-        j.max_fields, j.max_lines, j.max_vpulses = a.max_fields, a.L, MAX_VPULSES
-    rf = args[0].rf
-    _lib.check(rf.lib.ld_locate_fields_many(jobs, len(args), int(threads)))
-    return [_walk_result(a, j.nfields) for a, j in zip(args, jobs)]
+    return [_walk_result(a, n) for a, n in PendingWalk(calls, threads).wait()]
 
 
 def recheck(rf, locs, linelocs1, linebad, vpulses, facts, prev=None, params=None):
@@ -244,6 +286,24 @@ def recheck(rf, locs, linelocs1, linebad, vpulses, facts, prev=None, params=None
                                         vp.shape[1], n, ctypes.byref(prev) if prev is not None else None, fa,
                                         ctypes.byref(first)))
     return first.value, list(arr)
+
+
+def recheck_records(rf, locs, linelocs1, linebad, vpulses, facts, prev, params):
+    """recheck() on the walk's own records: locs a contiguous FIELDLOC_DTYPE
+    array (updated in place), facts a FACTS_DTYPE array of the same length.
+    Returns the first changed field or -1."""
+    n = len(locs)
+    if n == 0:
+        return -1
+    L = linelocs1.shape[1]
+    assert locs.flags.c_contiguous and linelocs1.flags.c_contiguous and linebad.flags.c_contiguous
+    assert vpulses.flags.c_contiguous and facts.flags.c_contiguous and len(facts) == n
+    first = ctypes.c_int(-1)
+    _lib.check(rf.lib.ld_locate_recheck(
+        ctypes.byref(params), ctypes.cast(locs.ctypes.data, ctypes.POINTER(_lib.FieldLoc)), _ptr(linelocs1),
+        _ptr(linebad), L, _ptr(vpulses), vpulses.shape[1], n, ctypes.byref(prev) if prev is not None else None,
+        ctypes.cast(facts.ctypes.data, ctypes.POINTER(_lib.FieldFacts)), ctypes.byref(first)))
+    return first.value
 
 
 def window_pulses(rf, edges, startloc, buflen):
@@ -364,7 +424,7 @@ class FieldWalker:
     def pulses(self, demod05, stream=None, bits=None):
         self.finder.launch(demod05, self.total, stream, bits=bits)
 
-    def first_call(self, start=0.0, prev=None, facts=None, edges_means=None):
+    def first_call(self, start=0.0, prev=None, facts=None, edges_means=None, params=None):
         """The walk() keyword arguments of this walker's first native call
         for walk(start, prev, facts, edges_means=...), so that a batch of
         walkers can make those calls together (walk_many) and then each
@@ -373,8 +433,37 @@ class FieldWalker:
         if self.max_fields > self.bufs.cap:
             self.bufs = WalkBuffers(self.rf, 2 * self.bufs.cap + self.max_fields)
         return dict(rf=self.rf, edges=edges, means=means, total=self.total, start=start, prev=prev, facts=facts,
-                    max_fields=self.max_fields, params=locate_params(self.rf, self.prov_delta), bufs=self.bufs,
-                    at=0)
+                    max_fields=self.max_fields, params=params or locate_params(self.rf, self.prov_delta),
+                    bufs=self.bufs, at=0)
+
+    def walk_records(self, demod05, start, prev, facts, first_args, nfields, stream=None, edges_means=None,
+                     origin=0):
+        """walk() continued from its first native call, made elsewhere (a
+        PendingWalk of first_call()'s arguments: `first_args` and the
+        `nfields` it visited).  Returns (records, linelocs1, linebad,
+        prev_out, next start, vpulses) with records a FIELDLOC_DTYPE view of
+        this walker's buffers (valid, like the arrays, until its next walk).
+        A walk the first call completes takes no per-field Python work."""
+        B = first_args.bufs
+        nf = int(nfields)
+        recs = B.out_np[:nf]
+        if nf == 0:
+            return recs, B.ll[:0], B.bad[:0], prev, start, B.vp[:0]
+        status = recs["status"]
+        tail = int(status[-1])
+        ok = status == _lib.LD_LOC_OK
+        if tail not in (_lib.LD_LOC_NEED_LEVELS, _lib.LD_LOC_NEED_IRE0) and \
+                (nf < self.max_fields or int(np.count_nonzero(ok)) >= self.max_fields):
+            ll, bad = B.ll[:nf], B.bad[:nf]
+            ll[~ok] = np.nan
+            bad[~ok] = 0
+            return recs, ll, bad, first_args.po, float(recs["start"][-1] + recs["offset"][-1]), B.vp[:nf]
+        locs, ll, bad, pv, nxt, vp = self.walk(demod05, start, prev, facts, stream, edges_means, origin,
+                                               first=_walk_result(first_args, nf))
+        out = self.bufs.out
+        for k, c in enumerate(locs):
+            out[k] = c
+        return self.bufs.out_np[:len(locs)], ll, bad, pv, nxt, vp
 
     def walk(self, demod05, start=0.0, prev=None, facts=None, stream=None, edges_means=None, origin=0, first=None):
         """Returns (locs, linelocs1, linebad, prev_out, next start,
@@ -450,6 +539,8 @@ class FieldWalker:
 
 def _prev_from(loc, ll1, rf, P, written, fact):
     """The state a resolved field leaves (what ld_locate_fields keeps)."""
+    if isinstance(fact, np.void):       # a FACTS_DTYPE record
+        fact = _lib.FieldFacts.from_buffer_copy(fact.tobytes())
     pv = _lib.PrevField()
     pv.fields_written = written + (1 if loc.status == _lib.LD_LOC_OK else 0)
     if loc.status != _lib.LD_LOC_OK:
