@@ -429,6 +429,17 @@ int ld_find_pulses_bits(ld_handle* h, const uint32_t* bits, int64_t n, int64_t* 
 int ld_pulse_means(ld_handle* h, const float* x, int64_t n, const int64_t* edges, const int64_t* count,
                    int64_t cap_pulses, double min_len, double trim, float* means, void* stream);
 
+/* The pulse list to the host: *host_count = *count (DEVICE), then
+ * host_edges[0, min(count, cap_edges)) and host_means[0, min(count,
+ * cap_edges) / 2) from the DEVICE edges and means, written by the GPU into
+ * pinned host memory (hipHostMalloc / a pinned torch tensor: LD_EHIP for
+ * memory the GPU cannot reach), so that only the edges found cross the bus,
+ * not the lists' capacity.  The host reads them after the stream's work up
+ * to this call has completed.  No counterpart in the reference, whose pulse
+ * list never leaves host memory (Field.getpulses, core.py:2242-2330). */
+int ld_pulse_export(ld_handle* h, const int64_t* edges, const int64_t* count, const float* means, int64_t cap_edges,
+                    int64_t* host_edges, float* host_means, int64_t* host_count, void* stream);
+
 /* np.mean (float32) of x[begin[i], end[i]) for `count` ranges (DEVICE x,
  * begin, end, out); an empty range gives NaN.  getpulses' black levels
  * (core.py:2296-2312). */

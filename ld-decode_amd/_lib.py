@@ -31,7 +31,7 @@ EXPORTS = (
     "ld_set_audio2", "ld_audio2_workspace_bytes", "ld_audio2_fields", "ld_audio_downscale",
     "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc", "ld_refine_burst_ntsc_chains",
     "ld_demod_batch", "ld_set_fft_variant", "ld_fft_forward", "ld_locate_fields_many",
-    "ld_locate_fields_many_start", "ld_locate_fields_many_wait",
+    "ld_locate_fields_many_start", "ld_locate_fields_many_wait", "ld_pulse_export",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -209,6 +209,7 @@ def _declare(lib):
         "ld_find_pulses_bits": (c_int, [c_vp, c_vp, c_i64, c_vp, c_i64, c_vp, c_vp, c_sz, c_vp]),
         "ld_pulse_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
         "ld_range_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp, c_vp]),
+        "ld_pulse_export": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
         "ld_field_facts_workspace_bytes": (c_sz, [c_vp, c_int]),
         "ld_field_facts_measure": (c_int, [c_vp, c_vp, c_i64, P(LineField), c_int, c_vp, c_i64, c_int, c_int, c_dbl,
                                            c_vp, c_vp, c_sz, c_vp]),

@@ -8,6 +8,8 @@
 //       the body of every long pulse, one wavefront per pulse, the samples
 //       staged in LDS
 //   k_range_means   the same mean over caller-listed ranges (black levels)
+//   k_pulse_export  the edges found and their levels (not the whole
+//       capacity) written straight into pinned host memory
 //   k_field_facts   skip_check's line medians (radix select in LDS, one
 //       workgroup per field line) and the final line location a field's
 //       successor reads
@@ -242,6 +244,25 @@ __global__ __launch_bounds__(64 * kMeanWaves) void k_pulse_means(const float* __
     }
     if (lane == 0) means[k] = r;
   }
+}
+
+// *hcount = *count; hedges[0, min(count, cap)) and hmeans[0, min(count, cap) / 2)
+// from the device lists.  The host pointers are device addresses of pinned
+// host memory; the system-scope release makes the stores visible to the host
+// before the kernel's completion is.
+__global__ __launch_bounds__(256) void k_pulse_export(const long long* __restrict__ edges,
+                                                      const long long* __restrict__ count,
+                                                      const float* __restrict__ means, long long cap,
+                                                      long long* __restrict__ hedges, float* __restrict__ hmeans,
+                                                      long long* __restrict__ hcount) {
+  const long long total = *count;
+  const long long n = total < cap ? total : cap;
+  const long long stride = (long long)gridDim.x * blockDim.x;
+  const long long t0 = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  for (long long i = t0; i < n; i += stride) hedges[i] = edges[i];
+  for (long long i = t0; i < n / 2; i += stride) hmeans[i] = means[i];
+  if (t0 == 0) *hcount = total;
+  __threadfence_system();
 }
 
 __global__ __launch_bounds__(64) void k_range_means(const float* __restrict__ x, long long n,
@@ -480,6 +501,30 @@ extern "C" int ld_pulse_means(ld_handle* h, const float* x, int64_t n, const int
   if (grid > 2048) grid = 2048;
   hipLaunchKernelGGL(k_pulse_means, dim3((unsigned)grid), dim3(64 * kMeanWaves), 0, st, x, (long long)n,
                      (const long long*)edges, (const long long*)count, (long long)cap_pulses, min_len, trim, means);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+extern "C" int ld_pulse_export(ld_handle* h, const int64_t* edges, const int64_t* count, const float* means,
+                               int64_t cap_edges, int64_t* host_edges, float* host_means, int64_t* host_count,
+                               void* stream) {
+  if (!h || !edges || !count || !means || cap_edges < 2 || !host_edges || !host_means || !host_count) {
+    set_error("ld_pulse_export: bad arguments");
+    return LD_EINVAL;
+  }
+  LD_HIP(hipSetDevice(h->device));
+  // device addresses of the pinned host buffers (an error for memory the
+  // GPU cannot reach, e.g. pageable memory)
+  void *de = nullptr, *dm = nullptr, *dc = nullptr;
+  LD_HIP(hipHostGetDevicePointer(&de, host_edges, 0));
+  LD_HIP(hipHostGetDevicePointer(&dm, host_means, 0));
+  LD_HIP(hipHostGetDevicePointer(&dc, host_count, 0));
+  hipStream_t st = (hipStream_t)stream;
+  long long grid = (cap_edges + 255) / 256;
+  if (grid > 256) grid = 256;
+  hipLaunchKernelGGL(k_pulse_export, dim3((unsigned)grid), dim3(256), 0, st, (const long long*)edges,
+                     (const long long*)count, means, (long long)cap_edges, (long long*)de, (float*)dm,
+                     (long long*)dc);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

@@ -123,13 +123,14 @@ class PulseFinder:
                                          self.count.data_ptr(), self.cap // 2,
                                          10.0 * ((rf.freq / rf.linelen) * rf.linelen),    # usectoinpx(10)
                                          float(rf.freq), self.means.data_ptr(), sp))
-        with torch.cuda.stream(st):
-            self.h_count.copy_(self.count, non_blocking=True)
-            self.h_edges.copy_(self.edges, non_blocking=True)
-            self.h_means.copy_(self.means, non_blocking=True)
-            if record:
-                self.done = torch.cuda.Event()
-                self.done.record(st)
+        # the edges found (not the whole capacity) and their levels, written
+        # by the GPU into the pinned host buffers
+        _lib.check(rf.lib.ld_pulse_export(rf.handle, self.edges.data_ptr(), self.count.data_ptr(),
+                                          self.means.data_ptr(), self.cap, self.h_edges.data_ptr(),
+                                          self.h_means.data_ptr(), self.h_count.data_ptr(), sp))
+        if record:
+            self.done = torch.cuda.Event()
+            self.done.record(st)
         self._args = (demod05, n, st, hi)
 
     def result(self):
