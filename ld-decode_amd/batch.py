@@ -303,7 +303,9 @@ class BatchDecoder:
 
     def _pulses_body(self, st):
         """Per system the pulse search over the demod's in-pulse bits, the
-        pulses' levels and their copies to pinned host memory."""
+        pulses' levels and their copies to pinned host memory.  (The two
+        systems' searches one after the other: forking the second onto a
+        side stream inside the graph made the pass slower.)"""
         cur = self.torch.cuda.current_stream(self.device)
         for s, sset in st["sys"].items():
             sset.finder.launch(sset.buffers["demod_05"], sset.total, cur,

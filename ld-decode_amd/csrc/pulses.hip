@@ -90,30 +90,46 @@ __global__ __launch_bounds__(kEdgeThreads) void k_edge_count(const float* __rest
 }
 
 // Exclusive scan of the per-tile counts in place; the total goes to *total.
+// Each thread sums a run of consecutive counts (four loads in flight), the
+// wavefronts scan their runs with shuffles, and one wavefront scans the
+// wavefront totals: two workgroup barriers.
 __global__ __launch_bounds__(kScanThreads) void k_edge_scan(long long* __restrict__ counts, long long ntiles,
                                                             long long* __restrict__ total) {
-  __shared__ long long part[kScanThreads];
-  const int t = threadIdx.x;
+  __shared__ long long wsum[kScanThreads / 64];
+  const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const long long per = (ntiles + kScanThreads - 1) / kScanThreads;
   const long long b = (long long)t * per;
   const long long e = b + per < ntiles ? b + per : ntiles;
   long long s = 0;
-  for (long long i = b; i < e; ++i) s += counts[i];
-  part[t] = s;
-  __syncthreads();
-  for (int o = 1; o < kScanThreads; o <<= 1) {
-    const long long v = t >= o ? part[t - o] : 0;
-    __syncthreads();
-    part[t] += v;
-    __syncthreads();
+  long long i = b;
+  for (; i + 4 <= e; i += 4) {
+    const long long c0 = counts[i], c1 = counts[i + 1], c2 = counts[i + 2], c3 = counts[i + 3];
+    s += (c0 + c1) + (c2 + c3);
   }
-  long long run = part[t] - s;
-  for (long long i = b; i < e; ++i) {
+  for (; i < e; ++i) s += counts[i];
+  long long inc = s;  // inclusive scan over the wavefront
+  for (int o = 1; o < 64; o <<= 1) {
+    const long long v = __shfl_up(inc, o);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) wsum[w] = inc;
+  __syncthreads();
+  if (w == 0) {
+    long long v = lane < kScanThreads / 64 ? wsum[lane] : 0;
+    for (int o = 1; o < kScanThreads / 64; o <<= 1) {
+      const long long u = __shfl_up(v, o);
+      if (lane >= o) v += u;
+    }
+    if (lane < kScanThreads / 64) wsum[lane] = v;  // inclusive wavefront offsets
+  }
+  __syncthreads();
+  long long run = inc - s + (w > 0 ? wsum[w - 1] : 0);
+  for (i = b; i < e; ++i) {
     const long long c = counts[i];
     counts[i] = run;
     run += c;
   }
-  if (t == kScanThreads - 1) *total = part[t];
+  if (t == kScanThreads - 1) *total = wsum[kScanThreads / 64 - 1];
 }
 
 __global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const uint16_t* __restrict__ masks,
@@ -215,34 +231,116 @@ __device__ float lds_np_mean(const float* s, int m) {
   return np_pairwise_sum_any<6>(at, 0, m) / (float)m;
 }
 
+// numpy's pairwise float32 sum of s[0, m) (m <= kMeanMax) by one wavefront:
+// the recursion's leaves (blocks of at most 128, at most 64 of them for
+// m <= 5000) are listed in order, lane j sums leaf j with numpy's
+// 8-accumulator block sum, and lane 0 adds the leaf sums up the recursion
+// tree in numpy's order.
+template <int D>
+__device__ void np_leaves(int off, int n, int* loff, int* llen, int& cnt) {
+  if constexpr (D == 0) {
+    loff[cnt] = off;
+    llen[cnt] = n;
+    ++cnt;
+  } else {
+    if (n <= 128) {
+      loff[cnt] = off;
+      llen[cnt] = n;
+      ++cnt;
+      return;
+    }
+    int n2 = n / 2;
+    n2 -= n2 % 8;
+    np_leaves<D - 1>(off, n2, loff, llen, cnt);
+    np_leaves<D - 1>(off + n2, n - n2, loff, llen, cnt);
+  }
+}
+template <int D>
+__device__ float np_tree(int n, const float* leaf, int& idx) {
+  if constexpr (D == 0) {
+    return leaf[idx++];
+  } else {
+    if (n <= 128) return leaf[idx++];
+    int n2 = n / 2;
+    n2 -= n2 % 8;
+    const float left = np_tree<D - 1>(n2, leaf, idx);
+    return left + np_tree<D - 1>(n - n2, leaf, idx);
+  }
+}
+
+struct MeanScratch {
+  float stage[kMeanMax];
+  float leaf[64];
+  int loff[64], llen[64];
+};
+
+// numpy's float32 mean of s->stage[0, m) by the calling wavefront; the
+// result is valid in lane 0.
+__device__ float wave_np_mean(MeanScratch* sc, int m, int lane) {
+  if (lane == 0) {
+    int cnt = 0;
+    np_leaves<6>(0, m, sc->loff, sc->llen, cnt);
+    for (int j = cnt; j < 64; ++j) sc->llen[j] = 0;
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  const int n = sc->llen[lane];
+  if (n > 0) {
+    const float* st = sc->stage;
+    auto at = [st](int i) { return st[i]; };
+    sc->leaf[lane] = np_block_sum(at, sc->loff[lane], n);
+  }
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+  __builtin_amdgcn_wave_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
+  float r = 0.0f;
+  if (lane == 0) {
+    int idx = 0;
+    r = np_tree<6>(m, sc->leaf, idx) / (float)m;
+  }
+  return r;
+}
+
+// 64 consecutive pulses per wavefront: lane i reads pulse k0 + i's edges and
+// writes NaN unless the pulse is long; the long ones are then averaged one
+// after the other by the whole wavefront.
 __global__ __launch_bounds__(64 * kMeanWaves) void k_pulse_means(const float* __restrict__ x, long long n,
                                                                  const long long* __restrict__ edges,
                                                                  const long long* __restrict__ count,
                                                                  long long cap_pulses, double min_len, double trim,
                                                                  float* __restrict__ means) {
-  __shared__ float stage[kMeanWaves][kMeanMax];
+  __shared__ MeanScratch scratch[kMeanWaves];
   const int lane = threadIdx.x & 63;
   const int w = __builtin_amdgcn_readfirstlane(threadIdx.x >> 6);
   long long np = *count / 2;
   if (np > cap_pulses) np = cap_pulses;
-  float* s = stage[w];
-  for (long long k = (long long)blockIdx.x * kMeanWaves + w; k < np; k += (long long)gridDim.x * kMeanWaves) {
-    const long long st = edges[2 * k], len = edges[2 * k + 1] - st;
-    float r = __builtin_nanf("");
-    if ((double)len > min_len && len <= kMeanMax) {
-      // int(start + trim) : int(start + len - trim), Python slice bounds
-      long long a, b;
-      np_slice((long long)((double)st + trim), (long long)((double)st + (double)len - trim), n, a, b);
-      const int m = (int)(b - a);
-      if (m > 0 && m <= kMeanMax) {
-        for (int i = lane; i < m; i += 64) s[i] = x[a + i];
-        __builtin_amdgcn_wave_barrier();
-        __builtin_amdgcn_fence(__ATOMIC_ACQ_REL, "wavefront");
-        if (lane == 0) r = lds_np_mean(s, m);
-        __builtin_amdgcn_wave_barrier();
+  MeanScratch* sc = &scratch[w];
+  const long long step = (long long)gridDim.x * kMeanWaves * 64;
+  for (long long k0 = ((long long)blockIdx.x * kMeanWaves + w) * 64; k0 < np; k0 += step) {
+    const long long k = k0 + lane;
+    long long a = 0, b = 0;
+    bool take = false;
+    if (k < np) {
+      const long long st = edges[2 * k], len = edges[2 * k + 1] - st;
+      if ((double)len > min_len && len <= kMeanMax) {
+        // int(start + trim) : int(start + len - trim), Python slice bounds
+        np_slice((long long)((double)st + trim), (long long)((double)st + (double)len - trim), n, a, b);
+        take = b - a > 0 && b - a <= kMeanMax;
       }
+      if (!take) means[k] = __builtin_nanf("");
     }
-    if (lane == 0) means[k] = r;
+    unsigned long long todo = __ballot(take);
+    while (todo) {
+      const int j = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const long long aj = __shfl(a, j), bj = __shfl(b, j);
+      const int m = (int)(bj - aj);
+      for (int i = lane; i < m; i += 64) sc->stage[i] = x[aj + i];
+      const float r = wave_np_mean(sc, m, lane);
+      if (lane == 0) means[k0 + j] = r;
+      __builtin_amdgcn_wave_barrier();
+    }
   }
 }
 
@@ -497,8 +595,8 @@ extern "C" int ld_pulse_means(ld_handle* h, const float* x, int64_t n, const int
   if (cap_pulses == 0) return LD_OK;
   hipStream_t st = (hipStream_t)stream;
   LD_HIP(hipSetDevice(h->device));
-  long long grid = (cap_pulses + kMeanWaves - 1) / kMeanWaves;
-  if (grid > 2048) grid = 2048;
+  long long grid = (cap_pulses + 64 * kMeanWaves - 1) / (64 * kMeanWaves);
+  if (grid > 1024) grid = 1024;
   hipLaunchKernelGGL(k_pulse_means, dim3((unsigned)grid), dim3(64 * kMeanWaves), 0, st, x, (long long)n,
                      (const long long*)edges, (const long long*)count, (long long)cap_pulses, min_len, trim, means);
   LD_HIP(hipGetLastError());
