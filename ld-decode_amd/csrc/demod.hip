@@ -1730,6 +1730,7 @@ void ld_destroy(ld_handle* h) {
   hipFree(h->mfma_tw);
   hipFree(h->sinc_lut);
   for (auto& f : h->filt) hipFree(f);
+  for (auto& f : h->filt_rolled) hipFree(f);
   for (auto& f : h->audio_filt) hipFree(f);
   for (auto& f : h->audio2_filt) hipFree(f);
   delete h;
@@ -1757,9 +1758,32 @@ int ld_set_filter(ld_handle* h, int id, const double* reim, int nbins, double sc
   }
   int rc = upload_complex(h, &h->filt[id], reim, nbins, scale);
   if (rc) return rc;
+  // the one-sample roll, in double: bin k times exp(+2 pi i k / N)
+  std::vector<double> rolled(2 * (size_t)nbins);
+  for (int k = 0; k < nbins; ++k) {
+    const double a = 2.0 * M_PI * (double)k / (double)h->blocklen, c = cos(a), sn = sin(a);
+    rolled[2 * k] = reim[2 * k] * c - reim[2 * k + 1] * sn;
+    rolled[2 * k + 1] = reim[2 * k] * sn + reim[2 * k + 1] * c;
+  }
+  rc = upload_complex(h, &h->filt_rolled[id], rolled.data(), nbins, scale);
+  if (rc) return rc;
   h->filt_dc[id] = dc_gain;
   h->filt_shift[id] = shift;
   return LD_OK;
+}
+
+// A c2r output's table and roll: out[i] = y[(i + cut + shift) mod N].  Where
+// cut + shift is odd, the rolled table (y advanced by one sample) with
+// shift - 1 gives the same output at an even offset, so that the kernels
+// store aligned 8-byte pairs.
+static void c2r_table(const ld_handle* h, int id, int cut, int shift, const float2*& table, int& out_shift) {
+  if (((cut + shift) & 1) != 0 && h->filt_rolled[id] != nullptr) {
+    table = h->filt_rolled[id];
+    out_shift = shift - 1;
+  } else {
+    table = h->filt[id];
+    out_shift = shift;
+  }
 }
 
 int ld_set_audio2(ld_handle* h, int ch, const double* reim, int nbins, double center_freq) {
@@ -1998,10 +2022,9 @@ static int build_demod_args(ld_handle* h, const void* input, int dtype, int64_t 
   a.mfma_g = (const u32x4_t*)h->mfma_g;
   a.mfma_tw = (const MfmaTw*)h->mfma_tw;
   a.h_rfv = h->filt[LD_FILT_RFVIDEO];
-  a.hpf.h = h->filt[LD_FILT_RFHPF];
+  c2r_table(h, LD_FILT_RFHPF, h->cut_begin, -h->video_rot, a.hpf.h, a.hpf.shift);
   a.hpf.out = out->rfhpf;
   a.hpf.dc = 0.0f;
-  a.hpf.shift = -h->video_rot;
   a.hpf.cut = h->cut_begin;
   a.hpf.len = S;
   const int ids[4] = {LD_FILT_VIDEO, LD_FILT_VIDEO05, LD_FILT_BURST, LD_FILT_PILOT};
@@ -2026,10 +2049,9 @@ static int build_demod_args(ld_handle* h, const void* input, int dtype, int64_t 
     if (!outs[i]) continue;
     if (i == 1 && out->demod_05_below) a.below_post = a.npost;
     C2rOut& o = a.post[a.npost++];
-    o.h = h->filt[ids[i]];
+    c2r_table(h, ids[i], vcut, h->filt_shift[ids[i]], o.h, o.shift);
     o.out = outs[i];
     o.dc = (float)(h->dc_offset * h->filt_dc[ids[i]]);
-    o.shift = h->filt_shift[ids[i]];
     o.cut = vcut;
     o.len = vlen;
   }

@@ -18,6 +18,10 @@ struct ld_handle {
   int sinc_folded = 0;
   int notch_lo = 0, notch_hi = 0;  // spur notch window (bins), lo == hi: off        // row p == row (65536-p) reversed for 0 < p < 65536
   float2* filt[LD_NFILT] = {};
+  // each table times exp(+2 pi i k / N): its c2r output advanced by one
+  // sample, used where a c2r output's offset is odd so that the output
+  // pairs stay 8-byte aligned
+  float2* filt_rolled[LD_NFILT] = {};
   double filt_dc[LD_NFILT] = {};
   int filt_shift[LD_NFILT] = {};
   float2* audio_filt[2] = {};
