@@ -440,6 +440,34 @@ int ld_pulse_means(ld_handle* h, const float* x, int64_t n, const int64_t* edges
 int ld_pulse_export(ld_handle* h, const int64_t* edges, const int64_t* count, const float* means, int64_t cap_edges,
                     int64_t* host_edges, float* host_means, int64_t* host_count, void* stream);
 
+/* One buffer of ld_pulse_search: ld_find_pulses_bits over `bits` (n
+ * samples), ld_pulse_means over x (DEVICE demod_05; min_len, trim as there,
+ * cap / 2 levels) and ld_pulse_export into the pinned host lists. */
+typedef struct ld_pulse_job {
+  const uint32_t* bits;
+  int64_t n;
+  const float* x;
+  int64_t* edges;       /* DEVICE, cap entries (cap even) */
+  int64_t* count;       /* DEVICE */
+  float* means;         /* DEVICE, cap / 2 entries */
+  int64_t cap;
+  double min_len;
+  double trim;
+  int64_t* host_edges;  /* pinned host, cap entries */
+  float* host_means;    /* pinned host, cap / 2 entries */
+  int64_t* host_count;  /* pinned host */
+} ld_pulse_job;
+
+/* The pulse searches of up to 4 buffers (a batch's systems) in four
+ * launches for all of them: each job gets exactly what ld_find_pulses_bits,
+ * ld_pulse_means and ld_pulse_export would give it, so the host reads each
+ * buffer's edges and levels after the stream's work up to this call.  DEVICE
+ * workspace of ld_pulse_search_workspace_bytes bytes.  No counterpart in the
+ * reference (one Field.getpulses per field, core.py:2242-2330). */
+size_t ld_pulse_search_workspace_bytes(const ld_handle* h, const ld_pulse_job* jobs, int njobs);
+int ld_pulse_search(ld_handle* h, const ld_pulse_job* jobs, int njobs, void* workspace, size_t workspace_bytes,
+                    void* stream);
+
 /* np.mean (float32) of x[begin[i], end[i]) for `count` ranges (DEVICE x,
  * begin, end, out); an empty range gives NaN.  getpulses' black levels
  * (core.py:2296-2312). */

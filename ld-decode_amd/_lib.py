@@ -32,6 +32,7 @@ EXPORTS = (
     "ld_demod_noalloc_max_bytes", "ld_demod_blocks_noalloc", "ld_refine_burst_ntsc_chains",
     "ld_demod_batch", "ld_set_fft_variant", "ld_fft_forward", "ld_locate_fields_many",
     "ld_locate_fields_many_start", "ld_locate_fields_many_wait", "ld_pulse_export",
+    "ld_pulse_search_workspace_bytes", "ld_pulse_search",
 )
 LD_LOC_OK, LD_LOC_FAILED, LD_LOC_NEED_LEVELS, LD_LOC_NEED_IRE0 = 0, 1, 2, 3
 
@@ -129,6 +130,14 @@ class VPulse(ctypes.Structure):
     _fields_ = [("start", ctypes.c_int64), ("len", ctypes.c_int32), ("kind", ctypes.c_int16), ("good", ctypes.c_int16)]
 
 
+class PulseJob(ctypes.Structure):
+    """ld_pulse_job: one buffer of ld_pulse_search."""
+    _fields_ = [("bits", ctypes.c_void_p), ("n", ctypes.c_int64), ("x", ctypes.c_void_p),
+                ("edges", ctypes.c_void_p), ("count", ctypes.c_void_p), ("means", ctypes.c_void_p),
+                ("cap", ctypes.c_int64), ("min_len", ctypes.c_double), ("trim", ctypes.c_double),
+                ("host_edges", ctypes.c_void_p), ("host_means", ctypes.c_void_p), ("host_count", ctypes.c_void_p)]
+
+
 class WalkJob(ctypes.Structure):
     """ld_walk_job: one ld_locate_fields call of ld_locate_fields_many."""
     _fields_ = [("p", ctypes.c_void_p), ("edges", ctypes.c_void_p), ("nedges", ctypes.c_int64),
@@ -210,6 +219,8 @@ def _declare(lib):
         "ld_pulse_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_i64, c_dbl, c_dbl, c_vp, c_vp]),
         "ld_range_means": (c_int, [c_vp, c_vp, c_i64, c_vp, c_vp, c_int, c_vp, c_vp]),
         "ld_pulse_export": (c_int, [c_vp, c_vp, c_vp, c_vp, c_i64, c_vp, c_vp, c_vp, c_vp]),
+        "ld_pulse_search_workspace_bytes": (c_sz, [c_vp, P(PulseJob), c_int]),
+        "ld_pulse_search": (c_int, [c_vp, P(PulseJob), c_int, c_vp, c_sz, c_vp]),
         "ld_field_facts_workspace_bytes": (c_sz, [c_vp, c_int]),
         "ld_field_facts_measure": (c_int, [c_vp, c_vp, c_i64, P(LineField), c_int, c_vp, c_i64, c_int, c_int, c_dbl,
                                            c_vp, c_vp, c_sz, c_vp]),

@@ -194,6 +194,9 @@ class BatchDecoder:
             tbl_dev = torch.empty(tbl_np.size, dtype=torch.uint8, device=self.device)
             self.sets.append({"sys": sets, "tbl_pin": tbl_pin, "tbl_dev": tbl_dev, "graph_demod": None,
                               "graph_pulses": None, "free": None, "lanes": [], "wave": -1})
+            st = self.sets[-1]
+            st["pulse_ws"] = torch.empty(locatemod.pulse_search_workspace(self._searches(st)), dtype=torch.uint8,
+                                         device=self.device)
         self.ws = torch.empty(max(1, rf0.lib.ld_demod_workspace_bytes(rf0.handle, self.table_cap)),
                               dtype=torch.uint8, device=self.device)
         self.handles = (ctypes.c_void_p * 2)(*[self.rfs[s].handle.value if s in self.rfs else None
@@ -302,14 +305,17 @@ class BatchDecoder:
                                           self.ws.data_ptr(), self.ws.numel(), ctypes.c_void_p(cur.cuda_stream)))
 
     def _pulses_body(self, st):
-        """Per system the pulse search over the demod's in-pulse bits, the
-        pulses' levels and their copies to pinned host memory.  (The two
-        systems' searches one after the other: forking the second onto a
-        side stream inside the graph made the pass slower.)"""
+        """Every system's pulse search over the demod's in-pulse bits, the
+        pulses' levels and their export to pinned host memory, in one
+        ld_pulse_search call (four launches for all systems: these kernels
+        sit between one demod launch and the next)."""
         cur = self.torch.cuda.current_stream(self.device)
-        for s, sset in st["sys"].items():
-            sset.finder.launch(sset.buffers["demod_05"], sset.total, cur,
-                               bits=(sset.buffers["demod_05_below"], sset.outputs.below_high), record=False)
+        locatemod.pulse_search(self._searches(st), st["pulse_ws"], cur)
+
+    def _searches(self, st):
+        """The set's pulse searches: every system's demod_05 and in-pulse bits."""
+        return [(sset.finder, sset.buffers["demod_05"], sset.total, sset.buffers["demod_05_below"])
+                for sset in st["sys"].values()]
 
     def _replay(self, st, name, body):
         """Run `body` as a HIP graph captured once per buffer set (its

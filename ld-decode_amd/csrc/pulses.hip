@@ -93,8 +93,8 @@ __global__ __launch_bounds__(kEdgeThreads) void k_edge_count(const float* __rest
 // Each thread sums a run of consecutive counts (four loads in flight), the
 // wavefronts scan their runs with shuffles, and one wavefront scans the
 // wavefront totals: two workgroup barriers.
-__global__ __launch_bounds__(kScanThreads) void k_edge_scan(long long* __restrict__ counts, long long ntiles,
-                                                            long long* __restrict__ total) {
+__device__ __forceinline__ void edge_scan_wg(long long* __restrict__ counts, long long ntiles,
+                                             long long* __restrict__ total) {
   __shared__ long long wsum[kScanThreads / 64];
   const int t = threadIdx.x, lane = t & 63, w = t >> 6;
   const long long per = (ntiles + kScanThreads - 1) / kScanThreads;
@@ -130,6 +130,10 @@ __global__ __launch_bounds__(kScanThreads) void k_edge_scan(long long* __restric
     run += c;
   }
   if (t == kScanThreads - 1) *total = wsum[kScanThreads / 64 - 1];
+}
+__global__ __launch_bounds__(kScanThreads) void k_edge_scan(long long* __restrict__ counts, long long ntiles,
+                                                            long long* __restrict__ total) {
+  edge_scan_wg(counts, ntiles, total);
 }
 
 __global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const uint16_t* __restrict__ masks,
@@ -184,11 +188,11 @@ __device__ __forceinline__ unsigned long long bit_edges(const uint32_t* __restri
   return e;
 }
 
-__global__ __launch_bounds__(kEdgeThreads) void k_bits_count(const uint32_t* __restrict__ bits, long long n,
-                                                             long long* __restrict__ counts) {
+__device__ __forceinline__ void bits_count_tile(const uint32_t* __restrict__ bits, long long n,
+                                                long long* __restrict__ counts, long long tile) {
   __shared__ int part[kEdgeThreads / 64];
   const int t = threadIdx.x;
-  const long long base = (long long)blockIdx.x * kBitTile + (long long)t * 64;
+  const long long base = tile * kBitTile + (long long)t * 64;
   int c = base < n ? __popcll(bit_edges(bits, n, base)) : 0;
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((t & 63) == 0) part[t >> 6] = c;
@@ -196,16 +200,16 @@ __global__ __launch_bounds__(kEdgeThreads) void k_bits_count(const uint32_t* __r
   if (t == 0) {
     long long s = 0;
     for (int w = 0; w < kEdgeThreads / 64; ++w) s += part[w];
-    counts[blockIdx.x] = s;
+    counts[tile] = s;
   }
 }
 
-__global__ __launch_bounds__(kEdgeThreads) void k_bits_write(const uint32_t* __restrict__ bits, long long n,
-                                                             const long long* __restrict__ offsets,
-                                                             long long* __restrict__ edges, long long cap) {
+__device__ __forceinline__ void bits_write_tile(const uint32_t* __restrict__ bits, long long n,
+                                                const long long* __restrict__ offsets, long long* __restrict__ edges,
+                                                long long cap, long long tile) {
   __shared__ int part[kEdgeThreads];
   const int t = threadIdx.x;
-  const long long base = (long long)blockIdx.x * kBitTile + (long long)t * 64;
+  const long long base = tile * kBitTile + (long long)t * 64;
   unsigned long long e = base < n ? bit_edges(bits, n, base) : 0ull;
   const int c = __popcll(e);
   part[t] = c;
@@ -216,13 +220,24 @@ __global__ __launch_bounds__(kEdgeThreads) void k_bits_write(const uint32_t* __r
     part[t] += v;
     __syncthreads();
   }
-  long long k = offsets[blockIdx.x] + part[t] - c;
+  long long k = offsets[tile] + part[t] - c;
   while (e) {
     const int i = __ffsll((long long)e) - 1;
     e &= e - 1;
     if (k < cap) edges[k] = base + i;
     ++k;
   }
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_bits_count(const uint32_t* __restrict__ bits, long long n,
+                                                             long long* __restrict__ counts) {
+  bits_count_tile(bits, n, counts, blockIdx.x);
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_bits_write(const uint32_t* __restrict__ bits, long long n,
+                                                             const long long* __restrict__ offsets,
+                                                             long long* __restrict__ edges, long long cap) {
+  bits_write_tile(bits, n, offsets, edges, cap, blockIdx.x);
 }
 
 // numpy's float32 mean of an LDS-staged range of at most kMeanMax samples.
@@ -346,8 +361,9 @@ __global__ __launch_bounds__(64 * kMeanWaves) void k_pulse_means(const float* __
 
 // *hcount = *count; hedges[0, min(count, cap)) and hmeans[0, min(count, cap) / 2)
 // from the device lists.  The host pointers are device addresses of pinned
-// host memory; the system-scope release makes the stores visible to the host
-// before the kernel's completion is.
+// host memory; the host reads them after waiting for an event recorded
+// behind this launch, whose completion releases the stores at system scope
+// (no fence per workgroup: a system-scope fence writes back the L2).
 __global__ __launch_bounds__(256) void k_pulse_export(const long long* __restrict__ edges,
                                                       const long long* __restrict__ count,
                                                       const float* __restrict__ means, long long cap,
@@ -360,7 +376,111 @@ __global__ __launch_bounds__(256) void k_pulse_export(const long long* __restric
   for (long long i = t0; i < n; i += stride) hedges[i] = edges[i];
   for (long long i = t0; i < n / 2; i += stride) hmeans[i] = means[i];
   if (t0 == 0) *hcount = total;
-  __threadfence_system();
+}
+
+// ------------------------------------------- all systems' searches at once
+// The pulse search of several buffers (a batch's systems) in four launches
+// instead of four or five per buffer: tile counts, one scan workgroup per
+// buffer, edge compaction, then the sync levels together with the export of
+// the list to pinned host memory.  Workgroup ranges per job come in the
+// arguments.
+constexpr int kMaxPulseJobs = 4;
+struct PulseJobDev {
+  const uint32_t* bits;
+  const float* x;
+  long long n;
+  long long* edges;
+  long long* count;
+  float* means;
+  long long cap;            // edges (even)
+  double min_len, trim;
+  long long* hedges;        // device addresses of the pinned host lists
+  float* hmeans;
+  long long* hcount;
+  long long* tiles;         // per-tile counts (workspace)
+  long long ntiles;
+};
+struct PulseSearchArgs {
+  int njobs;
+  PulseJobDev job[kMaxPulseJobs];
+  long long tile0[kMaxPulseJobs + 1];   // first tile workgroup of each job
+  long long mwg0[kMaxPulseJobs + 1];    // first means workgroup of each job
+};
+
+__device__ __forceinline__ int job_of(const long long* first, int njobs, long long b) {
+  int j = 0;
+  while (j + 1 < njobs && b >= first[j + 1]) ++j;
+  return j;
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_bits_count_multi(PulseSearchArgs a) {
+  const int j = job_of(a.tile0, a.njobs, blockIdx.x);
+  const PulseJobDev& d = a.job[j];
+  bits_count_tile(d.bits, d.n, d.tiles, (long long)blockIdx.x - a.tile0[j]);
+}
+
+__global__ __launch_bounds__(kScanThreads) void k_edge_scan_multi(PulseSearchArgs a) {
+  const PulseJobDev& d = a.job[blockIdx.x];
+  edge_scan_wg(d.tiles, d.ntiles, d.count);
+}
+
+__global__ __launch_bounds__(kEdgeThreads) void k_bits_write_multi(PulseSearchArgs a) {
+  const int j = job_of(a.tile0, a.njobs, blockIdx.x);
+  const PulseJobDev& d = a.job[j];
+  bits_write_tile(d.bits, d.n, d.tiles, d.edges, d.cap, (long long)blockIdx.x - a.tile0[j]);
+}
+
+// 64 pulses per one-wavefront workgroup (a small LDS footprint, so that the
+// workgroups find room beside other kernels'), as k_pulse_means, with each
+// edge and level also written to the host lists: edges [0, min(count, cap)), levels of the
+// pulses [0, min(count, cap) / 2), and the count.
+__global__ __launch_bounds__(64) void k_means_export_multi(PulseSearchArgs a) {
+  __shared__ MeanScratch scratch[1];
+  const int j = job_of(a.mwg0, a.njobs, blockIdx.x);
+  const PulseJobDev& d = a.job[j];
+  const int lane = threadIdx.x;
+  const long long total = *d.count;
+  const long long ne = total < d.cap ? total : d.cap;
+  const long long np = ne / 2;
+  const long long wg = (long long)blockIdx.x - a.mwg0[j];
+  if (wg == 0 && threadIdx.x == 0) *d.hcount = total;
+  MeanScratch* sc = &scratch[0];
+  const long long step = (a.mwg0[j + 1] - a.mwg0[j]) * 64;
+  for (long long k0 = wg * 64; 2 * k0 < ne; k0 += step) {
+    const long long k = k0 + lane;
+    long long lo = 0, hi = 0;
+    bool take = false;
+    if (2 * k < ne) {
+      const long long st = d.edges[2 * k];
+      d.hedges[2 * k] = st;
+      if (k < np) {
+        const long long en = d.edges[2 * k + 1], len = en - st;
+        d.hedges[2 * k + 1] = en;
+        if ((double)len > d.min_len && len <= kMeanMax) {
+          np_slice((long long)((double)st + d.trim), (long long)((double)st + (double)len - d.trim), d.n, lo, hi);
+          take = hi - lo > 0 && hi - lo <= kMeanMax;
+        }
+        if (!take) {
+          d.means[k] = __builtin_nanf("");
+          d.hmeans[k] = __builtin_nanf("");
+        }
+      }
+    }
+    unsigned long long todo = __ballot(take);
+    while (todo) {
+      const int q = __ffsll((long long)todo) - 1;
+      todo &= todo - 1;
+      const long long aq = __shfl(lo, q), bq = __shfl(hi, q);
+      const int m = (int)(bq - aq);
+      for (int i = lane; i < m; i += 64) sc->stage[i] = d.x[aq + i];
+      const float r = wave_np_mean(sc, m, lane);
+      if (lane == 0) {
+        d.means[k0 + q] = r;
+        d.hmeans[k0 + q] = r;
+      }
+      __builtin_amdgcn_wave_barrier();
+    }
+  }
 }
 
 __global__ __launch_bounds__(64) void k_range_means(const float* __restrict__ x, long long n,
@@ -623,6 +743,92 @@ extern "C" int ld_pulse_export(ld_handle* h, const int64_t* edges, const int64_t
   hipLaunchKernelGGL(k_pulse_export, dim3((unsigned)grid), dim3(256), 0, st, (const long long*)edges,
                      (const long long*)count, means, (long long)cap_edges, (long long*)de, (float*)dm,
                      (long long*)dc);
+  LD_HIP(hipGetLastError());
+  return LD_OK;
+}
+
+extern "C" size_t ld_pulse_search_workspace_bytes(const ld_handle*, const ld_pulse_job* jobs, int njobs) {
+  size_t b = 0;
+  for (int i = 0; i < njobs; ++i) {
+    const int64_t n = jobs ? jobs[i].n : 0;
+    const int64_t tiles = n > 0 ? (n + kBitTile - 1) / kBitTile : 1;
+    b += (size_t)(tiles + 1) * sizeof(long long);
+  }
+  return b > 0 ? b : sizeof(long long);
+}
+
+extern "C" int ld_pulse_search(ld_handle* h, const ld_pulse_job* jobs, int njobs, void* workspace, size_t ws_bytes,
+                               void* stream) {
+  if (!h || !jobs || njobs <= 0 || njobs > kMaxPulseJobs || !workspace) {
+    set_error("ld_pulse_search: bad arguments (1 to %d jobs)", kMaxPulseJobs);
+    return LD_EINVAL;
+  }
+  if (ws_bytes < ld_pulse_search_workspace_bytes(h, jobs, njobs)) {
+    set_error("ld_pulse_search: workspace %zu bytes < %zu needed", ws_bytes,
+              ld_pulse_search_workspace_bytes(h, jobs, njobs));
+    return LD_EINVAL;
+  }
+  LD_HIP(hipSetDevice(h->device));
+  PulseSearchArgs a{};
+  a.njobs = njobs;
+  long long* ws = (long long*)workspace;
+  for (int i = 0; i < njobs; ++i) {
+    const ld_pulse_job& q = jobs[i];
+    if (!q.bits || q.n <= 0 || !q.x || !q.edges || !q.count || !q.means || q.cap < 2 || (q.cap & 1) ||
+        !q.host_edges || !q.host_means || !q.host_count) {
+      set_error("ld_pulse_search: job %d: bad arguments", i);
+      return LD_EINVAL;
+    }
+    if (((uintptr_t)q.bits & 7) != 0) {
+      set_error("ld_pulse_search: job %d: bits must be 8-byte aligned", i);
+      return LD_EINVAL;
+    }
+    const long long tiles = (q.n + kBitTile - 1) / kBitTile;
+    if (tiles > 0x3fffffffLL) {
+      set_error("ld_pulse_search: job %d: buffer too long", i);
+      return LD_EINVAL;
+    }
+    PulseJobDev& d = a.job[i];
+    d.bits = q.bits;
+    d.x = q.x;
+    d.n = q.n;
+    d.edges = (long long*)q.edges;
+    d.count = (long long*)q.count;
+    d.means = q.means;
+    d.cap = q.cap;
+    d.min_len = q.min_len;
+    d.trim = q.trim;
+    void *de = nullptr, *dm = nullptr, *dc = nullptr;
+    LD_HIP(hipHostGetDevicePointer(&de, q.host_edges, 0));
+    LD_HIP(hipHostGetDevicePointer(&dm, q.host_means, 0));
+    LD_HIP(hipHostGetDevicePointer(&dc, q.host_count, 0));
+    d.hedges = (long long*)de;
+    d.hmeans = (float*)dm;
+    d.hcount = (long long*)dc;
+    d.tiles = ws;
+    d.ntiles = tiles;
+    ws += tiles + 1;
+    a.tile0[i + 1] = a.tile0[i] + tiles;
+    long long mwg = (q.cap / 2 + 63) / 64;
+    if (mwg > 2048) mwg = 2048;
+    a.mwg0[i + 1] = a.mwg0[i] + mwg;
+  }
+  for (int i = njobs; i < kMaxPulseJobs; ++i) {
+    a.tile0[i + 1] = a.tile0[njobs];
+    a.mwg0[i + 1] = a.mwg0[njobs];
+  }
+  if (a.tile0[njobs] > 0x7fffffffLL) {
+    set_error("ld_pulse_search: buffers too long");
+    return LD_EINVAL;
+  }
+  hipStream_t st = (hipStream_t)stream;
+  hipLaunchKernelGGL(k_bits_count_multi, dim3((unsigned)a.tile0[njobs]), dim3(kEdgeThreads), 0, st, a);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_edge_scan_multi, dim3(njobs), dim3(kScanThreads), 0, st, a);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_bits_write_multi, dim3((unsigned)a.tile0[njobs]), dim3(kEdgeThreads), 0, st, a);
+  LD_HIP(hipGetLastError());
+  hipLaunchKernelGGL(k_means_export_multi, dim3((unsigned)a.mwg0[njobs]), dim3(64), 0, st, a);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

@@ -145,6 +145,39 @@ class PulseFinder:
         return self.h_edges.numpy()[:cnt], self.h_means.numpy()[:cnt // 2]
 
 
+def pulse_search_jobs(searches):
+    """ld_pulse_job entries for [(PulseFinder, demod05 tensor, samples,
+    in-pulse bit words), ...]: each finder's device and pinned lists."""
+    jobs = (_lib.PulseJob * len(searches))()
+    for j, (pf, d05, n, bits) in zip(jobs, searches):
+        rf = pf.rf
+        assert n <= pf.n and d05.dtype == pf.torch.float32 and d05.is_contiguous()
+        j.bits, j.n, j.x = bits.data_ptr(), int(n), d05.data_ptr()
+        j.edges, j.count, j.means, j.cap = pf.edges.data_ptr(), pf.count.data_ptr(), pf.means.data_ptr(), pf.cap
+        j.min_len = 10.0 * ((rf.freq / rf.linelen) * rf.linelen)      # usectoinpx(10)
+        j.trim = float(rf.freq)
+        j.host_edges, j.host_means, j.host_count = (pf.h_edges.data_ptr(), pf.h_means.data_ptr(),
+                                                    pf.h_count.data_ptr())
+    return jobs
+
+
+def pulse_search(searches, workspace, stream):
+    """The pulse searches of several buffers in one ld_pulse_search call on
+    `stream` (each PulseFinder's result() then reads its lists once the
+    stream's work has completed)."""
+    jobs = pulse_search_jobs(searches)
+    rf = searches[0][0].rf
+    _lib.check(rf.lib.ld_pulse_search(rf.handle, jobs, len(searches), workspace.data_ptr(), workspace.numel(),
+                                      ctypes.c_void_p(stream.cuda_stream)))
+
+
+def pulse_search_workspace(searches):
+    """Device bytes ld_pulse_search needs for these buffers."""
+    jobs = pulse_search_jobs(searches)
+    rf = searches[0][0].rf
+    return rf.lib.ld_pulse_search_workspace_bytes(rf.handle, jobs, len(searches))
+
+
 def _facts_array(facts, n):
     """n FieldFacts: the given ones first, the rest unknown."""
     if facts is None:

@@ -158,6 +158,39 @@ def test_demod_in_pulse_bits(gpu_rf, system):
     np.testing.assert_array_equal(e_bits, e_float)
 
 
+def test_pulse_search_many_buffers(gpu_rf):
+    """ld_pulse_search over an NTSC and a PAL buffer at once (one launch per
+    step for both): each buffer's pinned host lists equal the numpy edges and
+    np.mean levels of its own samples, and its count; a third job whose
+    capacity is below its edge count exports exactly the first cap edges."""
+    import torch
+    bufs, finders, searches = [], [], []
+    for system, seed in (("NTSC", 4), ("PAL", 5), ("NTSC", 6)):
+        rf = gpu_rf(system, audio=False)
+        fl = 667_334 if system == "NTSC" else 800_000
+        cap = synth.make_capture(system, 2 * fl, seed=seed, snr_db=40.0, jitter_ns=50, audio=False)
+        dec = capture.CaptureDecoder(rf, torch.from_numpy(cap.data).cuda(), locate=True, max_fields=4)
+        dec.demod()
+        d05 = dec.buffers["demod_05"]
+        n = d05.numel()
+        pf = locate.PulseFinder(rf, n, cap_edges=1000 if len(finders) == 2 else None)
+        bufs.append((rf, d05.cpu().numpy()))
+        finders.append(pf)
+        searches.append((pf, d05, n, dec.buffers["demod_05_below"]))
+    ws = torch.empty(locate.pulse_search_workspace(searches), dtype=torch.uint8, device="cuda")
+    locate.pulse_search(searches, ws, torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    for k, ((rf, x), pf) in enumerate(zip(bufs, finders)):
+        want = _edges_np(x, locate.pulse_threshold(rf))
+        cnt = int(pf.h_count[0])
+        assert cnt == len(want), f"job {k}"
+        ne = min(cnt, pf.cap)
+        np.testing.assert_array_equal(pf.h_edges.numpy()[:ne], want[:ne], err_msg=f"job {k}")
+        lvl = _means_np(x, want[:ne], 10.0 * ((rf.freq / rf.linelen) * rf.linelen), rf.freq)
+        np.testing.assert_array_equal(pf.h_means.numpy()[:ne // 2], lvl[:ne // 2], err_msg=f"job {k}")
+    assert finders[2].cap < int(finders[2].h_count[0])
+
+
 def test_range_means_exact(gpu_rf):
     import torch
     rf = gpu_rf("NTSC", audio=False)
