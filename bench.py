@@ -378,6 +378,25 @@ def profile_step(args):
     log("profile step done")
 
 
+def trace_summary(db, top=40):
+    """Markdown table of a rocprofv3 --kernel-trace database (its `kernels`
+    view): dispatches grouped by kernel and launch shape, the `top` with the
+    most total time."""
+    import sqlite3
+    con = sqlite3.connect(str(db))
+    rows = con.execute(
+        "select name, grid_x, grid_y, workgroup_x, vgpr_count, accum_vgpr_count, lds_size, count(*), "
+        "avg(duration), min(duration), max(duration), sum(duration) from kernels "
+        "group by name, grid_x, grid_y, workgroup_x order by sum(duration) desc limit ?", (top,)).fetchall()
+    out = ["| kernel | grid (wg x y) | wg | VGPR | AGPR | LDS B | calls | avg us | min us | max us | total ms |",
+           "|---|---|---|---|---|---|---|---|---|---|---|"]
+    for name, gx, gy, wx, vg, ag, lds, n, avg, mn, mx, tot in rows:
+        short = name.split("(")[0].replace("void ", "")[:60]
+        out.append(f"| {short} | {gx // max(wx, 1)} x {gy} | {wx} | {vg} | {ag} | {lds} | {n} | {avg / 1e3:.2f} | "
+                   f"{mn / 1e3:.2f} | {mx / 1e3:.2f} | {tot / 1e6:.3f} |")
+    return "\n".join(out)
+
+
 def _pmc_run(counters, target, workdir, tag):
     """One rocprofv3 counter pass (the counters collected together) over
     `bench.py <target>` in a child process; returns {kernel name: {counter:
@@ -586,7 +605,12 @@ def main():
     ap.add_argument("--measure-traffic", action="store_true",
                     help="run the two rocprofv3 counter passes and write " + str(TRAFFIC_FILE.name))
     ap.add_argument("--traffic-out", default=None, help="where --measure-traffic / --measure-mfma write their JSON")
+    ap.add_argument("--trace-summary", default=None, metavar="DB",
+                    help="print the kernel table of a rocprofv3 --kernel-trace database and exit")
     args = ap.parse_args()
+    if args.trace_summary:
+        print(trace_summary(args.trace_summary))
+        return
     if (args.profile_step or args.measure_traffic) and "auto" in (args.blocklen, args.fft):
         ap.error("--profile-step / --measure-traffic take a fixed --blocklen and --fft")
     if args.profile_fft:
