@@ -270,6 +270,14 @@ template <class C, int M_>
 __device__ __forceinline__ float2 split_tw(float2 wt) {
   return cmul(wt, make_float2(Tw<C::N / C::T, M_, -1>::re, Tw<C::N / C::T, M_, -1>::im));
 }
+// Elements m and m + 8 of a thread are 8 T bins apart: with N / T = 32 and
+// P = 16, W_N^(T (m + 8)) = W_32^(m + 8) = -i W_32^m, so the twiddle of
+// element m + 8 is the one of element m times -i (an exact rotation).
+template <class C>
+__device__ __forceinline__ float2 split_tw_half(float2 w) {
+  static_assert(C::N / C::T == 32 && C::P == 16, "pairs m, m + 8 need N / T = 32, P = 16");
+  return make_float2(w.y, -w.x);
+}
 
 // Everything one fused launch needs.  Filters are half spectra (bins 0..M)
 // with the inverse-transform scale folded in.  The c2r outputs (rfhpf and the
@@ -336,16 +344,29 @@ __device__ __forceinline__ void r2c_split(const float2 (&v)[C::P], float2 (&x)[C
     s.im[pidx(C::M)] = v[0].y;
   }
   __syncthreads();
-  static_for<0, C::P>([&](auto mc) {
+  // elements m and m + 8 together, sharing one twiddle product
+  static_for<0, C::P / 2>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    float2 p = v[m];
-    float2 q = mir.template ld<POff<C::M - C::T * m>::value>();
-    float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
-    float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
-    x[m] = cadd(E, cmul(split_tw<C, m>(wt), O));
-    if (m == 0 && tid == 0) xM = p.x - p.y;
+    const float2 w = split_tw<C, m>(wt);
+    const float2 wh = split_tw_half<C>(w);
+    {
+      float2 p = v[m];
+      float2 q = mir.template ld<POff<C::M - C::T * m>::value>();
+      float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+      float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+      x[m] = cadd(E, cmul(w, O));
+      if (m == 0 && tid == 0) xM = p.x - p.y;
+    }
+    {
+      constexpr int m2 = m + C::P / 2;
+      float2 p = v[m2];
+      float2 q = mir.template ld<POff<C::M - C::T * m2>::value>();
+      float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
+      float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+      x[m2] = cadd(E, cmul(wh, O));
+    }
     // bound how far the scheduler hoists the mirror loads (register pressure)
-    if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    if constexpr ((m & 3) == 3) __builtin_amdgcn_sched_barrier(0);
   });
   __syncthreads();
 }
@@ -370,14 +391,26 @@ __device__ __forceinline__ void c2r_pre(const float2 (&x)[C::P], float xM, float
     s.im[pidx(C::M)] = zM.y;
   }
   __syncthreads();
-  static_for<0, C::P>([&](auto mc) {
+  static_for<0, C::P / 2>([&](auto mc) {
     constexpr int m = decltype(mc)::value;
-    float2 B = cconj(mir.template ld<POff<C::M - C::T * m>::value>());
-    float2 A = v[m];
-    float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
-    float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), split_tw<C, m>(wt));
-    v[m] = make_float2(E.x - O.y, -(E.y + O.x));
-    if constexpr ((m & 7) == 7) __builtin_amdgcn_sched_barrier(0);
+    const float2 w = split_tw<C, m>(wt);
+    const float2 wh = split_tw_half<C>(w);
+    {
+      float2 B = cconj(mir.template ld<POff<C::M - C::T * m>::value>());
+      float2 A = v[m];
+      float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
+      float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), w);
+      v[m] = make_float2(E.x - O.y, -(E.y + O.x));
+    }
+    {
+      constexpr int m2 = m + C::P / 2;
+      float2 B = cconj(mir.template ld<POff<C::M - C::T * m2>::value>());
+      float2 A = v[m2];
+      float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
+      float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), wh);
+      v[m2] = make_float2(E.x - O.y, -(E.y + O.x));
+    }
+    if constexpr ((m & 3) == 3) __builtin_amdgcn_sched_barrier(0);
   });
   // the FFT's first pass synchronises before it writes LDS
 }
@@ -762,16 +795,22 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       const float2 hM = cscale(bin_m[1], xM);  // RFVideo bin M
       const bool odd = (ph == kPhOdd);
       const uint32_t zmask = a.notch_hi > a.notch_lo ? notch_mask[tidv] : 0u;
-      static_for<0, C::P>([&](auto mc) {
-        constexpr int m = decltype(mc)::value;
+      static_for<0, C::P / 2>([&](auto mc) {
+        constexpr int m = decltype(mc)::value, m2 = m + C::P / 2;
+        const float2 w = split_tw<C, m>(wt);
+        const float2 wh = split_tw_half<C>(w);
         const int k = tid + C::T * m;
         float2 z = cmul(x[m], v[m]);
         if (zmask & (1u << m)) z = make_float2(0.0f, 0.0f);
         if (k == 0)
           z = odd ? csub(z, hM) : cadd(z, hM);
         else if (odd)
-          z = cmulc(z, split_tw<C, m>(wt));
+          z = cmulc(z, w);
         v[m] = cconj(z);
+        float2 z2 = cmul(x[m2], v[m2]);  // bin tid + T m2 > 0
+        if (zmask & (1u << m2)) z2 = make_float2(0.0f, 0.0f);
+        if (odd) z2 = cmulc(z2, wh);
+        v[m2] = cconj(z2);
       });
     } else if (ph == kPhDemod) {
       // instantaneous frequency (utils.py:898-926), clip (core.py:809).
