@@ -329,8 +329,13 @@ constexpr uint32_t kStrideB = kT * 8;           // bytes between a thread's elem
 
 // ---------------------------------------------------------------- helpers
 // After a forward M-point FFT of y[m] = x[2m] + i x[2m+1] (v[m] = Y[k],
-// k = tid+T*m) form the half spectrum X[k] = E + W_N^k O of the real
-// sequence x in x[m], and X[M] (real) in xM on the thread holding k = 0.
+// k = tid+T*m) form TWICE the half spectrum X[k] = E + W_N^k O of the real
+// sequence x in x[m], and 2 X[M] (real) in xM on the thread holding k = 0:
+// the halvings of E and O are left to the tables this spectrum meets (the
+// c2r tables carry a factor 1/4, see c2r_pre; the analytic signal, the
+// notch decision and the audio slices only see phases or ratios), which
+// saves a multiply per component.  Scaling by 2 is exact in float32, so the
+// outputs are those of the halved form.
 template <class C>
 __device__ __forceinline__ void r2c_split(const float2 (&v)[C::P], float2 (&x)[C::P], float& xM, LdsT<C::M>& s, float2 wt,
                                           int tid) {
@@ -352,17 +357,17 @@ __device__ __forceinline__ void r2c_split(const float2 (&v)[C::P], float2 (&x)[C
     {
       float2 p = v[m];
       float2 q = mir.template ld<POff<C::M - C::T * m>::value>();
-      float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
-      float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+      float2 E = make_float2(p.x + q.x, p.y - q.y);
+      float2 O = make_float2(p.y + q.y, q.x - p.x);
       x[m] = cadd(E, cmul(w, O));
-      if (m == 0 && tid == 0) xM = p.x - p.y;
+      if (m == 0 && tid == 0) xM = 2.0f * (p.x - p.y);
     }
     {
       constexpr int m2 = m + C::P / 2;
       float2 p = v[m2];
       float2 q = mir.template ld<POff<C::M - C::T * m2>::value>();
-      float2 E = make_float2(0.5f * (p.x + q.x), 0.5f * (p.y - q.y));
-      float2 O = make_float2(0.5f * (p.y + q.y), -0.5f * (p.x - q.x));
+      float2 E = make_float2(p.x + q.x, p.y - q.y);
+      float2 O = make_float2(p.y + q.y, q.x - p.x);
       x[m2] = cadd(E, cmul(wh, O));
     }
     // bound how far the scheduler hoists the mirror loads (register pressure)
@@ -371,10 +376,11 @@ __device__ __forceinline__ void r2c_split(const float2 (&v)[C::P], float2 (&x)[C
   __syncthreads();
 }
 
-// Input of a real inverse transform of Z = X * H (Hermitian half spectrum, H
-// scaled by 1/M; its bins tid + T m arrive in v, bin M in hM), conjugated
-// so that a forward FFT produces the conjugate of the inverse:
-// v[m] = conj(E + i O).
+// Input of a real inverse transform of Z = X * H (Hermitian half spectrum;
+// x holds 2 X (r2c_split) and the table H / (4 M), so that x H = Z / (2 M):
+// E and O come out without their halvings; its bins tid + T m arrive in v,
+// bin M in hM), conjugated so that a forward FFT produces the conjugate of
+// the inverse: v[m] = conj(E + i O).
 template <class C>
 __device__ __forceinline__ void c2r_pre(const float2 (&x)[C::P], float xM, float2 hM, float2 (&v)[C::P], LdsT<C::M>& s,
                                         float2 wt, int tid) {
@@ -398,16 +404,16 @@ __device__ __forceinline__ void c2r_pre(const float2 (&x)[C::P], float xM, float
     {
       float2 B = cconj(mir.template ld<POff<C::M - C::T * m>::value>());
       float2 A = v[m];
-      float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
-      float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), w);
+      float2 E = make_float2(A.x + B.x, A.y + B.y);
+      float2 O = cmulc(make_float2(A.x - B.x, A.y - B.y), w);
       v[m] = make_float2(E.x - O.y, -(E.y + O.x));
     }
     {
       constexpr int m2 = m + C::P / 2;
       float2 B = cconj(mir.template ld<POff<C::M - C::T * m2>::value>());
       float2 A = v[m2];
-      float2 E = make_float2(0.5f * (A.x + B.x), 0.5f * (A.y + B.y));
-      float2 O = cmulc(make_float2(0.5f * (A.x - B.x), 0.5f * (A.y - B.y)), wh);
+      float2 E = make_float2(A.x + B.x, A.y + B.y);
+      float2 O = cmulc(make_float2(A.x - B.x, A.y - B.y), wh);
       v[m2] = make_float2(E.x - O.y, -(E.y + O.x));
     }
     if constexpr ((m & 3) == 3) __builtin_amdgcn_sched_barrier(0);
@@ -1810,6 +1816,14 @@ static int upload_complex(ld_handle* h, float2** dst, const double* reim, int n,
   return replace_table(h, reinterpret_cast<void**>(dst), tmp.data(), sizeof(float2) * n);
 }
 
+// The c2r tables of the single-level kernels (8192 .. 32768 samples) and of
+// audio stage 2 carry a factor 1/4: r2c_split leaves its spectrum doubled and
+// c2r_pre its halves undivided (exact powers of two).  RFVideo, whose product
+// only feeds phases, and the 65536-sample kernels' tables are unscaled.
+static double c2r_table_scale(const ld_handle* h, int id) {
+  return (id == LD_FILT_RFVIDEO || h->blocklen == kN64) ? 1.0 : 0.25;
+}
+
 int ld_set_filter(ld_handle* h, int id, const double* reim, int nbins, double scale, double dc_gain,
                   int shift) {
   if (!h || !reim || id < 0 || id >= LD_NFILT) {
@@ -1824,6 +1838,7 @@ int ld_set_filter(ld_handle* h, int id, const double* reim, int nbins, double sc
     set_error("ld_set_filter: shift %d out of range", shift);
     return LD_EINVAL;
   }
+  scale *= c2r_table_scale(h, id);
   int rc = upload_complex(h, &h->filt[id], reim, nbins, scale);
   if (rc) return rc;
   // the one-sample roll, in double: bin k times exp(+2 pi i k / N)
@@ -1863,7 +1878,8 @@ int ld_set_audio2(ld_handle* h, int ch, const double* reim, int nbins, double ce
     set_error("ld_set_audio2: nbins %d != blocklen/2+1 = %d", nbins, h->blocklen / 2 + 1);
     return LD_EINVAL;
   }
-  int rc = upload_complex(h, &h->audio2_filt[ch], reim, nbins, 1.0 / (h->blocklen / 2));
+  // k_audio2_block's r2c_split / c2r_pre pair: the factor 1/4 (see c2r_table_scale)
+  int rc = upload_complex(h, &h->audio2_filt[ch], reim, nbins, 0.25 / (h->blocklen / 2));
   if (rc) return rc;
   h->audio2_center[ch] = center_freq;
   return LD_OK;
