@@ -46,6 +46,7 @@ struct LineRec {
   double w[3];           // 1 / (t_{j+1} - t_j) of those segments
   long long first[3];    // first pixel of those segments
   long long p0;          // pixel index of the line's first output pixel
+  double p0d;            // the same as a double
   long long demod_offset, demod_len, out_offset;
   long long lineloc_offset;
   double inl, rinl, outscale;
@@ -256,6 +257,7 @@ __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* 
       r.w[i] = 1.0 / (((double)(js + 1) * inl) - ((double)js * inl));
     }
     r.p0 = p0;
+    r.p0d = (double)p0;
     r.demod_offset = fd.demod_offset;
     r.demod_len = fd.demod_len;
     r.out_offset = fd.out_offset + (long long)line * W;
@@ -305,22 +307,28 @@ __device__ __forceinline__ float comp(const float4& f, int c) {
 
 // Per-pixel state between the LUT row addresses and the tap sum.
 struct PixelPrep {
-  long long start;  // first tap's sample index (field-relative)
+  double start;        // first tap's sample index (field-relative), an integer
   double level;
   float alpha;
-  int fold;
-  uint32_t ou, ov;  // byte offsets of the ws / we groups, see pixel_prep
-  int step;         // byte step between groups (-16 in a folded row)
+  uint32_t fold_mask;  // all ones in a folded row (groups and taps reversed)
+  uint32_t ou, ov;     // byte offsets of the ws / we groups, see pixel_prep
 };
 // LUT offset past the table: buffer loads there return zeros
 constexpr uint32_t kLutNone = 0x7fffff00u;
+
+// A wave-uniform double, marked as such (its loads then stay scalar).
+__device__ __forceinline__ double wave_uniform(double v) {
+  const unsigned long long b = (unsigned long long)__double_as_longlong(v);
+  const unsigned lo = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)b);
+  const unsigned hi = (unsigned)__builtin_amdgcn_readfirstlane((int)(unsigned)(b >> 32));
+  return __longlong_as_double((long long)(((unsigned long long)hi << 32) | lo));
+}
 
 // Source position, level factor and LUT rows of pixel t of the line.
 __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec& rc, rsrc_t lut, int t,
                                            PixelPrep& q) {
   const int W = rc.W;
-  const long long p = rc.p0 + t;
-  const double x = (double)p * rc.outscale;
+  const double x = (rc.p0d + (double)t) * rc.outscale;  // p0 + t is exact in a double
   // segment of the k=1 spline holding x: knots at j*inlinelen, right-
   // continuous; the guess x*(1/inl) is within one of it
   int j = (int)(x * rc.rinl);
@@ -328,9 +336,20 @@ __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec&
   if ((double)j * rc.inl > x) --j;
   j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
   const int jl = j - rc.jB;
+  const int jl0 = __builtin_amdgcn_readfirstlane(jl);
   double llj, llj1, lev, ema, w;
   long long first;
-  if (jl >= 0 && jl <= 2) {
+  if (__ballot(jl != jl0) == 0 && jl0 >= 0 && jl0 <= 2) {
+    // the wavefront's pixels share one cached segment (all but the line's
+    // first pixels): the record's values picked by a scalar index, no
+    // per-pixel memory round trip
+    llj = wave_uniform(rc.ll[jl0]);
+    llj1 = wave_uniform(rc.ll[jl0 + 1]);
+    lev = wave_uniform(rc.lev[jl0]);
+    w = wave_uniform(rc.w[jl0]);
+    ema = rc.ema[jl0];
+    first = rc.first[jl0];
+  } else if (jl >= 0 && jl <= 2) {
     llj = rc.ll[jl];
     llj1 = rc.ll[jl + 1];
     lev = rc.lev[jl];
@@ -353,21 +372,25 @@ __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec&
   q.level = lev;
   if (a.smoothing > 0 && j != 0) {
     const double keep = 1.0 - 1.0 / (a.smoothing * W);
-    q.level = lev + pow(keep, (double)(p - first + 1)) * (ema - lev);
+    q.level = lev + pow(keep, (double)(rc.p0 + t - first + 1)) * (ema - lev);
   }
-  long long ci;
+  // int(coord) truncates toward zero; the integer is kept as a float /
+  // double (trunc), which equals the converted integer wherever the
+  // coordinate has a fraction, and the coordinate itself where it has none
+  double ci;
   int ph0;
   float alpha;
   if (a.coord_f32) {
     const float c = (float)loc;
-    ci = fabsf(c) < 2.0e9f ? (long long)(int)c : (long long)c;  // truncation toward zero, as int()
-    const float frac = c - (float)ci;
+    const float tc = truncf(c);
+    const float frac = c - tc;
     const float pp = frac * 65536.0f;
     ph0 = (int)pp;
     alpha = pp - (float)ph0;
+    ci = (double)tc;
   } else {
-    ci = fabs(loc) < 2.0e9 ? (long long)(int)loc : (long long)loc;
-    const double frac = loc - (double)ci;
+    ci = trunc(loc);
+    const double frac = loc - ci;
     const double pp = frac * 65536.0;
     ph0 = (int)pp;
     alpha = (float)(pp - (double)ph0);
@@ -379,7 +402,7 @@ __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec&
     alpha = 0.0f;
   }
   q.alpha = alpha;
-  q.start = ci - (kSincTaps / 2 - 1);
+  q.start = ci - (double)(kSincTaps / 2 - 1);
   // Weights ws = row ph0, we = row ph0+1.  In a folded table rows above
   // 32768 are read as tap-reversed rows of the lower half: ws = reversed row
   // 65536-ph0, we = reversed row 65535-ph0 (for ph0 = 65535 the last row
@@ -391,10 +414,9 @@ __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec&
   const bool last = ph0 == kSincPhases - 1;
   const uint32_t ou = fold ? rowB + 48u : rowA;
   const uint32_t ov = fold ? (last ? rowB : rowA) + 48u : rowB;
-  q.step = fold ? -16 : 16;
   q.ou = ou;
   q.ov = ov;
-  q.fold = fold;
+  q.fold_mask = fold ? 0xffffffffu : 0u;
 }
 
 // Value of lane g of the thread's 4-lane group (DPP quad broadcast).
@@ -418,24 +440,28 @@ __device__ __forceinline__ int quad_bcast(int v) {
 __device__ __forceinline__ int wts_slot(int P, int c) { return 4 * (P ^ ((P >> 2) & 3)) + ((c + P) & 3); }
 struct QuadRows {
   float4 u[4], v[4];
-  int fold[4];
+  uint32_t fold_mask[4];
   float alpha[4];
 };
 __device__ __forceinline__ void quad_fetch(rsrc_t lut, const PixelPrep& q, QuadRows& r, int tid) {
-  const int l = tid & 3;
+  const uint32_t l16 = (uint32_t)(tid & 3) * 16u;
   auto fetch = [&](auto gc) {
     constexpr int g = decltype(gc)::value;
     const uint32_t ou = (uint32_t)quad_bcast<g>((int)q.ou), ov = (uint32_t)quad_bcast<g>((int)q.ov);
-    const int step = quad_bcast<g>(q.step);
-    r.fold[g] = quad_bcast<g>(q.fold);
+    const uint32_t m = (uint32_t)quad_bcast<g>((int)q.fold_mask);
+    const uint32_t d = (l16 ^ m) - m;  // +16 l, or -16 l in a folded row
+    r.fold_mask[g] = m;
     r.alpha[g] = __int_as_float(quad_bcast<g>(__float_as_int(q.alpha)));
-    r.u[g] = lut_load(lut, ou + (uint32_t)(l * step));
-    r.v[g] = lut_load(lut, ov + (uint32_t)(l * step));
+    r.u[g] = lut_load(lut, ou + d);
+    r.v[g] = lut_load(lut, ov + d);
   };
   fetch(std::integral_constant<int, 0>());
   fetch(std::integral_constant<int, 1>());
   fetch(std::integral_constant<int, 2>());
   fetch(std::integral_constant<int, 3>());
+}
+__device__ __forceinline__ float bit_select(uint32_t m, float a, float b) {
+  return __uint_as_float((__float_as_uint(a) & m) | (__float_as_uint(b) & ~m));
 }
 __device__ __forceinline__ void quad_store(const QuadRows& r, float4* wts, int tid) {
   const int l = tid & 3;
@@ -448,27 +474,34 @@ __device__ __forceinline__ void quad_store(const QuadRows& r, float4* wts, int t
       const float ws = comp(r.u[g], c), we = comp(r.v[g], c);
       w[c] = fmaf(r.alpha[g], we - ws, ws);
     }
-    // a folded row holds the taps of the group in reverse order
-    const float4 o = r.fold[g] ? make_float4(w[3], w[2], w[1], w[0]) : make_float4(w[0], w[1], w[2], w[3]);
+    // a folded row holds the taps of the group in reverse order (bitwise
+    // selects: one instruction per component, no branch)
+    const uint32_t m = r.fold_mask[g];
+    const float4 o = make_float4(bit_select(m, w[3], w[0]), bit_select(m, w[2], w[1]), bit_select(m, w[1], w[2]),
+                                 bit_select(m, w[0], w[3]));
     wts[wts_slot(qb + g, l)] = o;
   }
 }
 
 // 16-tap sum, level factor and output codes of pixel t.
 __device__ __forceinline__ void pixel_finish(const TbcPixelArgs& a, const LineRec& rc, const float* stage,
-                                             const float* src, long long slo, int nst, int t, const PixelPrep& q,
+                                             const float* src, double slo, int nst, int t, const PixelPrep& q,
                                              const float4* wts, int c0) {
-  const long long rel = q.start - slo;
+  const double rel = q.start - slo;  // integers below 2^53: exact
   float sv[kSincTaps];
-  if (rel >= 0 && rel + kSincTaps <= nst) {
+  if (rel >= 0.0 && rel <= (double)(nst - kSincTaps)) {
     const int r0 = (int)rel;
 #pragma unroll
     for (int e = 0; e < kSincTaps; ++e) sv[e] = stage[r0 + e];
   } else {
+    // taps outside the staged span, from the buffer (0 outside it; a
+    // coordinate too large for an int64 has every tap outside)
+    const bool finite = fabs(q.start) < 4.0e18;
+    const long long s0 = finite ? (long long)q.start : 0;
 #pragma unroll
     for (int e = 0; e < kSincTaps; ++e) {
-      const long long idx = q.start + e;
-      sv[e] = (idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
+      const long long idx = s0 + e;
+      sv[e] = (finite && idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
     }
   }
   float4 wq[4];
@@ -495,8 +528,11 @@ __device__ __forceinline__ void pixel_finish(const TbcPixelArgs& a, const LineRe
 // leave the span (decreasing line locations, spans over kChunkStage) read
 // them from global memory.
 __global__ __launch_bounds__(kTbcThreads) void k_tbc_pixels(TbcPixelArgs a) {
-  __shared__ float stage[kChunkStage];
-  __shared__ float4 wts[kTbcThreads * 4];
+  // staged samples at LDS offset 0, so that a pixel's 16 reads share one
+  // address with the tap index in the instruction's offset field
+  __shared__ float4 lds[kChunkStage / 4 + kTbcThreads * 4];
+  float* stage = reinterpret_cast<float*>(lds);
+  float4* wts = lds + kChunkStage / 4;
   const int line = blockIdx.x / a.chunks, chunk = blockIdx.x - line * a.chunks;
   const LineRec& rc = a.recs[line];
   const int W = rc.W;
@@ -514,30 +550,32 @@ __global__ __launch_bounds__(kTbcThreads) void k_tbc_pixels(TbcPixelArgs a) {
   // LUT loads first: the in-order load counter then lets the weights be
   // formed while the staging loads issued after them are in flight
   PixelPrep q;
-  q.start = 0;
+  q.start = 0.0;
   q.level = 0.0;
   q.alpha = 0.0f;
-  q.fold = 0;
+  q.fold_mask = 0u;
   q.ou = q.ov = kLutNone;
-  q.step = 16;
   if (t <= c1) pixel_prep(a, rc, lut, t, q);
   QuadRows rows;
   quad_fetch(lut, q, rows, tid);
+  // staged samples i with 0 <= slo + i < demod_len and i < nst (bounds
+  // formed once per workgroup in int64, compared per element in int32)
+  const long long below = -slo, above = rc.demod_len - slo;
+  const int ilo = below <= 0 ? 0 : (below >= nst ? nst : (int)below);
+  const int ihi = above <= 0 ? 0 : (above >= nst ? nst : (int)above);
   float sreg[kStagePerThread];
 #pragma unroll
   for (int k = 0; k < kStagePerThread; ++k) {
     const int i = tid + k * kTbcThreads;
-    const long long idx = slo + i;
-    sreg[k] = (i < nst && idx >= 0 && idx < rc.demod_len) ? src[idx] : 0.0f;
+    sreg[k] = (i >= ilo && i < ihi) ? src[slo + i] : 0.0f;
   }
   quad_store(rows, wts, tid);
+  // zeros past nst are never read (a pixel reads the stage only when its
+  // 16 taps lie below nst)
 #pragma unroll
-  for (int k = 0; k < kStagePerThread; ++k) {
-    const int i = tid + k * kTbcThreads;
-    if (i < nst) stage[i] = sreg[k];
-  }
+  for (int k = 0; k < kStagePerThread; ++k) stage[tid + k * kTbcThreads] = sreg[k];
   __syncthreads();
-  if (t <= c1) pixel_finish(a, rc, stage, src, slo, nst, t, q, wts, c0);
+  if (t <= c1) pixel_finish(a, rc, stage, src, (double)slo, nst, t, q, wts, c0);
 }
 
 // ------------------------------------------------------------- hsync refine
