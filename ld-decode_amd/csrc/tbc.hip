@@ -45,6 +45,13 @@ struct LineRec {
   double lev[3], ema[3]; // per-segment level and EMA start, segments jB .. jB+2
   double w[3];           // 1 / (t_{j+1} - t_j) of those segments
   long long first[3];    // first pixel of those segments
+  int bnd[4];            // first pixel of segments jB .. jB+3 relative to p0 (the line's pixel t
+                         // lies in segment jB + k for bnd[k] <= t < bnd[k+1])
+  double tk[4];          // knot times (jB + k) * inlinelen
+  // the segment holding most of the line's pixels (jB + 1 unless empty):
+  // its pixel range relative to p0 and its values, at fixed offsets
+  int mb0, mb1, mj;
+  double m_ll0, m_ll1, m_lev, m_w, m_t0, m_t1;
   long long p0;          // pixel index of the line's first output pixel
   double p0d;            // the same as a double
   long long demod_offset, demod_len, out_offset;
@@ -258,6 +265,21 @@ __global__ __launch_bounds__(kLineThreads) void k_tbc_lines(const ld_tbc_field* 
     }
     r.p0 = p0;
     r.p0d = (double)p0;
+    for (int i = 0; i < 4; ++i) {
+      // segments past the last are empty (their bound is the pixel count)
+      r.bnd[i] = (int)(row->first_px[jB + i <= nseg ? jB + i : nseg] - p0);
+      r.tk[i] = (double)(jB + i) * inl;
+    }
+    const int k = r.bnd[1] < r.bnd[2] ? 1 : 0;
+    r.mb0 = r.bnd[k];
+    r.mb1 = r.bnd[k + 1];
+    r.mj = jB + k;
+    r.m_ll0 = r.ll[k];
+    r.m_ll1 = r.ll[k + 1];
+    r.m_lev = r.lev[k];
+    r.m_w = r.w[k];
+    r.m_t0 = r.tk[k];
+    r.m_t1 = r.tk[k + 1];
     r.demod_offset = fd.demod_offset;
     r.demod_len = fd.demod_len;
     r.out_offset = fd.out_offset + (long long)line * W;
@@ -329,45 +351,70 @@ __device__ __forceinline__ void pixel_prep(const TbcPixelArgs& a, const LineRec&
                                            PixelPrep& q) {
   const int W = rc.W;
   const double x = (rc.p0d + (double)t) * rc.outscale;  // p0 + t is exact in a double
-  // segment of the k=1 spline holding x: knots at j*inlinelen, right-
-  // continuous; the guess x*(1/inl) is within one of it
-  int j = (int)(x * rc.rinl);
-  if ((double)(j + 1) * rc.inl <= x) ++j;
-  if ((double)j * rc.inl > x) --j;
-  j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
-  const int jl = j - rc.jB;
-  const int jl0 = __builtin_amdgcn_readfirstlane(jl);
-  double llj, llj1, lev, ema, w;
+  // segment of the k=1 spline holding x (knots at j*inlinelen, right-
+  // continuous): inside the three cached segments from the line's segment
+  // bounds (pixel indices k_tbc_lines found with the same search), else
+  // searched here
+  int j;
+  double llj, llj1, lev, ema, w, t0, t1;
   long long first;
-  if (__ballot(jl != jl0) == 0 && jl0 >= 0 && jl0 <= 2) {
-    // the wavefront's pixels share one cached segment (all but the line's
-    // first pixels): the record's values picked by a scalar index, no
-    // per-pixel memory round trip
-    llj = wave_uniform(rc.ll[jl0]);
-    llj1 = wave_uniform(rc.ll[jl0 + 1]);
-    lev = wave_uniform(rc.lev[jl0]);
-    w = wave_uniform(rc.w[jl0]);
-    ema = rc.ema[jl0];
-    first = rc.first[jl0];
-  } else if (jl >= 0 && jl <= 2) {
-    llj = rc.ll[jl];
-    llj1 = rc.ll[jl + 1];
-    lev = rc.lev[jl];
-    ema = rc.ema[jl];
-    first = rc.first[jl];
-    w = rc.w[jl];
+  // the main segment's values, loaded with the record's other fields (both
+  // paths below read them, so their loads are not deferred to a branch)
+  const int mj = rc.mj;
+  const double m_ll0 = wave_uniform(rc.m_ll0), m_ll1 = wave_uniform(rc.m_ll1);
+  const double m_lev = wave_uniform(rc.m_lev), m_w = wave_uniform(rc.m_w);
+  const double m_t0 = wave_uniform(rc.m_t0), m_t1 = wave_uniform(rc.m_t1);
+  if (__ballot(t < rc.mb0 || t >= rc.mb1) == 0) {
+    // the wavefront's pixels all lie in the line's main segment (all
+    // wavefronts but those holding a segment boundary): no per-pixel
+    // memory round trip
+    j = mj;
+    llj = m_ll0;
+    llj1 = m_ll1;
+    lev = m_lev;
+    w = m_w;
+    t0 = m_t0;
+    t1 = m_t1;
+    ema = rc.ema[j - rc.jB];
+    first = rc.first[j - rc.jB];
   } else {
-    const double* ll = a.linelocs + rc.lineloc_offset;
-    const LineRow* row = a.rows + rc.field;
-    llj = ll[j];
-    llj1 = ll[j + 1];
-    lev = row->level[j];
-    ema = row->ema_start[j];
-    first = row->first_px[j];
-    w = 1.0 / (((double)(j + 1) * rc.inl) - ((double)j * rc.inl));
+    int jl = -1;
+    if (t >= rc.bnd[0] && t < rc.bnd[3]) jl = (t >= rc.bnd[1] ? 1 : 0) + (t >= rc.bnd[2] ? 1 : 0);
+    if (jl >= 0 && rc.jB + jl == mj) {
+      j = mj;
+      llj = m_ll0;
+      llj1 = m_ll1;
+      lev = m_lev;
+      ema = rc.ema[jl];
+      first = rc.first[jl];
+      w = m_w;
+    } else if (jl >= 0) {
+      j = rc.jB + jl;
+      llj = rc.ll[jl];
+      llj1 = rc.ll[jl + 1];
+      lev = rc.lev[jl];
+      ema = rc.ema[jl];
+      first = rc.first[jl];
+      w = rc.w[jl];
+    } else {
+      // the guess x*(1/inl) is within one of the segment
+      j = (int)(x * rc.rinl);
+      if ((double)(j + 1) * rc.inl <= x) ++j;
+      if ((double)j * rc.inl > x) --j;
+      j = j < 0 ? 0 : (j > rc.nseg - 1 ? rc.nseg - 1 : j);
+      const double* ll = a.linelocs + rc.lineloc_offset;
+      const LineRow* row = a.rows + rc.field;
+      llj = ll[j];
+      llj1 = ll[j + 1];
+      lev = row->level[j];
+      ema = row->ema_start[j];
+      first = row->first_px[j];
+      w = 1.0 / (((double)(j + 1) * rc.inl) - ((double)j * rc.inl));
+    }
+    t0 = (double)j * rc.inl;
+    t1 = (double)(j + 1) * rc.inl;
   }
   // value of the k=1 B-spline (scipy's de Boor order of operations)
-  const double t0 = (double)j * rc.inl, t1 = (double)(j + 1) * rc.inl;
   const double loc = llj * (w * (t1 - x)) + llj1 * (w * (x - t0));
   q.level = lev;
   if (a.smoothing > 0 && j != 0) {
