@@ -300,6 +300,7 @@ struct DemodArgs {
   const u32x4_t* mfma_g;   // split-BF16 MFMA variant: DFT-64 fragments (kMfmaTabEntries)
   const MfmaTw* mfma_tw;   // and its twiddle factor tables, copied to LDS
   const float2* h_rfv;
+  const float2* h_rfv_odd; // RFVideo times exp(+2 pi i k / N): the odd analytic samples' table
   C2rOut hpf;              // out == nullptr: skip
   C2rOut post[4];
   int npost;
@@ -797,26 +798,17 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     } else if (ph == kPhEven || ph == kPhOdd) {
       // analytic signal (core.py:800-805), conjugated: even samples from
       // Z[k] (+ Z[M] folded into bin 0), odd samples from Z[k] W_N^-k
-      // (- Z[M] folded into bin 0)
+      // (- Z[M] folded into bin 0).  The odd phase's table is RFVideo
+      // times W_N^-k (formed in double on the host; bin 0 unchanged).
       const float2 hM = cscale(bin_m[1], xM);  // RFVideo bin M
       const bool odd = (ph == kPhOdd);
       const uint32_t zmask = a.notch_hi > a.notch_lo ? notch_mask[tidv] : 0u;
-      static_for<0, C::P / 2>([&](auto mc) {
-        constexpr int m = decltype(mc)::value, m2 = m + C::P / 2;
-        const float2 w = split_tw<C, m>(wt);
-        const float2 wh = split_tw_half<C>(w);
-        const int k = tid + C::T * m;
+      static_for<0, C::P>([&](auto mc) {
+        constexpr int m = decltype(mc)::value;
         float2 z = cmul(x[m], v[m]);
         if (zmask & (1u << m)) z = make_float2(0.0f, 0.0f);
-        if (k == 0)
-          z = odd ? csub(z, hM) : cadd(z, hM);
-        else if (odd)
-          z = cmulc(z, w);
+        if (m == 0 && tid == 0) z = odd ? csub(z, hM) : cadd(z, hM);
         v[m] = cconj(z);
-        float2 z2 = cmul(x[m2], v[m2]);  // bin tid + T m2 > 0
-        if (zmask & (1u << m2)) z2 = make_float2(0.0f, 0.0f);
-        if (odd) z2 = cmulc(z2, wh);
-        v[m2] = cconj(z2);
       });
     } else if (ph == kPhDemod) {
       // instantaneous frequency (utils.py:898-926), clip (core.py:809).
@@ -913,7 +905,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
         load_half<C>(a.post[0].h, v, tidv);
     } else if (ph == kPhEven) {
       stage_own<C>(v, s, tidv);
-      load_half<C>(a.h_rfv, v, tidv);  // RFVideo again for the odd half
+      load_half<C>(a.h_rfv_odd, v, tidv);  // the odd half's table
       const LdsPair own(s.re, s.im, pidx(tidv));
       static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
@@ -2106,6 +2098,7 @@ static int build_demod_args(ld_handle* h, const void* input, int dtype, int64_t 
   a.mfma_g = (const u32x4_t*)h->mfma_g;
   a.mfma_tw = (const MfmaTw*)h->mfma_tw;
   a.h_rfv = h->filt[LD_FILT_RFVIDEO];
+  a.h_rfv_odd = h->filt_rolled[LD_FILT_RFVIDEO];
   c2r_table(h, LD_FILT_RFHPF, h->cut_begin, -h->video_rot, a.hpf.h, a.hpf.shift);
   a.hpf.out = out->rfhpf;
   a.hpf.dc = 0.0f;
