@@ -697,6 +697,31 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   const int tid = threadIdx.x;
   const rsrc_t zev = make_rsrc(a.ws_even + (int64_t)bw * C::M, C::M * 8u);
   const uint32_t vo = tid * 8u;
+  // v: the transform's data; between phases it carries the next phase's
+  // filter (bins tid + T m), loaded after the transform and before that
+  // phase's output stores (see stage_own)
+  float2 v[C::P], x[C::P];
+  if constexpr (!kPacked) {
+    // the block's samples (core.py:761: y[j] = x[2j] + i x[2j+1]), loaded
+    // before the table setup so that the loads overlap it; int16 samples
+    // stay raw bits in v until phase 0 converts them (a conversion here
+    // would wait for the loads)
+    const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(C::N * a.in_bytes));
+    if (a.in_bytes == 4) {
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) v[m] = bload2(in, vo, m * C::StrideB);
+    } else if ((s0 & 1) == 0) {
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) v[m].x = bload1(in, tid * 4u, m * (C::T * 4u));  // both samples
+    } else {  // odd start: 16-bit loads
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) {
+        uint32_t o = tid * 4u + m * (C::T * 4u);
+        v[m].x = __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o, 0, 0));
+        v[m].y = __uint_as_float((uint32_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o + 2, 0, 0));
+      }
+    }
+  }
   {  // twiddle tables into LDS
     const rsrc_t src = make_rsrc(a.tw_lds, C::Tw::count * 8u);
     float2* dst = reinterpret_cast<float2*>(&twl);
@@ -716,10 +741,6 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   __syncthreads();
   const TwSrcLdsT<typename C::Tw> twf{&twl};
   const float2 wt0 = bload2(make_rsrc(a.tw_split, C::T * 8u), vo, 0u);
-  // v: the transform's data; between phases it carries the next phase's
-  // filter (bins tid + T m), loaded after the transform and before that
-  // phase's output stores (see stage_own)
-  float2 v[C::P], x[C::P];
   float xM = 0.0f;
   const int last = kPhPost + a.npost;
   const bool has_hpf = a.hpf.out != nullptr;
@@ -766,26 +787,16 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
     asm volatile("" : "+v"(wt.x), "+v"(wt.y), "+v"(tidv));
     // ---------------------------------------------------------- pre
     if (ph == kPhIn) {
-      // spectrum of the block (core.py:761): y[j] = x[2j] + i x[2j+1]
-      const rsrc_t in = make_rsrc((const char*)a.in + s0 * a.in_bytes, (uint32_t)(C::N * a.in_bytes));
-      if constexpr (kPacked) {
-        // decoded before the phase loop (see the prologue)
-      } else if (a.in_bytes == 4) {
+      // the block's samples are in v (loaded or decoded before the phase
+      // loop); int16 samples from their raw bits
+      if constexpr (!kPacked) {
+        if (a.in_bytes != 4) {
+          const bool pair = (s0 & 1) == 0;
 #pragma unroll
-        for (int m = 0; m < C::P; ++m) v[m] = bload2(in, vo, m * C::StrideB);
-      } else if ((s0 & 1) == 0) {
-#pragma unroll
-        for (int m = 0; m < C::P; ++m) {
-          uint32_t w = __builtin_bit_cast(uint32_t, bload1(in, tid * 4u, m * (C::T * 4u)));
-          v[m] = make_float2((float)(int16_t)(w & 0xffffu), (float)(int16_t)(w >> 16));
-        }
-      } else {  // odd start: 16-bit loads
-#pragma unroll
-        for (int m = 0; m < C::P; ++m) {
-          uint32_t o = tid * 4u + m * (C::T * 4u);
-          int16_t lo = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o, 0, 0);
-          int16_t hi = (int16_t)__builtin_amdgcn_raw_buffer_load_b16(in, (int)o + 2, 0, 0);
-          v[m] = make_float2((float)lo, (float)hi);
+          for (int m = 0; m < C::P; ++m) {
+            const uint32_t w0 = __float_as_uint(v[m].x), w1 = pair ? (w0 >> 16) : __float_as_uint(v[m].y);
+            v[m] = make_float2((float)(int16_t)(w0 & 0xffffu), (float)(int16_t)(w1 & 0xffffu));
+          }
         }
       }
     } else if (ph == kPhHpf) {
@@ -816,18 +827,23 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       // conjugates is minus the wanted increment.  Odd samples go to LDS
       // shifted by one so each thread reads z[2j-1] at its own index j.
       const LdsPair sh(s.re, s.im, pidx(tidv + 1)), own(s.re, s.im, pidx(tidv));
+      // the even samples' loads go out first, so that their latency
+      // overlaps the exchange and its barrier; every one of them comes
+      // before any demod_raw store (the in-order memory counter never
+      // makes a load wait for a store)
+      float2 ev[C::P];
+#pragma unroll
+      for (int m = 0; m < C::P; ++m) ev[m] = bload2(zev, vo, m * C::StrideB);
       static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         sh.template st<POff<C::T * m>::value>(v[m]);
       });
       __syncthreads();
       const float hz_per_rad = a.fs / kTwoPi;
-      // every even-sample load comes before any demod_raw store, so the
-      // in-order memory counter never makes a load wait for a store
       static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
         const int j = tidv + C::T * m;
-        float2 e = bload2(zev, vo, m * C::StrideB), od = v[m];
+        float2 e = ev[m], od = v[m];
         float2 prev = own.template ld<POff<C::T * m>::value>();
         float2 p = cmulc(e, prev);
         float h0 = (j > 0) ? inc_to_hz(-disc_atan2(p.y, p.x), hz_per_rad, a.fs) : 0.0f;
