@@ -27,8 +27,11 @@ namespace ldgpu {
 constexpr int kEdgeThreads = 256;
 constexpr int kEdgePer = 16;                          // samples per thread
 constexpr int kEdgeTile = kEdgeThreads * kEdgePer;    // samples per workgroup
-constexpr int kScanThreads = 1024;
+constexpr int kScanThreads = 256;                     // small: the scan finds a CU beside other kernels
 constexpr int kMeanMax = 5000;                        // findpulses' longest pulse
+constexpr int kMeanStage = 1280;                      // pulses up to this long are staged in LDS
+constexpr int kMeanPulses = 16;                       // pulses per wavefront of the fused mean kernel
+constexpr long long kMeanWgMax = 8192;                // its workgroups per job
 constexpr int kMeanWaves = 4;                         // pulses per workgroup
 constexpr int kFactThreads = 256;
 constexpr int kFactMax = 8192;                        // longest skip_check line slice
@@ -90,9 +93,11 @@ __global__ __launch_bounds__(kEdgeThreads) void k_edge_count(const float* __rest
 }
 
 // Exclusive scan of the per-tile counts in place; the total goes to *total.
-// Each thread sums a run of consecutive counts (four loads in flight), the
-// wavefronts scan their runs with shuffles, and one wavefront scans the
-// wavefront totals: two workgroup barriers.
+// Each thread sums a run of consecutive counts (up to kScanPer of them held
+// in registers, all loads in flight at once; longer runs four at a time and
+// read again for the write), the wavefronts scan their runs with shuffles,
+// and one wavefront scans the wavefront totals: two workgroup barriers.
+constexpr int kScanPer = 32;
 __device__ __forceinline__ void edge_scan_wg(long long* __restrict__ counts, long long ntiles,
                                              long long* __restrict__ total) {
   __shared__ long long wsum[kScanThreads / 64];
@@ -100,13 +105,22 @@ __device__ __forceinline__ void edge_scan_wg(long long* __restrict__ counts, lon
   const long long per = (ntiles + kScanThreads - 1) / kScanThreads;
   const long long b = (long long)t * per;
   const long long e = b + per < ntiles ? b + per : ntiles;
+  const bool held = per <= kScanPer;  // uniform
+  long long c[kScanPer];
   long long s = 0;
   long long i = b;
-  for (; i + 4 <= e; i += 4) {
-    const long long c0 = counts[i], c1 = counts[i + 1], c2 = counts[i + 2], c3 = counts[i + 3];
-    s += (c0 + c1) + (c2 + c3);
+  if (held) {
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) c[q] = b + q < e ? counts[b + q] : 0;
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) s += c[q];
+  } else {
+    for (; i + 4 <= e; i += 4) {
+      const long long c0 = counts[i], c1 = counts[i + 1], c2 = counts[i + 2], c3 = counts[i + 3];
+      s += (c0 + c1) + (c2 + c3);
+    }
+    for (; i < e; ++i) s += counts[i];
   }
-  for (; i < e; ++i) s += counts[i];
   long long inc = s;  // inclusive scan over the wavefront
   for (int o = 1; o < 64; o <<= 1) {
     const long long v = __shfl_up(inc, o);
@@ -124,10 +138,18 @@ __device__ __forceinline__ void edge_scan_wg(long long* __restrict__ counts, lon
   }
   __syncthreads();
   long long run = inc - s + (w > 0 ? wsum[w - 1] : 0);
-  for (i = b; i < e; ++i) {
-    const long long c = counts[i];
-    counts[i] = run;
-    run += c;
+  if (held) {
+#pragma unroll
+    for (int q = 0; q < kScanPer; ++q) {
+      if (b + q < e) counts[b + q] = run;
+      run += c[q];
+    }
+  } else {
+    for (i = b; i < e; ++i) {
+      const long long ci = counts[i];
+      counts[i] = run;
+      run += ci;
+    }
   }
   if (t == kScanThreads - 1) *total = wsum[kScanThreads / 64 - 1];
 }
@@ -284,14 +306,15 @@ __device__ float np_tree(int n, const float* leaf, int& idx) {
 }
 
 struct MeanScratch {
-  float stage[kMeanMax];
+  float stage[kMeanStage];
   float leaf[64];
   int loff[64], llen[64];
 };
 
-// numpy's float32 mean of s->stage[0, m) by the calling wavefront; the
+// numpy's float32 mean of at(0 .. m-1) by the calling wavefront; the
 // result is valid in lane 0.
-__device__ float wave_np_mean(MeanScratch* sc, int m, int lane) {
+template <class F>
+__device__ float wave_np_mean_at(MeanScratch* sc, F at, int m, int lane) {
   if (lane == 0) {
     int cnt = 0;
     np_leaves<6>(0, m, sc->loff, sc->llen, cnt);
@@ -301,11 +324,7 @@ __device__ float wave_np_mean(MeanScratch* sc, int m, int lane) {
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
   const int n = sc->llen[lane];
-  if (n > 0) {
-    const float* st = sc->stage;
-    auto at = [st](int i) { return st[i]; };
-    sc->leaf[lane] = np_block_sum(at, sc->loff[lane], n);
-  }
+  if (n > 0) sc->leaf[lane] = np_block_sum(at, sc->loff[lane], n);
   __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
   __builtin_amdgcn_wave_barrier();
   __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
@@ -315,6 +334,17 @@ __device__ float wave_np_mean(MeanScratch* sc, int m, int lane) {
     r = np_tree<6>(m, sc->leaf, idx) / (float)m;
   }
   return r;
+}
+
+// The mean of src[0, m) (m <= kMeanMax): staged in LDS when it fits (the
+// leaf sums then read LDS), else the leaves read the samples in place.
+__device__ float pulse_mean(MeanScratch* sc, const float* __restrict__ src, int m, int lane) {
+  if (m <= kMeanStage) {
+    for (int i = lane; i < m; i += 64) sc->stage[i] = src[i];
+    const float* st = sc->stage;
+    return wave_np_mean_at(sc, [st](int i) { return st[i]; }, m, lane);
+  }
+  return wave_np_mean_at(sc, [src](int i) { return src[i]; }, m, lane);
 }
 
 // 64 consecutive pulses per wavefront: lane i reads pulse k0 + i's edges and
@@ -350,9 +380,7 @@ __global__ __launch_bounds__(64 * kMeanWaves) void k_pulse_means(const float* __
       const int j = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
       const long long aj = __shfl(a, j), bj = __shfl(b, j);
-      const int m = (int)(bj - aj);
-      for (int i = lane; i < m; i += 64) sc->stage[i] = x[aj + i];
-      const float r = wave_np_mean(sc, m, lane);
+      const float r = pulse_mean(sc, x + aj, (int)(bj - aj), lane);
       if (lane == 0) means[k0 + j] = r;
       __builtin_amdgcn_wave_barrier();
     }
@@ -430,10 +458,11 @@ __global__ __launch_bounds__(kEdgeThreads) void k_bits_write_multi(PulseSearchAr
   bits_write_tile(d.bits, d.n, d.tiles, d.edges, d.cap, (long long)blockIdx.x - a.tile0[j]);
 }
 
-// 64 pulses per one-wavefront workgroup (a small LDS footprint, so that the
-// workgroups find room beside other kernels'), as k_pulse_means, with each
-// edge and level also written to the host lists: edges [0, min(count, cap)), levels of the
-// pulses [0, min(count, cap) / 2), and the count.
+// kMeanPulses pulses per one-wavefront workgroup and pass (a small LDS
+// footprint and a short chain of means, so that the workgroups find room
+// beside other kernels' and finish soon), as k_pulse_means, with each edge
+// and level also written to the host lists: edges [0, min(count, cap)),
+// levels of the pulses [0, min(count, cap) / 2), and the count.
 __global__ __launch_bounds__(64) void k_means_export_multi(PulseSearchArgs a) {
   __shared__ MeanScratch scratch[1];
   const int j = job_of(a.mwg0, a.njobs, blockIdx.x);
@@ -445,12 +474,12 @@ __global__ __launch_bounds__(64) void k_means_export_multi(PulseSearchArgs a) {
   const long long wg = (long long)blockIdx.x - a.mwg0[j];
   if (wg == 0 && threadIdx.x == 0) *d.hcount = total;
   MeanScratch* sc = &scratch[0];
-  const long long step = (a.mwg0[j + 1] - a.mwg0[j]) * 64;
-  for (long long k0 = wg * 64; 2 * k0 < ne; k0 += step) {
+  const long long step = (a.mwg0[j + 1] - a.mwg0[j]) * kMeanPulses;
+  for (long long k0 = wg * kMeanPulses; 2 * k0 < ne; k0 += step) {
     const long long k = k0 + lane;
     long long lo = 0, hi = 0;
     bool take = false;
-    if (2 * k < ne) {
+    if (lane < kMeanPulses && 2 * k < ne) {
       const long long st = d.edges[2 * k];
       d.hedges[2 * k] = st;
       if (k < np) {
@@ -471,9 +500,7 @@ __global__ __launch_bounds__(64) void k_means_export_multi(PulseSearchArgs a) {
       const int q = __ffsll((long long)todo) - 1;
       todo &= todo - 1;
       const long long aq = __shfl(lo, q), bq = __shfl(hi, q);
-      const int m = (int)(bq - aq);
-      for (int i = lane; i < m; i += 64) sc->stage[i] = d.x[aq + i];
-      const float r = wave_np_mean(sc, m, lane);
+      const float r = pulse_mean(sc, d.x + aq, (int)(bq - aq), lane);
       if (lane == 0) {
         d.means[k0 + q] = r;
         d.hmeans[k0 + q] = r;
@@ -809,8 +836,8 @@ extern "C" int ld_pulse_search(ld_handle* h, const ld_pulse_job* jobs, int njobs
     d.ntiles = tiles;
     ws += tiles + 1;
     a.tile0[i + 1] = a.tile0[i] + tiles;
-    long long mwg = (q.cap / 2 + 63) / 64;
-    if (mwg > 2048) mwg = 2048;
+    long long mwg = (q.cap / 2 + kMeanPulses - 1) / kMeanPulses;
+    if (mwg > kMeanWgMax) mwg = kMeanWgMax;
     a.mwg0[i + 1] = a.mwg0[i] + mwg;
   }
   for (int i = njobs; i < kMaxPulseJobs; ++i) {
