@@ -397,6 +397,58 @@ def trace_summary(db, top=40):
     return "\n".join(out)
 
 
+def pass_timeline(db, kernel, waves, steps, isolated=5, top=12):
+    """Markdown summary of the timed C4 passes in a rocprofv3 --kernel-trace
+    database of a default bench run: the headline demod kernel's last
+    `steps * waves` launches before its `isolated` lone ones bound the window;
+    reported per pass: the window, the time any kernel runs (union of
+    dispatch intervals), each kernel's summed duration (kernels overlap across
+    the two streams, so these add up to more than the window), and the gaps
+    between consecutive demod launches with what starts inside them."""
+    import sqlite3
+    con = sqlite3.connect(str(db))
+    ks = sorted(con.execute("select start, end, name from kernels").fetchall())
+    dem = [k for k in ks if k[2].split("(")[0].replace("void ", "").endswith(kernel)]
+    need = steps * waves + isolated
+    if len(dem) < need:
+        return f"(only {len(dem)} {kernel} launches, {need} needed for {steps} passes of {waves} waves)"
+    timed = dem[-need:-isolated]
+    t0, t_next = timed[0][0], dem[-isolated][0]
+    win = [k for k in ks if t0 <= k[0] < t_next]
+    t1 = max(k[1] for k in win)
+    busy, cs, ce = 0, None, None
+    for s, e, _ in win:
+        if ce is None or s > ce:
+            if ce is not None:
+                busy += ce - cs
+            cs, ce = s, e
+        else:
+            ce = max(ce, e)
+    busy += ce - cs
+    tot = {}
+    for s, e, n in win:
+        key = n.split("(")[0].replace("void ", "")[:60]
+        v = tot.setdefault(key, [0, 0])
+        v[0] += e - s
+        v[1] += 1
+    gaps, ing = [], {}
+    for (s0, e0, _), (s1, _, _) in zip(timed, timed[1:]):
+        gaps.append(max(s1 - e0, 0))
+        for s, e, n in win:
+            if e0 <= s < s1:
+                key = n.split("(")[0].replace("void ", "")[:60]
+                ing[key] = ing.get(key, 0) + min(e, s1) - s
+    ps = 1e6 * steps
+    out = [f"Timed passes: {steps} x {waves} waves; per pass: window {(t1 - t0) / ps:.1f} ms, some kernel running "
+           f"{busy / ps:.1f} ms; {kernel} launches {sum(e - s for s, e, _ in timed) / ps:.1f} ms; gaps between "
+           f"consecutive demod launches {sum(gaps) / ps:.1f} ms (median {sorted(gaps)[len(gaps) // 2] / 1e3:.0f} us).",
+           "", "| kernel | ms per pass (summed durations) | calls per pass | avg us | in the demod gaps, ms per pass |",
+           "|---|---|---|---|---|"]
+    for key, (d, c) in sorted(tot.items(), key=lambda kv: -kv[1][0])[:top]:
+        out.append(f"| {key} | {d / ps:.2f} | {c / steps:.0f} | {d / c / 1e3:.1f} | {ing.get(key, 0) / ps:.2f} |")
+    return "\n".join(out)
+
+
 def _pmc_run(counters, target, workdir, tag):
     """One rocprofv3 counter pass (the counters collected together) over
     `bench.py <target>` in a child process; returns {kernel name: {counter:
@@ -607,9 +659,16 @@ def main():
     ap.add_argument("--traffic-out", default=None, help="where --measure-traffic / --measure-mfma write their JSON")
     ap.add_argument("--trace-summary", default=None, metavar="DB",
                     help="print the kernel table of a rocprofv3 --kernel-trace database and exit")
+    ap.add_argument("--trace-waves", type=int, default=0,
+                    help="with --trace-summary: also summarise the timed C4 passes (waves per pass, --steps passes)")
+    ap.add_argument("--trace-kernel", default="k_demod_batch8",
+                    help="with --trace-waves: the headline demod kernel whose launches bound the passes")
     args = ap.parse_args()
     if args.trace_summary:
         print(trace_summary(args.trace_summary))
+        if args.trace_waves > 0:
+            print()
+            print(pass_timeline(args.trace_summary, args.trace_kernel, args.trace_waves, args.steps))
         return
     if (args.profile_step or args.measure_traffic) and "auto" in (args.blocklen, args.fft):
         ap.error("--profile-step / --measure-traffic take a fixed --blocklen and --fft")
