@@ -191,6 +191,48 @@ def test_pulse_search_many_buffers(gpu_rf):
     assert finders[2].cap < int(finders[2].h_count[0])
 
 
+def test_pulse_means_long_pulses(gpu_rf):
+    """Pulse levels whose trimmed bodies straddle the 1280 samples the mean
+    kernels stage in LDS (longer ones are read in place) up to the 5000-sample
+    limit (findpulses' longest): numpy's float32 pairwise mean of each body,
+    bit for bit, from the single-buffer finder and from ld_pulse_search."""
+    import torch
+    rf = gpu_rf("NTSC", audio=False)
+    trim = int(rf.freq)
+    rng = np.random.default_rng(11)
+    lengths = [500, 1000, 1279 + 2 * trim, 1280 + 2 * trim, 1281 + 2 * trim, 1409 + 2 * trim, 2000, 4096,
+               4999, 5000, 5001, 7000]
+    gap = 3000
+    n = gap + sum(L + gap for L in lengths) + 123
+    x = (1.0 + 0.05 * rng.standard_normal(n)).astype(np.float32)
+    pos = gap
+    for L in lengths:
+        x[pos:pos + L] = (0.1 + 0.05 * rng.standard_normal(L)).astype(np.float32)
+        pos += L + gap
+    high = 0.5
+    want = _edges_np(x, high)
+    assert len(want) == 2 * len(lengths)
+    lvl = _means_np(x, want, 10.0 * ((rf.freq / rf.linelen) * rf.linelen), rf.freq)
+    assert np.isnan(lvl[-2:]).all() and not np.isnan(lvl[:-2]).any()    # 5001 and 7000: past the limit
+    e, m = _pulses_gpu(rf, x, high)
+    np.testing.assert_array_equal(e, want)
+    np.testing.assert_array_equal(np.isnan(m), np.isnan(lvl))
+    np.testing.assert_array_equal(m[~np.isnan(m)], lvl[~np.isnan(lvl)])
+    # the fused multi-buffer search, from the in-pulse bits of the same samples
+    xt = torch.from_numpy(x).cuda()
+    bits = torch.from_numpy(_pack_bits(x.astype(np.float64) <= high)).cuda()
+    pf = locate.PulseFinder(rf, n)
+    searches = [(pf, xt, n, bits)]
+    ws = torch.empty(locate.pulse_search_workspace(searches), dtype=torch.uint8, device="cuda")
+    locate.pulse_search(searches, ws, torch.cuda.current_stream())
+    torch.cuda.synchronize()
+    assert int(pf.h_count[0]) == len(want)
+    np.testing.assert_array_equal(pf.h_edges.numpy()[:len(want)], want)
+    got = pf.h_means.numpy()[:len(lengths)]
+    np.testing.assert_array_equal(np.isnan(got), np.isnan(lvl))
+    np.testing.assert_array_equal(got[~np.isnan(got)], lvl[~np.isnan(lvl)])
+
+
 def test_range_means_exact(gpu_rf):
     import torch
     rf = gpu_rf("NTSC", audio=False)
