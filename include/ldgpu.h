@@ -255,9 +255,9 @@ typedef struct ld_tbc_field {
   int64_t demod_len;      /* samples readable from there (taps outside read 0) */
   int64_t lineloc_offset; /* first entry of this field in the linelocs array */
   int32_t nlines;         /* line locations (proclines), 2..1024 */
-  int32_t lineoffset;     /* Field.lineoffset (core.py:1614, :3231) */
-  int32_t linesout;       /* output lines (outlinecount, core.py:1624) */
-  int32_t outwidth;       /* output pixels per line (outlinelen) */
+  int32_t lineoffset;     /* Field.lineoffset (core.py:1614, :3231), 0..1024 */
+  int32_t linesout;       /* output lines (outlinecount, core.py:1624), 1..1024 */
+  int32_t outwidth;       /* output pixels per line (outlinelen), 1..2048 */
   double inlinelen;       /* Field.inlinelen = RFDecode.linelen (core.py:1609) */
   int64_t out_offset;     /* first output pixel of this field */
 } ld_tbc_field;

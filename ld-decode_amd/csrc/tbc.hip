@@ -798,7 +798,7 @@ int ld_tbc_fields(ld_handle* h, const float* demod, int64_t demod_total, const l
   for (int f = 0; f < nfields; ++f) {
     const ld_tbc_field& fd = fields[f];
     if (fd.nlines < 2 || fd.nlines > kLinesMax || fd.outwidth <= 0 || fd.outwidth > kTbcMaxWidth ||
-        fd.linesout <= 0 || fd.linesout > kMaxLinesOut || fd.lineoffset < 0 ||
+        fd.linesout <= 0 || fd.linesout > kMaxLinesOut || fd.lineoffset < 0 || fd.lineoffset > kLinesMax ||
         !(fd.inlinelen > 0) || fd.lineloc_offset < 0 || fd.lineloc_offset + fd.nlines > nlinelocs) {
       set_error("ld_tbc_fields: field %d: bad geometry (nlines %d, max %d) or line range", f, fd.nlines, kLinesMax);
       return LD_EINVAL;

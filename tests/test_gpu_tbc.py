@@ -10,7 +10,7 @@ float32-coordinate (reference) and FP64-coordinate modes.
 import numpy as np
 import pytest
 
-from lddecode_amd import capture, synth
+from lddecode_amd import _lib, capture, synth
 from lddecode_amd import field as fieldmod
 from oracle import dsp as O
 from oracle import tbc as T
@@ -161,6 +161,26 @@ def test_batched_fields_equal_single(gpu_rf):
         singles.append(_np(u))
     _, u16 = fieldmod.tbc_fields(rf, demod, fields, torch.from_numpy(np.concatenate(lls)).cuda())
     np.testing.assert_array_equal(_np(u16), np.concatenate(singles))
+
+
+def test_tbc_fields_rejects_bad_geometry(gpu_rf):
+    """ld_tbc_fields checks every field before it launches anything: a line
+    offset outside 0..1024, output widths outside 1..2048, too few line
+    locations and a demod span past the buffer are errors with their codes."""
+    import torch
+    rf = gpu_rf("PAL", audio=False)
+    demod = torch.zeros(900_000, dtype=torch.float32, device="cuda")
+    ll = torch.from_numpy(3000 + np.arange(326) * 2560.0).cuda()
+    good = dict(demod_offset=0, demod_len=900_000, lineloc_offset=0, nlines=326, lineoffset=2, linesout=313,
+                outwidth=1135, inlinelen=2560.0, out_offset=0)
+    fieldmod.tbc_fields(rf, demod, [good], ll)
+    for bad, code in ((dict(lineoffset=1025), _lib.LD_EINVAL), (dict(lineoffset=-1), _lib.LD_EINVAL),
+                      (dict(outwidth=2049), _lib.LD_EINVAL), (dict(outwidth=0), _lib.LD_EINVAL),
+                      (dict(nlines=1), _lib.LD_EINVAL), (dict(demod_len=900_001), _lib.LD_ERANGE)):
+        with pytest.raises(_lib.LdGpuError) as e:
+            fieldmod.tbc_fields(rf, demod, [dict(good, **bad)], ll)
+        assert e.value.code == code, bad
+    torch.cuda.synchronize()
 
 
 def test_capture_decoder_end_to_end(gpu_rf):
