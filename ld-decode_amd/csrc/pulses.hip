@@ -535,6 +535,7 @@ struct FactArgs {
   double line_samples;
   int outlinecount;
   int pal;
+  int smax;  // samples the dynamic LDS holds: every slice's length
 };
 
 // Order-preserving unsigned key of a float (finite values).
@@ -590,7 +591,7 @@ __device__ unsigned radix_select(const unsigned* s, int m, int k, unsigned* hist
 // line slice j, as numba's np.median (exact order statistics by radix
 // select); j == 0 also stores ll_linecount.
 __global__ __launch_bounds__(kFactThreads) void k_field_facts(FactArgs a, const ld_line_field* __restrict__ fields) {
-  __shared__ unsigned s[kFactMax];
+  extern __shared__ unsigned s[];  // a.smax keys (a slice of about one line, sized by the host)
   __shared__ unsigned hist[kFactThreads];
   __shared__ unsigned shared[4];
   const int f = blockIdx.x >> 3, j = blockIdx.x & 7;
@@ -614,7 +615,7 @@ __global__ __launch_bounds__(kFactThreads) void k_field_facts(FactArgs a, const 
     if (b >= 0.0 && isfinite(b)) np_slice((long long)b, (long long)(b + a.line_samples + 1.0), fd.demod_len, lo, hi);
   }
   const long long m = hi - lo;
-  if (m > 0 && m <= kFactMax) {
+  if (m > 0 && m <= a.smax) {
     const float* src = a.demod + fd.demod_offset + lo;
     for (int i = t; i < m; i += kFactThreads) s[i] = fkey(src[i]);
     __syncthreads();
@@ -909,8 +910,13 @@ extern "C" int ld_field_facts_measure(ld_handle* h, const float* demod, int64_t 
   // a pageable source is staged before hipMemcpyAsync returns
   ld_line_field* dfields = (ld_line_field*)workspace;
   LD_HIP(hipMemcpyAsync(dfields, fields, sizeof(ld_line_field) * nfields, hipMemcpyHostToDevice, st));
-  FactArgs a{demod, linelocs, facts, line_samples, outlinecount, pal ? 1 : 0};
-  hipLaunchKernelGGL(k_field_facts, dim3(nfields * 8), dim3(kFactThreads), 0, st, a, (const ld_line_field*)dfields);
+  // a slice int(b) .. int(b + line_samples + 1) holds at most
+  // ceil(line_samples) + 2 samples: the LDS is sized for that (about 10 KB
+  // at 40 MSPS instead of kFactMax's 32 KB, so that more workgroups fit)
+  const int smax = (int)ceil(line_samples) + 3 < kFactMax ? (int)ceil(line_samples) + 3 : kFactMax;
+  FactArgs a{demod, linelocs, facts, line_samples, outlinecount, pal ? 1 : 0, smax};
+  hipLaunchKernelGGL(k_field_facts, dim3(nfields * 8), dim3(kFactThreads), sizeof(unsigned) * (size_t)smax, st, a,
+                     (const ld_line_field*)dfields);
   LD_HIP(hipGetLastError());
   return LD_OK;
 }

@@ -393,3 +393,62 @@ def test_field_compute_linelocs(gpu_rf, system):
     np.testing.assert_array_equal(ll, b["res"].linelocs1)
     assert fb.line0loc == b["res"].line0loc and fb.sync_confidence == b["res"].sync_confidence
     assert off == b["res"].nextfieldoffset
+
+
+@pytest.mark.parametrize("system", ["NTSC", "PAL"])
+def test_field_facts_match_oracle(gpu_rf, system):
+    """ld_field_facts_measure (skip_check's eight line medians and the final
+    location of line linecount) against the oracle's field_facts on the same
+    buffers, bit for bit: first and second fields, repeated samples (ties in
+    the order statistics), and fields too short for some of the lines
+    (NaN medians, NaN linecount location)."""
+    import torch
+    rf = gpu_rf(system, audio=False)
+    p = LL.LocParams.from_rf(rf)
+    length = p.usectoinpx(p.line_period)
+    oc = p.outlinecount
+    lo_of = (lambda first: (2 if first else 3)) if system == "PAL" else (lambda first: 0)
+    lc_of = (lambda first: 312 if first else 313) if system == "PAL" else (lambda first: 263 if first else 262)
+    rng = np.random.default_rng(21)
+    ire0, hz_ire = rf.DecoderParams["ire0"], rf.DecoderParams["hz_ire"]
+    specs = [(p.proclines, 1), (p.proclines, 0), (oc + 4, 1), (oc + 12, 0)]
+    demods, lls, arr = [], [], (_lib.LineField * len(specs))()
+    off = loff = 0
+    for f, (nlines, first) in enumerate(specs):
+        span = int((nlines + 3) * p.linelen)
+        d = (ire0 + hz_ire * rng.uniform(-45.0, 105.0, span)).astype(np.float32)
+        d[::7] = d[3]                                  # ties
+        final = 100.0 + np.arange(nlines) * (p.linelen + 0.37) + rng.uniform(-3.0, 3.0, nlines)
+        r = arr[f]
+        r.demod_offset, r.demod_len, r.lineloc_offset, r.nlines = off, span, loff, nlines
+        r.linecount, r.first_field, r.chain = 0, first, 0
+        demods.append(d)
+        lls.append(final)
+        off += span
+        loff += nlines
+    dt = torch.from_numpy(np.concatenate(demods)).cuda()
+    lt = torch.from_numpy(np.concatenate(lls)).cuda()
+    n = len(specs)
+    facts = torch.empty(n * locate.FACTS_DTYPE.itemsize, dtype=torch.uint8, device="cuda")
+    ws = torch.empty(rf.lib.ld_field_facts_workspace_bytes(rf.handle, n), dtype=torch.uint8, device="cuda")
+    _lib.check(rf.lib.ld_field_facts_measure(rf.handle, dt.data_ptr(), dt.numel(), arr, n, lt.data_ptr(), lt.numel(),
+                                             1 if system == "PAL" else 0, oc, float(length), facts.data_ptr(),
+                                             ws.data_ptr(), ws.numel(), None))
+    got = facts.cpu().numpy().view(locate.FACTS_DTYPE)
+    for f, (nlines, first) in enumerate(specs):
+        d, final = demods[f], lls[f]
+        want = []
+        for j in range(8):
+            line = oc + j + lo_of(first)
+            b = final[line] if line < nlines else None
+            want.append(LL.numba_median(d[int(b):int(b + length + 1)]) if b is not None else np.nan)
+        lc = lc_of(first)
+        want_llc = final[lc] if lc < nlines else np.nan
+        if nlines == p.proclines:                     # the oracle's own function, where it applies
+            llc_o, meds_o = LL.field_facts(p, d, final, bool(first))
+            assert llc_o == want_llc and meds_o == want
+        np.testing.assert_array_equal(got["medians"][f], np.array(want), err_msg=f"field {f}")
+        np.testing.assert_array_equal(got["ll_linecount"][f], want_llc, err_msg=f"field {f}")
+        assert got["known"][f] == 1
+        if nlines == oc + 4:
+            assert np.isnan(got["medians"][f][-1]) and not np.isnan(got["medians"][f][0])
