@@ -137,6 +137,10 @@ struct TwSrcLdsT {
   __device__ __forceinline__ float2 get(int k, int i) const {
     return t->template get<NS>(k, i);
   }
+  template <int NS, std::enable_if_t<HasTwRow<TW, NS>::value, int> = 0>
+  __device__ __forceinline__ float2 row(int k, int r) const {
+    return t->template row<NS>(k, r);
+  }
 };
 using TwSrcLds = TwSrcLdsT<TwLds>;
 
@@ -174,24 +178,29 @@ struct TwLds8k {
 static_assert(TwLds8k::count <= TwLds::count, "the handle table holds either image");
 
 // The 4096-point, 256-thread transform's pass twiddles: passes (R 16, NS 16),
-// (R 16, NS 256 = 16 h + l).
+// its full rows W_256^{r k} (r, k < 16) stored by r (a read's lanes take
+// consecutive k: no bank conflicts), and (R 16, NS 256 = 16 h + l).
 struct TwLds4k {
-  float2 p16[16 * 4];
+  float2 p16[16 * 16];
   float2 a256[16 * 4];
   float2 b256[16 * 4];
-  static constexpr int count = 16 * 4 * 3;
+  static constexpr int count = 16 * 16 + 16 * 4 * 2;
   template <int NS>
   __device__ __forceinline__ float2 get(int k, int i) const {
     if constexpr (NS == 16) {
-      return p16[k * 4 + i];
+      return p16[(1 << i) * 16 + k];
     } else {
       static_assert(NS == 256, "M-point transform passes");
       return cmul(a256[(k >> 4) * 4 + i], b256[(k & 15) * 4 + i]);
     }
   }
+  template <int NS, std::enable_if_t<NS == 16, int> = 0>
+  __device__ __forceinline__ float2 row(int k, int r) const {
+    return p16[r * 16 + k];
+  }
   static void fill(float2* t, float2 (*cexp)(double, double)) {
-    for (int k = 0; k < 16; ++k)
-      for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * k, 256);
+    for (int r = 0; r < 16; ++r)
+      for (int k = 0; k < 16; ++k) *t++ = cexp((double)r * k, 256);
     for (int h = 0; h < 16; ++h)
       for (int i = 0; i < 4; ++i) *t++ = cexp((double)(1 << i) * 16 * h, 4096);
     for (int l = 0; l < 16; ++l)
@@ -199,8 +208,10 @@ struct TwLds4k {
   }
 };
 
+static_assert(TwLds4k::count <= TwLds::count, "the handle table holds either image");
+
 template <int M>
-struct LdsT {
+struct alignas(16) LdsT {
   float re[LdsSize<M>::floats];
   float im[LdsSize<M>::floats];
 };
@@ -688,9 +699,7 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
   __shared__ MfmaTw mtw;
   __shared__ float2 aslice[2][C::NA / 2 + 1];
   __shared__ typename C::Tw twl;
-  __shared__ float notch_mag[C::NotchMax];
   __shared__ int notch_flag[C::NotchMax + 4];
-  __shared__ double notch_red[128];
   __shared__ uint32_t notch_mask[C::T];  // bit m: bin tid + T m zeroed by the spur notch
   // bin M of every filter table: 0 rfhpf, 1 RFVideo, 2.. post filters, 6 EFM
   __shared__ float2 bin_m[2 + 4 + 1];
@@ -885,6 +894,14 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       r2c_split<C>(v, x, xM, s, wt, tidv);
       if (ph == kPhIn) {
         if (a.notch_hi > a.notch_lo) {  // PAL spur notch decision (core.py:774-798)
+          // the magnitudes and the reduction use the exchange area, free
+          // from the split's last barrier until the next phase's first LDS
+          // store, which every thread reaches only after the decision's
+          // second barrier (the flags, read after it, have their own array)
+          float* notch_mag = s.re;
+          double* notch_red = reinterpret_cast<double*>(s.im);
+          static_assert(C::NotchMax <= LdsSize<C::M>::floats && 128 * 2 <= LdsSize<C::M>::floats,
+                        "notch scratch within the exchange area");
           static_for<0, C::P>([&](auto mc) {
             constexpr int m = decltype(mc)::value;
             const int k = tid + C::T * m;

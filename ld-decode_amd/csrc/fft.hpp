@@ -229,6 +229,15 @@ __device__ __forceinline__ void lds_barrier() {
 }
 
 // ------------------------------------------------------------ Stockham pass
+// Twiddle sources may hold a pass's full rows, W^{r k} for every r < R
+// (member template row<NS>(k, r), defined only for those passes): the pass
+// then reads each twiddle instead of forming the powers of the base ones.
+template <class TWF, int NS, class = void>
+struct HasTwRow : std::false_type {};
+template <class TWF, int NS>
+struct HasTwRow<TWF, NS, std::void_t<decltype(std::declval<const TWF&>().template row<NS>(0, 0))>>
+    : std::true_type {};
+
 // Pass of radix R after NS points have been combined.  TW is this pass's
 // twiddle rows: TW[k*log2(R) + i] = exp(-2*pi*i*2^i*k/(NS*R)) for k < NS
 // (unused when NS == 1); DIR = -1 forward, +1 inverse (conjugated twiddles).  FROM_LDS: inputs come from LDS (else from registers in natural
@@ -262,7 +271,14 @@ __device__ __forceinline__ void stockham_pass(float2 (&v)[N / T], float* __restr
     float2 u[R];
 #pragma unroll
     for (int r = 0; r < R; ++r) u[r] = v[q + r * Q];
-    if constexpr (NS > 1) {
+    if constexpr (NS > 1 && HasTwRow<TWF, NS>::value) {
+      static_for_chunked<1, R, 8>([&](auto rc) {
+        constexpr int r = decltype(rc)::value;
+        float2 w = twf.template row<NS>(k, r);
+        if constexpr (DIR > 0) w.y = -w.y;
+        u[r] = cmul(u[r], w);
+      });
+    } else if constexpr (NS > 1) {
       // w_r = W^{r k}: the log2(R) powers W^{2^i k} come from the table, the
       // others are products over the set bits of r (at most log2(R)-1
       // multiplies deep: a few float32 ulp)
