@@ -686,7 +686,11 @@ def main():
         sys.exit(2)
     __graft_entry__.load_package()
     import torch
-    from lddecode_amd import tune
+    from lddecode_amd import capture, tune
+    # the one-capture pipeline's stream pair first, as an application that
+    # decodes captures one at a time makes it: streams made after the batch
+    # decoders' can share a hardware queue and then run one after the other
+    capture._pipeline_streams(torch, 0)
     tuned = None
     if "auto" in (args.blocklen, args.fft):
         lengths = tune.LENGTHS if args.blocklen == "auto" else (int(args.blocklen),)
