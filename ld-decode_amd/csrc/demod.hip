@@ -851,11 +851,14 @@ __device__ __forceinline__ void demod_block(const DemodArgs& a, int64_t s0, int 
       const float hz_per_rad = a.fs / kTwoPi;
       static_for<0, C::P>([&](auto mc) {
         constexpr int m = decltype(mc)::value;
-        const int j = tidv + C::T * m;
         float2 e = ev[m], od = v[m];
         float2 prev = own.template ld<POff<C::T * m>::value>();
         float2 p = cmulc(e, prev);
-        float h0 = (j > 0) ? inc_to_hz(-disc_atan2(p.y, p.x), hz_per_rad, a.fs) : 0.0f;
+        // sample 0 has no predecessor: 0 (only element 0 of thread 0; the
+        // test on the plain thread index, as tidv is opaque to the compiler
+        // and would cost a branch per element)
+        float h0 = inc_to_hz(-disc_atan2(p.y, p.x), hz_per_rad, a.fs);
+        if (m == 0 && tid == 0) h0 = 0.0f;
         float2 q = cmulc(od, e);
         float h1 = inc_to_hz(-disc_atan2(q.y, q.x), hz_per_rad, a.fs);
         v[m] = make_float2(h0, h1);

@@ -183,10 +183,14 @@ __global__ __launch_bounds__(kEdgeThreads) void k_edge_write(const uint16_t* __r
   }
 }
 
-// The same compaction from in-pulse bits, 64 samples per thread: a tile
-// covers kBitTile samples.  Word w holds samples 32 w .. 32 w + 31 paired:
-// bit j < 16 is sample 32 w + 2 j, bit 16 + j sample 32 w + 2 j + 1.
-constexpr int kBitTile = kEdgeThreads * 64;
+// The same compaction from in-pulse bits, kBitGroups runs of 64 samples
+// per thread (consecutive): a tile covers kBitTile samples.  Four runs per
+// thread keep a wave's tiles few (about 2,000 per system for 1.3e8 samples),
+// so the count and write launches are not bound by dispatching workgroups
+// that have 2 KB to read each.  Word w holds samples 32 w .. 32 w + 31
+// paired: bit j < 16 is sample 32 w + 2 j, bit 16 + j sample 32 w + 2 j + 1.
+constexpr int kBitGroups = 4;
+constexpr int kBitTile = kEdgeThreads * 64 * kBitGroups;
 
 __device__ __forceinline__ uint32_t spread16(uint32_t x) {
   x = (x | (x << 8)) & 0x00ff00ffu;
@@ -214,8 +218,13 @@ __device__ __forceinline__ void bits_count_tile(const uint32_t* __restrict__ bit
                                                 long long* __restrict__ counts, long long tile) {
   __shared__ int part[kEdgeThreads / 64];
   const int t = threadIdx.x;
-  const long long base = tile * kBitTile + (long long)t * 64;
-  int c = base < n ? __popcll(bit_edges(bits, n, base)) : 0;
+  const long long base = tile * kBitTile + (long long)t * (64 * kBitGroups);
+  int c = 0;
+#pragma unroll
+  for (int g = 0; g < kBitGroups; ++g) {
+    const long long bg = base + 64 * g;
+    if (bg < n) c += __popcll(bit_edges(bits, n, bg));
+  }
   for (int o = 32; o > 0; o >>= 1) c += __shfl_xor(c, o);
   if ((t & 63) == 0) part[t >> 6] = c;
   __syncthreads();
@@ -231,9 +240,15 @@ __device__ __forceinline__ void bits_write_tile(const uint32_t* __restrict__ bit
                                                 long long cap, long long tile) {
   __shared__ int part[kEdgeThreads];
   const int t = threadIdx.x;
-  const long long base = tile * kBitTile + (long long)t * 64;
-  unsigned long long e = base < n ? bit_edges(bits, n, base) : 0ull;
-  const int c = __popcll(e);
+  const long long base = tile * kBitTile + (long long)t * (64 * kBitGroups);
+  unsigned long long e[kBitGroups];
+  int c = 0;
+#pragma unroll
+  for (int g = 0; g < kBitGroups; ++g) {
+    const long long bg = base + 64 * g;
+    e[g] = bg < n ? bit_edges(bits, n, bg) : 0ull;
+    c += __popcll(e[g]);
+  }
   part[t] = c;
   __syncthreads();
   for (int o = 1; o < kEdgeThreads; o <<= 1) {
@@ -243,11 +258,15 @@ __device__ __forceinline__ void bits_write_tile(const uint32_t* __restrict__ bit
     __syncthreads();
   }
   long long k = offsets[tile] + part[t] - c;
-  while (e) {
-    const int i = __ffsll((long long)e) - 1;
-    e &= e - 1;
-    if (k < cap) edges[k] = base + i;
-    ++k;
+#pragma unroll
+  for (int g = 0; g < kBitGroups; ++g) {
+    unsigned long long eg = e[g];
+    while (eg) {
+      const int i = __ffsll((long long)eg) - 1;
+      eg &= eg - 1;
+      if (k < cap) edges[k] = base + 64 * g + i;
+      ++k;
+    }
   }
 }
 
